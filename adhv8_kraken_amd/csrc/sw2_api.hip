@@ -1,0 +1,733 @@
+// C ABI of the SW2 Newton-step library (include/adh_sw2_hip.h): model
+// set-up on the host (geometry, node/element adjacency, the reference's
+// insertion-ordered sparsity pattern, assembly slot lists), device memory
+// management, and the host side of fe_newton (src/fe/fe_newton.c:214-912),
+// which only reads back three norms per Newton iteration and the BiCGSTAB
+// flag every opts.check_every iterations.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "sw2_ctx.h"
+
+using namespace sw2;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(ADH_SW2_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct adh_sw2_hip {
+    Ctx c;
+    std::vector<void *> allocs;
+};
+
+extern "C" const char *adh_sw2_hip_last_error(void) { return g_err.c_str(); }
+
+extern "C" void adh_sw2_hip_default_opts(adh_sw2_opts *o) {
+    o->tol_nonlin = 1e-8;
+    o->inc_nonlin = 1e-14;
+    o->max_nonlin_it = 20;
+    o->max_lin_it = 500;
+    o->max_cuts = 5;
+    o->lin_rtol = 1e-5;
+    o->lin_atol = 1e-10;
+    o->precision = 0;
+    o->preconditioner = 2;
+    o->check_every = 8;
+}
+
+static crlog::tables g_host_tables;
+static bool g_host_tables_ready = false;
+
+extern "C" double adh_sw2_hip_cr_log(double x) {
+    if (!g_host_tables_ready) {
+        crlog::make_tables(&g_host_tables);
+        g_host_tables_ready = true;
+    }
+    return crlog::cr_log_t(x, &g_host_tables);
+}
+
+template <typename T>
+static int dalloc(adh_sw2_hip *h, T **p, size_t n) {
+    void *q = nullptr;
+    size_t bytes = sizeof(T) * (n ? n : 1);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(ADH_SW2_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+static int dupload(adh_sw2_hip *h, T **p, const T *src, size_t n) {
+    int rc = dalloc(h, p, n);
+    if (rc) return rc;
+    if (n && src) HIPCHK(hipMemcpy(*p, src, sizeof(T) * n, hipMemcpyHostToDevice));
+    else if (n) HIPCHK(hipMemset(*p, 0, sizeof(T) * n));
+    return 0;
+}
+
+// Triangle djac and shape gradients, src/elem/element_djac_nrml_gradPhi.c:193-274.
+static void tri_geometry(double x1, double x2, double x3, double y1, double y2, double y3, double *djac, double *g) {
+    double s1x = x2 - x1, s1y = y2 - y1;
+    double s2x = x3 - x1, s2y = y3 - y1;
+    double dj = (s1x * s2y) - (s2x * s1y);
+    if (fabs(dj) > SMALL) {
+        double inv = 1. / dj;
+        g[0] = (s1y - s2y) * inv;
+        g[1] = (s2x - s1x) * inv;
+        g[2] = s2y * inv;
+        g[3] = -s2x * inv;
+        g[4] = -s1y * inv;
+        g[5] = s1x * inv;
+    } else {
+        for (int k = 0; k < 6; k++) g[k] = -9999999.9;
+    }
+    *djac = dj * (1.0 / 2.0);
+}
+
+// Edge length and outward normal, element_djac_nrml_gradPhi.c:24-36 and
+// src/structs/selem_1d.c:118-160.
+static void edge_geometry(const adh_sw2_mesh *m, int ie, double *djac, double *nrml) {
+    int n1 = m->e1d[2 * ie], n2 = m->e1d[2 * ie + 1];
+    double x1 = m->x[n1], x2 = m->x[n2], y1 = m->y[n1], y2 = m->y[n2];
+    *djac = sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
+    double delx = x2 - x1, dely = y2 - y1;
+    double dist = sqrt(delx * delx + dely * dely);
+    nrml[0] = dely / dist;
+    nrml[1] = -delx / dist;
+    const int *t = m->e2d + 3 * m->e1d_adj[ie];
+    double cx = one_3 * (m->x[t[0]] + m->x[t[1]] + m->x[t[2]]);
+    double cy = one_3 * (m->y[t[0]] + m->y[t[1]] + m->y[t[2]]);
+    delx = cx - x1;
+    dely = cy - y1;
+    dist = delx * nrml[0] + dely * nrml[1];
+    if (dist > 0.) {
+        nrml[0] *= -1.;
+        nrml[1] *= -1.;
+    }
+}
+
+static int validate(const adh_sw2_mesh *m) {
+    if (!m || m->nnodes <= 0 || m->nelems2d <= 0 || m->nmat <= 0 || m->nelems1d < 0)
+        return fail(ADH_SW2_EINVAL, "mesh sizes must be positive");
+    if (!m->x || !m->y || !m->z || !m->e2d || !m->e2d_mat || !m->e2d_ks || !m->mat_turb || !m->mat_ev ||
+        !m->mat_cor_speed)
+        return fail(ADH_SW2_EINVAL, "missing required mesh array");
+    if (m->nelems1d && (!m->e1d || !m->e1d_bc || !m->e1d_val || (!m->e1d_adj && !(m->e1d_djac && m->e1d_nrml))))
+        return fail(ADH_SW2_EINVAL, "missing boundary-edge array");
+    for (int i = 0; i < 3 * m->nelems2d; i++)
+        if (m->e2d[i] < 0 || m->e2d[i] >= m->nnodes) return fail(ADH_SW2_EINVAL, "triangle node id out of range");
+    for (int i = 0; i < m->nelems2d; i++)
+        if (m->e2d_mat[i] < 0 || m->e2d_mat[i] >= m->nmat) return fail(ADH_SW2_EINVAL, "element material out of range");
+    for (int i = 0; i < 2 * m->nelems1d; i++)
+        if (m->e1d[i] < 0 || m->e1d[i] >= m->nnodes) return fail(ADH_SW2_EINVAL, "edge node id out of range");
+    for (int i = 0; i < m->nelems1d; i++) {
+        int b = m->e1d_bc[i];
+        if (b != BCT_VEL_NEU && b != BCT_PRS_NEU && b != BCT_OUTFLOW)
+            return fail(ADH_SW2_EUNSUPPORTED, "boundary edge type not supported (NB VEL/OTW/OUT only)");
+        if (m->e1d_adj && (m->e1d_adj[i] < 0 || m->e1d_adj[i] >= m->nelems2d))
+            return fail(ADH_SW2_EINVAL, "edge adjacent element out of range");
+    }
+    for (int i = 0; i < m->nmat; i++) {
+        int t = m->mat_turb[i];
+        if (t != 0 && t != TURB_EVS && t != TURB_EEV) return fail(ADH_SW2_EINVAL, "bad turbulence closure");
+        if (t == TURB_EEV && (!m->mat_eev_mode || m->mat_eev_mode[i] < 1 || m->mat_eev_mode[i] > 3))
+            return fail(ADH_SW2_EINVAL, "EEV mode must be 1, 2 or 3");
+    }
+    if (m->node_bc) {
+        for (int i = 0; i < m->nnodes; i++) {
+            int b = m->node_bc[i];
+            if (b != 0 && b != BCT_PRS_DIR && b != BCT_VEL_DIR && b != BCT_VEL_PRS_DIR)
+                return fail(ADH_SW2_EUNSUPPORTED, "node string type not supported (Dirichlet only)");
+            if (b && !m->node_bc_val) return fail(ADH_SW2_EINVAL, "Dirichlet node without values");
+        }
+    }
+    return 0;
+}
+
+static int alloc_solver(adh_sw2_hip *h) {
+    Ctx &c = h->c;
+    int rc;
+    size_t n3 = 3 * (size_t)c.N;
+    if ((rc = dalloc(h, &c.kb, n3)) || (rc = dalloc(h, &c.kr, n3)) || (rc = dalloc(h, &c.kp, n3)) ||
+        (rc = dalloc(h, &c.kAp, n3)) || (rc = dalloc(h, &c.ks_, n3)) || (rc = dalloc(h, &c.kAs, n3)) ||
+        (rc = dalloc(h, &c.kq, n3)) || (rc = dalloc(h, &c.ku, n3)))
+        return rc;
+    if ((rc = dalloc(h, &c.fb, n3)) || (rc = dalloc(h, &c.fr, n3)) || (rc = dalloc(h, &c.fp, n3)) ||
+        (rc = dalloc(h, &c.fAp, n3)) || (rc = dalloc(h, &c.fs, n3)) || (rc = dalloc(h, &c.fAs, n3)) ||
+        (rc = dalloc(h, &c.fq, n3)) || (rc = dalloc(h, &c.fu, n3)))
+        return rc;
+    if ((rc = dalloc(h, &c.diag32, 9 * (size_t)c.N)) || (rc = dalloc(h, &c.offd32, 9 * (size_t)c.nnz)) ||
+        (rc = dalloc(h, &c.dinv32, 9 * (size_t)c.N)))
+        return rc;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_hip **out) {
+    if (!out) return fail(ADH_SW2_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = validate(m);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(ADH_SW2_EHIP, "no HIP device available: the SW2 library runs on the GPU only");
+    adh_sw2_hip *h = new adh_sw2_hip();  // value-initialised: all device pointers start null
+    Ctx &c = h->c;
+    c.N = m->nnodes; c.E = m->nelems2d; c.B = m->nelems1d; c.M = m->nmat;
+    if (opts) c.opts = *opts;
+    else adh_sw2_hip_default_opts(&c.opts);
+    if (c.opts.check_every < 1) c.opts.check_every = 1;
+    c.dt = m->dt;
+    const int N = c.N, E = c.E, B = c.B;
+
+    // ---- geometry (host)
+    std::vector<double> djac(E), grad(6 * (size_t)E), djac1d(B), nrml1d(2 * (size_t)B);
+    if (m->e2d_djac && m->e2d_grad) {
+        memcpy(djac.data(), m->e2d_djac, sizeof(double) * E);
+        memcpy(grad.data(), m->e2d_grad, sizeof(double) * 6 * E);
+    } else {
+        for (int i = 0; i < E; i++) {
+            const int *t = m->e2d + 3 * i;
+            tri_geometry(m->x[t[0]], m->x[t[1]], m->x[t[2]], m->y[t[0]], m->y[t[1]], m->y[t[2]], &djac[i],
+                         &grad[6 * (size_t)i]);
+        }
+    }
+    for (int i = 0; i < B; i++) {
+        if (m->e1d_djac && m->e1d_nrml) {
+            djac1d[i] = m->e1d_djac[i];
+            nrml1d[2 * i] = m->e1d_nrml[2 * i];
+            nrml1d[2 * i + 1] = m->e1d_nrml[2 * i + 1];
+        } else {
+            edge_geometry(m, i, &djac1d[i], &nrml1d[2 * (size_t)i]);
+        }
+    }
+
+    // ---- adjacency (ascending element order)
+    std::vector<int> n2e_ptr(N + 1, 0), n2b_ptr(N + 1, 0);
+    for (int i = 0; i < 3 * E; i++) n2e_ptr[m->e2d[i] + 1]++;
+    for (int i = 0; i < 2 * B; i++) n2b_ptr[m->e1d[i] + 1]++;
+    for (int n = 0; n < N; n++) { n2e_ptr[n + 1] += n2e_ptr[n]; n2b_ptr[n + 1] += n2b_ptr[n]; }
+    std::vector<int> n2e(3 * (size_t)E), n2b(2 * (size_t)B), fill(N);
+    for (int n = 0; n < N; n++) fill[n] = n2e_ptr[n];
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < 3; i++) n2e[fill[m->e2d[3 * e + i]]++] = 3 * e + i;
+    for (int n = 0; n < N; n++) fill[n] = n2b_ptr[n];
+    for (int e = 0; e < B; e++)
+        for (int i = 0; i < 2; i++) n2b[fill[m->e1d[2 * e + i]]++] = 2 * e + i;
+    std::vector<int> first_elem(N, -1);
+    for (int n = 0; n < N; n++)
+        if (n2e_ptr[n + 1] > n2e_ptr[n]) first_elem[n] = n2e[n2e_ptr[n]] / 3;
+
+    // ---- sparsity pattern in spv_load insertion order (2D elements, then 1D)
+    std::vector<std::vector<int>> rows(N);
+    auto insert = [&](int a, int b) {
+        if (a == b) return;
+        std::vector<int> &r = rows[a];
+        for (int v : r) if (v == b) return;
+        r.push_back(b);
+    };
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) insert(m->e2d[3 * e + i], m->e2d[3 * e + j]);
+    for (int e = 0; e < B; e++)
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) insert(m->e1d[2 * e + i], m->e1d[2 * e + j]);
+    c.h_row_ptr.assign(N + 1, 0);
+    for (int n = 0; n < N; n++) c.h_row_ptr[n + 1] = c.h_row_ptr[n] + (int)rows[n].size();
+    c.nnz = c.h_row_ptr[N];
+    c.h_col.resize(c.nnz);
+    for (int n = 0; n < N; n++)
+        for (size_t k = 0; k < rows[n].size(); k++) c.h_col[c.h_row_ptr[n] + k] = rows[n][k];
+    // slot of (a, b): diagonal a, or N + position of b in row a
+    auto slot_of = [&](int a, int b) -> int {
+        if (a == b) return a;
+        const std::vector<int> &r = rows[a];
+        for (size_t k = 0; k < r.size(); k++) if (r[k] == b) return N + c.h_row_ptr[a] + (int)k;
+        return -1;
+    };
+    const int nslots = N + c.nnz;
+    std::vector<int> slot_ptr(nslots + 1, 0);
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) slot_ptr[slot_of(m->e2d[3 * e + i], m->e2d[3 * e + j]) + 1]++;
+    for (int e = 0; e < B; e++)
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) slot_ptr[slot_of(m->e1d[2 * e + i], m->e1d[2 * e + j]) + 1]++;
+    for (int s = 0; s < nslots; s++) slot_ptr[s + 1] += slot_ptr[s];
+    std::vector<int> slot_src(slot_ptr[nslots]), sfill(slot_ptr.begin(), slot_ptr.end() - 1);
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                slot_src[sfill[slot_of(m->e2d[3 * e + i], m->e2d[3 * e + j])]++] = 9 * e + 3 * i + j;
+    for (int e = 0; e < B; e++)
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++)
+                slot_src[sfill[slot_of(m->e1d[2 * e + i], m->e1d[2 * e + j])]++] = 9 * E + 4 * e + 2 * i + j;
+    rows.clear();
+    rows.shrink_to_fit();
+
+    for (int k = 0; k < 3; k++) c.loc_bc[k] = (m->node_bc && k < N) ? m->node_bc[k] : 0;
+
+    // ---- device arrays
+    hipError_t he = hipStreamCreate(&c.stream);
+    if (he != hipSuccess) { delete h; return fail(ADH_SW2_EHIP, "hipStreamCreate failed"); }
+    for (int i = 0; i < 8; i++) hipEventCreate(&c.ev[i]);
+    std::vector<int> zeros_i(N, 0);
+    std::vector<double> zeros_d;
+#define UP(dst, src, n)                                    \
+    if ((rc = dupload(h, &c.dst, src, (size_t)(n)))) {     \
+        adh_sw2_hip_destroy(h);                            \
+        return rc;                                         \
+    }
+    UP(x, m->x, N); UP(y, m->y, N); UP(z, m->z, N);
+    UP(head, (const double *)nullptr, N); UP(u, (const double *)nullptr, N); UP(v, (const double *)nullptr, N);
+    UP(old_head, (const double *)nullptr, N); UP(old_u, (const double *)nullptr, N); UP(old_v, (const double *)nullptr, N);
+    UP(older_head, (const double *)nullptr, N); UP(older_u, (const double *)nullptr, N);
+    UP(older_v, (const double *)nullptr, N);
+    UP(node_bc, m->node_bc ? m->node_bc : zeros_i.data(), N);
+    UP(node_bc_val, m->node_bc_val, 3 * (size_t)N);
+    UP(first_elem, first_elem.data(), N);
+    UP(wd_tmp, (const int *)nullptr, N);
+    UP(e2d, m->e2d, 3 * (size_t)E);
+    UP(mat, m->e2d_mat, E);
+    UP(ks, m->e2d_ks, E);
+    std::vector<int> srcf(E, 0);
+    if (m->e2d_src_flag) memcpy(srcf.data(), m->e2d_src_flag, sizeof(int) * E);
+    UP(src_flag, srcf.data(), E);
+    UP(src, m->e2d_src, E);
+    UP(wd_flag, (const int *)nullptr, E);
+    UP(djac, djac.data(), E);
+    UP(grad, grad.data(), 6 * (size_t)E);
+    UP(e1d, m->e1d, 2 * (size_t)B);
+    UP(bc1d, m->e1d_bc, B);
+    UP(val1d, m->e1d_val, B);
+    UP(djac1d, djac1d.data(), B);
+    UP(nrml1d, nrml1d.data(), 2 * (size_t)B);
+    UP(mat_turb, m->mat_turb, c.M);
+    std::vector<int> eevm(c.M, 0);
+    std::vector<double> eevc(c.M, 0.0);
+    if (m->mat_eev_mode) memcpy(eevm.data(), m->mat_eev_mode, sizeof(int) * c.M);
+    if (m->mat_eev_coef) memcpy(eevc.data(), m->mat_eev_coef, sizeof(double) * c.M);
+    UP(mat_eev_mode, eevm.data(), c.M);
+    UP(mat_eev_coef, eevc.data(), c.M);
+    UP(mat_ev, m->mat_ev, 3 * (size_t)c.M);
+    UP(mat_cor, m->mat_cor_speed, c.M);
+    {
+        crlog::tables T;
+        crlog::make_tables(&T);
+        UP(logt, &T, 1);
+    }
+    UP(n2e_ptr, n2e_ptr.data(), N + 1);
+    UP(n2e, n2e.data(), n2e.size());
+    UP(n2b_ptr, n2b_ptr.data(), N + 1);
+    UP(n2b, n2b.data(), n2b.size());
+    UP(row_ptr, c.h_row_ptr.data(), N + 1);
+    UP(col, c.h_col.data(), c.nnz);
+    UP(slot_ptr, slot_ptr.data(), nslots + 1);
+    UP(slot_src, slot_src.data(), slot_src.size());
+    UP(ER, (const double *)nullptr, 9 * (size_t)E + 6 * (size_t)B);
+    UP(EM, (const double *)nullptr, 81 * (size_t)E + 36 * (size_t)B);
+    UP(R, (const double *)nullptr, 3 * (size_t)N);
+    UP(sol, (const double *)nullptr, 3 * (size_t)N);
+    UP(diag, (const double *)nullptr, 9 * (size_t)N);
+    UP(offd, (const double *)nullptr, 9 * (size_t)c.nnz);
+    UP(scale, (const double *)nullptr, 3 * (size_t)N);
+    UP(dinv, (const double *)nullptr, 9 * (size_t)N);
+    UP(ksc, (const KScal *)nullptr, 1);
+    UP(norms, (const Norms *)nullptr, 1);
+    c.red.maxblocks = nblocks_for(N, 256) + 8;
+    UP(red.part, (const double *)nullptr, 4 * (size_t)c.red.maxblocks);
+    UP(red.count, (const unsigned *)nullptr, 8);
+#undef UP
+    if ((rc = alloc_solver(h))) { adh_sw2_hip_destroy(h); return rc; }
+    if (hipHostMalloc((void **)&c.h_ksc, sizeof(KScal)) != hipSuccess ||
+        hipHostMalloc((void **)&c.h_norms, sizeof(Norms)) != hipSuccess ||
+        hipHostMalloc((void **)&c.h_flag, 64) != hipSuccess) {
+        adh_sw2_hip_destroy(h);
+        return fail(ADH_SW2_ENOMEM, "hipHostMalloc failed");
+    }
+    {   // scale_vect starts at 1.0 (fe_main.c:263)
+        std::vector<double> ones(3 * (size_t)N, 1.0);
+        HIPCHK(hipMemcpy(c.scale, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
+    }
+    ElemData &D = c.ed;
+    D.x = c.x; D.y = c.y; D.z = c.z;
+    D.head = c.head; D.u = c.u; D.v = c.v;
+    D.old_head = c.old_head; D.old_u = c.old_u; D.old_v = c.old_v;
+    D.older_head = c.older_head; D.older_u = c.older_u; D.older_v = c.older_v;
+    D.node_bc = c.node_bc; D.first_elem = c.first_elem;
+    D.e2d = c.e2d; D.djac = c.djac; D.grad = c.grad; D.mat = c.mat; D.ks = c.ks; D.src_flag = c.src_flag;
+    D.src = c.src; D.wd_flag = c.wd_flag;
+    D.e1d = c.e1d; D.djac1d = c.djac1d; D.nrml1d = c.nrml1d; D.bc1d = c.bc1d; D.val1d = c.val1d;
+    D.mat_turb = c.mat_turb; D.mat_eev_mode = c.mat_eev_mode; D.mat_eev_coef = c.mat_eev_coef;
+    D.mat_ev = c.mat_ev; D.mat_cor = c.mat_cor;
+    D.g = m->gravity; D.dt = m->dt; D.alpha = m->tau_temporal; D.tau_pg = m->tau_pg;
+    D.dll = m->drying_lower_limit; D.dul = m->drying_upper_limit; D.visc = m->viscosity;
+    D.logt = c.logt;
+    HIPCHK(hipDeviceSynchronize());
+    *out = h;
+    return ADH_SW2_OK;
+}
+
+extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
+    if (!h) return;
+    Ctx &c = h->c;
+    if (c.stream) hipStreamSynchronize(c.stream);
+    for (void *p : h->allocs) hipFree(p);
+    if (c.h_ksc) hipHostFree(c.h_ksc);
+    if (c.h_norms) hipHostFree(c.h_norms);
+    if (c.h_flag) hipHostFree(c.h_flag);
+    for (int i = 0; i < 8; i++) if (c.ev[i]) hipEventDestroy(c.ev[i]);
+    if (c.stream) hipStreamDestroy(c.stream);
+    delete h;
+}
+
+extern "C" int adh_sw2_hip_set_opts(adh_sw2_hip *h, const adh_sw2_opts *o) {
+    if (!h || !o) return fail(ADH_SW2_EINVAL, "null argument");
+    if (o->precision != 0 && o->precision != 1) return fail(ADH_SW2_EINVAL, "precision must be 0 or 1");
+    if (o->preconditioner != 2) return fail(ADH_SW2_EUNSUPPORTED, "preconditioner must be 2 (block Jacobi)");
+    h->c.opts = *o;
+    if (h->c.opts.check_every < 1) h->c.opts.check_every = 1;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_set_dt(adh_sw2_hip *h, double dt) {
+    if (!h || !(dt > 0)) return fail(ADH_SW2_EINVAL, "dt must be positive");
+    h->c.dt = dt;
+    h->c.ed.dt = dt;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_geometry(adh_sw2_hip *h, double *djac, double *grad, double *djac1d, double *nrml1d) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    if (djac) HIPCHK(hipMemcpy(djac, c.djac, sizeof(double) * c.E, hipMemcpyDeviceToHost));
+    if (grad) HIPCHK(hipMemcpy(grad, c.grad, sizeof(double) * 6 * c.E, hipMemcpyDeviceToHost));
+    if (djac1d && c.B) HIPCHK(hipMemcpy(djac1d, c.djac1d, sizeof(double) * c.B, hipMemcpyDeviceToHost));
+    if (nrml1d && c.B) HIPCHK(hipMemcpy(nrml1d, c.nrml1d, sizeof(double) * 2 * c.B, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_pattern_nnz(adh_sw2_hip *h) { return h ? h->c.nnz : ADH_SW2_EINVAL; }
+
+extern "C" int adh_sw2_hip_pattern(adh_sw2_hip *h, int *row_ptr, int *col) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    if (row_ptr) memcpy(row_ptr, h->c.h_row_ptr.data(), sizeof(int) * (h->c.N + 1));
+    if (col && h->c.nnz) memcpy(col, h->c.h_col.data(), sizeof(int) * h->c.nnz);
+    return 0;
+}
+
+static int set_pair(Ctx &c, double *dh, double *du, double *dv, const double *hh, const double *vel) {
+    std::vector<double> u(c.N), v(c.N);
+    for (int i = 0; i < c.N; i++) { u[i] = vel[2 * i]; v[i] = vel[2 * i + 1]; }
+    HIPCHK(hipMemcpy(dh, hh, sizeof(double) * c.N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(du, u.data(), sizeof(double) * c.N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dv, v.data(), sizeof(double) * c.N, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int get_pair(Ctx &c, const double *dh, const double *du, const double *dv, double *hh, double *vel) {
+    if (hh) HIPCHK(hipMemcpy(hh, dh, sizeof(double) * c.N, hipMemcpyDeviceToHost));
+    if (vel) {
+        std::vector<double> u(c.N), v(c.N);
+        HIPCHK(hipMemcpy(u.data(), du, sizeof(double) * c.N, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v.data(), dv, sizeof(double) * c.N, hipMemcpyDeviceToHost));
+        for (int i = 0; i < c.N; i++) { vel[2 * i] = u[i]; vel[2 * i + 1] = v[i]; }
+    }
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_set_state(adh_sw2_hip *h, const double *head, const double *vel, const double *old_head,
+                                     const double *old_vel, const double *older_head, const double *older_vel) {
+    if (!h || !head || !vel || !old_head || !old_vel || !older_head || !older_vel)
+        return fail(ADH_SW2_EINVAL, "null argument");
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    int rc;
+    if ((rc = set_pair(c, c.head, c.u, c.v, head, vel))) return rc;
+    if ((rc = set_pair(c, c.old_head, c.old_u, c.old_v, old_head, old_vel))) return rc;
+    if ((rc = set_pair(c, c.older_head, c.older_u, c.older_v, older_head, older_vel))) return rc;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_get_state(adh_sw2_hip *h, double *head, double *vel, double *old_head, double *old_vel,
+                                     double *older_head, double *older_vel) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    int rc;
+    if ((rc = get_pair(c, c.head, c.u, c.v, head, vel))) return rc;
+    if ((rc = get_pair(c, c.old_head, c.old_u, c.old_v, old_head, old_vel))) return rc;
+    if ((rc = get_pair(c, c.older_head, c.older_u, c.older_v, older_head, older_vel))) return rc;
+    return 0;
+}
+
+static int sync_check(Ctx &c) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_shift(adh_sw2_hip *h) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    launch_shift(&h->c);
+    return sync_check(h->c);
+}
+
+extern "C" int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    int *dmask = nullptr;
+    if (bc_mask) HIPCHK(hipMalloc(&dmask, sizeof(int) * 3 * c.N));
+    launch_init(&c, dmask);
+    int rc = sync_check(c);
+    if (!rc && bc_mask) rc = hipMemcpy(bc_mask, dmask, sizeof(int) * 3 * c.N, hipMemcpyDeviceToHost) == hipSuccess ? 0 : fail(ADH_SW2_EHIP, "copy bc_mask");
+    if (dmask) hipFree(dmask);
+    if (!rc && wd_flag) HIPCHK(hipMemcpy(wd_flag, c.wd_flag, sizeof(int) * c.E, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+extern "C" int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    launch_resid(&c);
+    int rc = sync_check(c);
+    if (rc) return rc;
+    if (residual) HIPCHK(hipMemcpy(residual, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdiag, double *residual) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    launch_jacobian(&c);
+    launch_assemble(&c);
+    int rc = sync_check(c);
+    if (rc) return rc;
+    if (diagonal) HIPCHK(hipMemcpy(diagonal, c.diag, sizeof(double) * 9 * c.N, hipMemcpyDeviceToHost));
+    if (offdiag && c.nnz) HIPCHK(hipMemcpy(offdiag, c.offd, sizeof(double) * 9 * c.nnz, hipMemcpyDeviceToHost));
+    if (residual) HIPCHK(hipMemcpy(residual, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *em1d) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (em2d) HIPCHK(hipMemcpy(em2d, c.EM, sizeof(double) * 81 * c.E, hipMemcpyDeviceToHost));
+    if (em1d && c.B) HIPCHK(hipMemcpy(em1d, c.EM + 81 * (size_t)c.E, sizeof(double) * 36 * c.B, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    if (sol) HIPCHK(hipMemcpy(c.sol, sol, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice));
+    launch_inc(&c, 1.0);
+    return sync_check(c);
+}
+
+extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    launch_setup(&c);
+    int rc = sync_check(c);
+    if (rc) return rc;
+    if (scale_vect) HIPCHK(hipMemcpy(scale_vect, c.scale, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// BiCGSTAB on the scaled device system; returns the reference's YES/NO.
+static int krylov(Ctx &c, int *lin_it, int *verdict) {
+    KScal init{};
+    init.atol = c.opts.lin_atol;
+    init.rtol = c.opts.lin_rtol;
+    init.max_it = c.opts.max_lin_it;
+    HIPCHK(hipMemcpyAsync(c.ksc, &init, sizeof(KScal), hipMemcpyHostToDevice, c.stream));
+    launch_krylov_prologue(&c);
+    for (;;) {
+        HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        if (c.h_ksc->done) break;
+        int left = c.opts.max_lin_it - c.h_ksc->it;
+        int k = c.opts.check_every < left ? c.opts.check_every : left;
+        if (k < 1) k = 1;
+        launch_krylov_iters(&c, k);
+        HIPCHK(hipGetLastError());
+    }
+    launch_krylov_epilogue(&c);
+    HIPCHK(hipGetLastError());
+    if (lin_it) *lin_it = c.h_ksc->it;
+    *verdict = (c.h_ksc->it == c.opts.max_lin_it) ? ADH_SW2_NO : ADH_SW2_YES;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *lin_it, int *verdict) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    int vd = ADH_SW2_NO;
+    int rc = krylov(c, lin_it, &vd);
+    if (rc) return rc;
+    if ((rc = sync_check(c))) return rc;
+    if (verdict) *verdict = vd;
+    if (sol) HIPCHK(hipMemcpy(sol, c.sol, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, const double *offdiag, const double *rhs) {
+    if (!h || !diagonal || !rhs || (!offdiag && h->c.nnz)) return fail(ADH_SW2_EINVAL, "null argument");
+    Ctx &c = h->c;
+    HIPCHK(hipMemcpy(c.diag, diagonal, sizeof(double) * 9 * c.N, hipMemcpyHostToDevice));
+    if (c.nnz) HIPCHK(hipMemcpy(c.offd, offdiag, sizeof(double) * 9 * c.nnz, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.R, rhs, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.N));
+    return 0;
+}
+
+static float elapsed(Ctx &c, int a, int b) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c.ev[a], c.ev[b]);
+    return ms;
+}
+
+static int read_norms(Ctx &c, Norms *out) {
+    HIPCHK(hipMemcpyAsync(c.h_norms, c.norms, sizeof(Norms), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    *out = *c.h_norms;
+    return 0;
+}
+
+static void hist(adh_sw2_report *r, const Norms &n, int lin_it) {
+    if (!r) return;
+    if (r->nhist < r->hist_cap) {
+        if (r->hist_max) r->hist_max[r->nhist] = n.maxn;
+        if (r->hist_l2) r->hist_l2[r->nhist] = n.l2;
+        if (r->hist_inc) r->hist_inc[r->nhist] = n.incn;
+        if (r->hist_lin_it) r->hist_lin_it[r->nhist] = lin_it;
+    }
+    r->nhist++;
+}
+
+// fe_newton (src/fe/fe_newton.c:214-912), serial non-PETSc branch with the
+// SW2 callbacks.
+static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
+    const adh_sw2_opts &o = c.opts;
+    int rc;
+    Norms nm;
+    hipEventRecord(c.ev[0], c.stream);
+    launch_init(&c, nullptr);
+    hipEventRecord(c.ev[1], c.stream);
+    launch_resid(&c);
+    launch_norms(&c);
+    hipEventRecord(c.ev[2], c.stream);
+    if ((rc = read_norms(c, &nm))) return rc;
+    if (rep) rep->ms_resid += elapsed(c, 1, 2);
+    hist(rep, nm, 0);
+    int it = 0, solv_flag = ADH_SW2_YES, keep = 0;
+    do {
+        int lin_it = 0, vd = ADH_SW2_YES;
+        hipEventRecord(c.ev[3], c.stream);
+        launch_jacobian(&c);
+        launch_assemble(&c);
+        launch_zero(&c, c.sol, 3 * (size_t)c.N);
+        hipEventRecord(c.ev[4], c.stream);
+        launch_setup(&c);
+        hipEventRecord(c.ev[5], c.stream);
+        if ((rc = krylov(c, &lin_it, &vd))) return rc;
+        hipEventRecord(c.ev[6], c.stream);
+        solv_flag = vd;
+        launch_inc(&c, 1.0);
+        launch_scale_sol(&c, -1.0);
+        double old_l2 = nm.l2;
+        launch_resid(&c);
+        launch_norms(&c);
+        hipEventRecord(c.ev[7], c.stream);
+        if ((rc = read_norms(c, &nm))) return rc;
+        if (rep) {
+            rep->ms_load += elapsed(c, 3, 4);
+            rep->ms_setup += elapsed(c, 4, 5);
+            rep->ms_solve += elapsed(c, 5, 6);
+            rep->ms_resid += elapsed(c, 6, 7);
+            rep->lin_iterations += lin_it;
+        }
+        if (isnan(nm.l2) || isinf(nm.l2)) solv_flag = ADH_SW2_NO;
+        int cuts = 0;
+        while (nm.maxn > o.tol_nonlin && nm.l2 > old_l2 && cuts < o.max_cuts) {
+            launch_scale_sol(&c, 0.5);
+            launch_inc(&c, 1.0);
+            launch_resid(&c);
+            launch_norms(&c);
+            if ((rc = read_norms(c, &nm))) return rc;
+            cuts++;
+        }
+        it++;
+        hist(rep, nm, lin_it);
+        keep = (solv_flag == ADH_SW2_YES) && (it < o.max_nonlin_it) && (nm.maxn < 1.0E+20) &&
+               (nm.maxn > o.tol_nonlin && nm.incn > o.inc_nonlin);
+    } while (keep);
+    bool failed = (solv_flag != ADH_SW2_YES) || ((nm.maxn > o.tol_nonlin) && (nm.incn > o.inc_nonlin)) ||
+                  (nm.maxn >= 1.0E+20);
+    if (rep) {
+        rep->iterations += it;
+        rep->converged = !failed;
+        rep->solv_fail = solv_flag != ADH_SW2_YES;
+        rep->ms_total += elapsed(c, 0, 7);
+    }
+    if (failed) {
+        launch_init(&c, nullptr);  // fe_newton.c:883
+        if ((rc = sync_check(c))) return rc;
+        *verdict = ADH_SW2_NO;
+        return 0;
+    }
+    *verdict = ADH_SW2_YES;
+    return sync_check(c);
+}
+
+extern "C" int adh_sw2_hip_newton(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict) {
+    if (!h || !verdict) return fail(ADH_SW2_EINVAL, "null argument");
+    if (rep) {
+        rep->iterations = rep->lin_iterations = rep->nhist = 0;
+        rep->ms_load = rep->ms_setup = rep->ms_solve = rep->ms_resid = rep->ms_total = 0;
+    }
+    return newton(h->c, rep, verdict);
+}
+
+extern "C" int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int max_retries) {
+    if (!h || !verdict) return fail(ADH_SW2_EINVAL, "null argument");
+    if (rep) {
+        rep->iterations = rep->lin_iterations = rep->nhist = 0;
+        rep->ms_load = rep->ms_setup = rep->ms_solve = rep->ms_resid = rep->ms_total = 0;
+    }
+    for (int attempt = 0; attempt <= max_retries; attempt++) {
+        launch_shift(&h->c);   // fe_main calls fe_sw2 on every attempt (fe_main.c:348-353)
+        int rc = newton(h->c, rep, verdict);
+        if (rc) return rc;
+        if (*verdict == ADH_SW2_YES) return 0;
+        double dt = h->c.dt * 0.25;   // tc_scale, DT_REDUCE_FACTOR (define.h:123)
+        if (dt <= SMALL) return fail(ADH_SW2_EINVAL, "time step reduced below DBL_EPSILON");
+        adh_sw2_hip_set_dt(h, dt);
+    }
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_timestep_batch(adh_sw2_hip **hs, int n, adh_sw2_report *reps, int *verdicts) {
+    if (!hs || n < 0 || !verdicts) return fail(ADH_SW2_EINVAL, "null argument");
+    for (int i = 0; i < n; i++) {
+        int rc = adh_sw2_hip_timestep(hs[i], reps ? reps + i : nullptr, verdicts + i, 0);
+        if (rc) return rc;
+    }
+    return 0;
+}

@@ -1,0 +1,103 @@
+// Internal state of one device-resident SW2 model and the launch helpers
+// shared by the kernel and API translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+
+#include "../../include/adh_sw2_hip.h"
+#include "sw2_elem.h"
+
+namespace sw2 {
+
+// Device-side scalars of one BiCGSTAB solve (solv_bcgstab.c:60-70).
+struct KScal {
+    double rho, rhop, alpha, omega, beta, gamma;
+    double rnorm, bnorm, tol;
+    double atol, rtol;
+    int it, max_it, done, pad;
+};
+
+// Reduction slots: per-block partials and completion counters.
+struct Red {
+    double *part;      // [3 * maxblocks]
+    unsigned *count;   // [8] counters, one per reduction site
+    int maxblocks;
+};
+
+// Newton norms (get_residual_norms, fe_newton.c:40-151)
+struct Norms {
+    double maxn, l2, incn, pad;
+};
+
+struct Ctx {
+    int N, E, B, M, nnz;
+    adh_sw2_opts opts;
+    double dt;
+    hipStream_t stream;
+    hipEvent_t ev[8];
+
+    // node fields
+    double *x, *y, *z;
+    double *head, *u, *v, *old_head, *old_u, *old_v, *older_head, *older_u, *older_v;
+    int *node_bc;
+    double *node_bc_val;
+    int *first_elem;
+    int *wd_tmp;
+    int loc_bc[3];   // node-string flags of nodes 0..2 (the reference's local-index read)
+    // elements
+    int *e2d, *mat, *src_flag, *wd_flag;
+    double *djac, *grad, *ks, *src;
+    int *e1d, *bc1d;
+    double *djac1d, *nrml1d, *val1d;
+    int *mat_turb, *mat_eev_mode;
+    double *mat_eev_coef, *mat_ev, *mat_cor;
+    crlog::tables *logt;
+    // adjacency
+    int *n2e_ptr, *n2e;    // node -> (element*3 + local), ascending element
+    int *n2b_ptr, *n2b;    // node -> (edge*2 + local)
+    // pattern + assembly
+    int *row_ptr, *col;    // off-diagonal blocks, insertion order
+    int *slot_ptr, *slot_src;  // N diagonal slots then nnz off-diagonal slots
+    std::vector<int> h_row_ptr, h_col;
+    // work
+    double *ER;            // element residuals [9E + 6B]
+    double *EM;            // element matrices [81E + 36B]
+    double *R;             // residual / rhs [3N]
+    double *sol;           // [3N]
+    double *diag, *offd;   // [9N], [9 nnz]
+    double *scale;         // [3N]
+    double *dinv;          // [9N]
+    // FP32 system for mixed precision
+    float *diag32, *offd32, *dinv32;
+    // Krylov vectors (FP64) and FP32 twins
+    double *kb, *kr, *kp, *kAp, *ks_, *kAs, *kq, *ku;
+    float *fb, *fr, *fp, *fAp, *fs, *fAs, *fq, *fu;
+    KScal *ksc;
+    KScal *h_ksc;          // pinned
+    Red red;
+    Norms *norms, *h_norms;
+    int *h_flag;           // pinned scratch
+    ElemData ed;
+};
+
+// launchers (sw2_kernels.hip / sw2_solver.hip)
+void launch_resid(Ctx *c);
+void launch_jacobian(Ctx *c);
+void launch_assemble(Ctx *c);
+void launch_init(Ctx *c, int *bc_mask_dev);
+void launch_shift(Ctx *c);
+void launch_inc(Ctx *c, double factor_on_sol);   // state += sol
+void launch_scale_sol(Ctx *c, double f);          // sol *= f
+void launch_norms(Ctx *c);
+void launch_setup(Ctx *c);
+void launch_krylov_prologue(Ctx *c);
+void launch_krylov_iters(Ctx *c, int k);
+void launch_krylov_epilogue(Ctx *c);
+void launch_geometry(Ctx *c);
+void launch_zero(Ctx *c, double *p, size_t n);
+
+int nblocks_for(int n, int bs);
+
+}  // namespace sw2

@@ -1,0 +1,347 @@
+// Element-level kernels of the SW2 Newton step on gfx950: residual,
+// finite-difference Jacobian, ordered assembly into 3x3 block rows, and the
+// per-node state kernels (init / wet-dry flags / shift / increment / norms).
+//
+// Layout in HBM: node fields are structure-of-arrays doubles; element
+// connectivity and geometry are element-indexed arrays, so one thread per
+// element (or per element column) issues coalesced loads for everything but
+// the node gathers.  Element results go to contiguous per-element slabs
+// (ER: 9 doubles per triangle, EM: 81 per triangle) and are then gathered
+// per node / per matrix block in ascending element order -- the order in
+// which the reference accumulates (fe_sw2_resid.c:74-191,
+// fe_assemble_matrix.c:166-224) -- so sums are bit-identical to it without
+// atomics.
+#include "sw2_ctx.h"
+
+namespace sw2 {
+
+int nblocks_for(int n, int bs) { return n <= 0 ? 1 : (n + bs - 1) / bs; }
+
+// ------------------------------------------------------------------ residual
+__global__ void __launch_bounds__(128) k_resid2d(ElemData D, int E, double *ER) {
+    int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= E) return;
+    d3 r[3];
+    body_resid(D, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
+    double *o = ER + 9 * (size_t)ie;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        o[3 * i + 0] = r[i].x_eq;
+        o[3 * i + 1] = r[i].y_eq;
+        o[3 * i + 2] = r[i].c_eq;
+    }
+}
+
+__global__ void __launch_bounds__(128) k_resid1d(ElemData D, int B, double *ER1) {
+    int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= B) return;
+    d3 r[3];
+    boundary_resid(D, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
+    double *o = ER1 + 6 * (size_t)ie;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        o[3 * i + 0] = r[i].x_eq;
+        o[3 * i + 1] = r[i].y_eq;
+        o[3 * i + 2] = r[i].c_eq;
+    }
+}
+
+// residual[3n..] = -(sum of element contributions) in ascending element
+// order, 2D first then 1D; dry-node momentum contributions are dropped and
+// dry-node velocities zeroed (fe_sw2_resid.c:85-93, :154-162).
+__global__ void k_resid_gather(int N, const int *n2e_ptr, const int *n2e, const int *n2b_ptr, const int *n2b,
+                               const double *ER, const double *ER1, const double *old_head, double *u, double *v,
+                               double *R) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    bool dry = old_head[n] <= 0.;
+    double rx = 0.0, ry = 0.0, rc = 0.0;
+    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
+        int s = n2e[k];
+        const double *o = ER + 9 * (size_t)(s / 3) + 3 * (s % 3);
+        if (!dry) { rx -= o[0]; ry -= o[1]; }
+        rc -= o[2];
+    }
+    for (int k = n2b_ptr[n]; k < n2b_ptr[n + 1]; k++) {
+        int s = n2b[k];
+        const double *o = ER1 + 6 * (size_t)(s / 2) + 3 * (s % 2);
+        if (!dry) { rx -= o[0]; ry -= o[1]; }
+        rc -= o[2];
+    }
+    R[3 * n] = rx;
+    R[3 * n + 1] = ry;
+    R[3 * n + 2] = rc;
+    if (dry && (n2e_ptr[n + 1] > n2e_ptr[n] || n2b_ptr[n + 1] > n2b_ptr[n])) {
+        u[n] = 0.;
+        v[n] = 0.;
+    }
+}
+
+void launch_resid(Ctx *c) {
+    ElemData D = c->ed;
+    D.first_elem = c->first_elem;
+    if (c->E) hipLaunchKernelGGL(k_resid2d, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->ER);
+    if (c->B) hipLaunchKernelGGL(k_resid1d, dim3(nblocks_for(c->B, 128)), dim3(128), 0, c->stream, D, c->B,
+                                 c->ER + 9 * (size_t)c->E);
+    hipLaunchKernelGGL(k_resid_gather, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->n2e_ptr,
+                       c->n2e, c->n2b_ptr, c->n2b, c->ER, c->ER + 9 * (size_t)c->E, c->old_head, c->u, c->v, c->R);
+}
+
+// ------------------------------------------------------------------ Jacobian
+// One thread per (triangle, column): column q = 3*j + var (var 0 u, 1 v, 2 h).
+// EM[e*81 + (i*3+j)*9 + r*3 + var] = d elem_rhs[i].eq_r / d var_j.
+__global__ void __launch_bounds__(128) k_jac2d(ElemData D, int E, int b0, int b1, int b2, double *EM) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * E) return;
+    int ie = t / 9, q = t % 9, j = q / 3, var = q % 3;
+    double col[9];
+    jac_column<3>(D, ie, j, var, col);
+    const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
+    const int lb[3] = {b0, b1, b2};
+    jac_column_db<3>(D, nodes, lb, j, var, col);
+    double *o = EM + 81 * (size_t)ie;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) o[(3 * i + j) * 9 + 3 * r + var] = col[3 * i + r];
+}
+
+__global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, double *EM1) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * B) return;
+    int ie = t / 6, q = t % 6, j = q / 3, var = q % 3;
+    double col[9];
+    jac_column<2>(D, ie, j, var, col);
+    const int nodes[2] = {D.e1d[2 * ie], D.e1d[2 * ie + 1]};
+    const int lb[2] = {b0, b1};
+    jac_column_db<2>(D, nodes, lb, j, var, col);
+    double *o = EM1 + 36 * (size_t)ie;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) o[(2 * i + j) * 9 + 3 * r + var] = col[3 * i + r];
+}
+
+void launch_jacobian(Ctx *c) {
+    ElemData D = c->ed;
+    D.first_elem = nullptr;
+    if (c->E)
+        hipLaunchKernelGGL(k_jac2d, dim3(nblocks_for(9 * c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->loc_bc[0],
+                           c->loc_bc[1], c->loc_bc[2], c->EM);
+    if (c->B)
+        hipLaunchKernelGGL(k_jac1d, dim3(nblocks_for(6 * c->B, 128)), dim3(128), 0, c->stream, D, c->B, c->loc_bc[0],
+                           c->loc_bc[1], c->EM + 81 * (size_t)c->E);
+}
+
+// Ordered block assembly: thread per (slot, entry).  slot_src lists 9-double
+// element-block offsets in EM in ascending element order, 2D before 1D.
+__global__ void k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_src, const double *EM,
+                           double *diag, double *offd) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nslots * 9) return;
+    int s = (int)(t / 9), k = (int)(t % 9);
+    double a = 0.0;
+    for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) a += EM[9 * (size_t)slot_src[p] + k];
+    if (s < N) diag[9 * (size_t)s + k] = a;
+    else offd[9 * (size_t)(s - N) + k] = a;
+}
+
+// diagonal floor and residual tweak, fe_sw2_load.c:218-236
+__global__ void k_diag_floor(int N, double *diag, double *R, const double *u, const double *v) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double f = NOT_QUITE_SMALL;
+    double *D = diag + 9 * (size_t)n;
+    if (fabs(D[0]) < f) {
+        D[0] = f;
+        if (fabs(R[3 * n]) < f) R[3 * n] = -f * u[n];
+    }
+    if (fabs(D[4]) < f) {
+        D[4] = f;
+        if (fabs(R[3 * n + 1]) < f) R[3 * n + 1] = -f * v[n];
+    }
+    if (fabs(D[8]) < f) D[8] = f;
+}
+
+void launch_assemble(Ctx *c) {
+    int nslots = c->N + c->nnz;
+    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(9 * nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
+                       c->slot_ptr, c->slot_src, c->EM, c->diag, c->offd);
+    hipLaunchKernelGGL(k_diag_floor, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->diag, c->R,
+                       c->u, c->v);
+}
+
+// ------------------------------------------------------- init / wet-dry flags
+// fe_sw2_init (fe_sw2_init.c:33-79)
+__global__ void k_init_nodes(int N, const double *old_head, const double *old_u, const double *old_v, double *head,
+                             double *u, double *v, const int *node_bc, const double *bcval, int *bc_mask,
+                             int *wd_tmp) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double hh = old_head[n], uu = old_u[n], vv = old_v[n];
+    int m0 = ADH_SW2_NO, m1 = ADH_SW2_NO, m2 = ADH_SW2_NO;
+    int bc = node_bc[n];
+    if (bc == BCT_VEL_DIR) {
+        m0 = m1 = ADH_SW2_YES;
+        uu = bcval[3 * n]; vv = bcval[3 * n + 1];
+    } else if (bc == BCT_PRS_DIR) {
+        m2 = ADH_SW2_YES;
+        hh = bcval[3 * n + 2];
+    } else if (bc == BCT_VEL_PRS_DIR) {
+        m0 = m1 = m2 = ADH_SW2_YES;
+        uu = bcval[3 * n]; vv = bcval[3 * n + 1]; hh = bcval[3 * n + 2];
+    }
+    head[n] = hh; u[n] = uu; v[n] = vv;
+    if (bc_mask) { bc_mask[3 * n] = m0; bc_mask[3 * n + 1] = m1; bc_mask[3 * n + 2] = m2; }
+    wd_tmp[n] = 0;
+}
+
+// fe_sw2_wdflag pass 1 (fe_sw2_wdflag.c:41-51): all writers store the same value
+__global__ void k_wd_mark(int E, const int *e2d, const double *head, int *wd_tmp) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int a = e2d[3 * e], b = e2d[3 * e + 1], c = e2d[3 * e + 2];
+    int wet = (head[a] > 0.0) + (head[b] > 0.0) + (head[c] > 0.0);
+    if (wet != 3) { wd_tmp[a] = 1; wd_tmp[b] = 1; wd_tmp[c] = 1; }
+}
+
+// fe_sw2_wdflag pass 2 (:57-73)
+__global__ void k_wd_flag(int E, const int *e2d, const double *head, const double *old_head, const int *wd_tmp,
+                          int *wd_flag) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int a = e2d[3 * e], b = e2d[3 * e + 1], c = e2d[3 * e + 2];
+    int f = (wd_tmp[a] != 0 || wd_tmp[b] != 0 || wd_tmp[c] != 0) ? 1 : 0;
+    if (head[a] < 0 && head[b] < 0 && head[c] < 0 && old_head[a] < 0 && old_head[b] < 0 && old_head[c] < 0) f = 2;
+    wd_flag[e] = f;
+}
+
+void launch_init(Ctx *c, int *bc_mask_dev) {
+    hipLaunchKernelGGL(k_init_nodes, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->old_head,
+                       c->old_u, c->old_v, c->head, c->u, c->v, c->node_bc, c->node_bc_val, bc_mask_dev, c->wd_tmp);
+    if (c->E) {
+        hipLaunchKernelGGL(k_wd_mark, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
+                           c->wd_tmp);
+        hipLaunchKernelGGL(k_wd_flag, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
+                           c->old_head, c->wd_tmp, c->wd_flag);
+    }
+}
+
+// fe_sw2 (fe_sw2.c:42-50)
+__global__ void k_shift(int N, double *h, double *u, double *v, double *oh, double *ou, double *ov, double *ooh,
+                        double *oou, double *oov) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    ooh[n] = oh[n]; oou[n] = ou[n]; oov[n] = ov[n];
+    oh[n] = h[n]; ou[n] = u[n]; ov[n] = v[n];
+}
+
+void launch_shift(Ctx *c) {
+    hipLaunchKernelGGL(k_shift, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->head, c->u, c->v,
+                       c->old_head, c->old_u, c->old_v, c->older_head, c->older_u, c->older_v);
+}
+
+// fe_sw2_inc (fe_sw2_inc.c:47-54): factor 1.0
+__global__ void k_inc(int N, const double *sol, double *h, double *u, double *v) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    u[n] += 1. * sol[3 * n];
+    v[n] += 1. * sol[3 * n + 1];
+    h[n] += 1. * sol[3 * n + 2];
+}
+
+void launch_inc(Ctx *c, double) {
+    hipLaunchKernelGGL(k_inc, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->sol, c->head, c->u,
+                       c->v);
+}
+
+__global__ void k_scale_vec(size_t n, double *x, double f) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= f;
+}
+
+void launch_scale_sol(Ctx *c, double f) {
+    hipLaunchKernelGGL(k_scale_vec, dim3(nblocks_for(3 * c->N, 256)), dim3(256), 0, c->stream, (size_t)3 * c->N,
+                       c->sol, f);
+}
+
+__global__ void k_zero(size_t n, double *x) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+void launch_zero(Ctx *c, double *p, size_t n) {
+    hipLaunchKernelGGL(k_zero, dim3(nblocks_for((int)n, 256)), dim3(256), 0, c->stream, n, p);
+}
+
+// ------------------------------------------------------------------- norms
+// get_residual_norms (fe_newton.c:79-150) with include_node_in_residual
+// (:177-187): max norms over nodes with min(h, h_old) > 0, l2 over all DOFs.
+// Block partials, then the last block to finish reduces them in block order.
+__global__ void __launch_bounds__(256) k_norms(int N, const double *R, const double *sol, const double *h,
+                                               const double *oh, double *part, unsigned *count, Norms *out) {
+    __shared__ double smax[256], ssum[256], sinc[256];
+    __shared__ bool last;
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0, s = 0.0, in = 0.0;
+    if (n < N) {
+        bool inc = dmin(h[n], oh[n]) > 0;
+        for (int j = 0; j < 3; j++) {
+            double r = R[3 * n + j], q = sol[3 * n + j];
+            if (inc && fabs(r) > m) m = fabs(r);
+            if (inc && fabs(q) > in) in = fabs(q);
+            s += r * r;
+        }
+    }
+    smax[threadIdx.x] = m; ssum[threadIdx.x] = s; sinc[threadIdx.x] = in;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            sinc[threadIdx.x] = fmax(sinc[threadIdx.x], sinc[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = smax[0];
+        part[3 * blockIdx.x + 1] = ssum[0];
+        part[3 * blockIdx.x + 2] = sinc[0];
+        __threadfence();
+        unsigned prev = atomicAdd(count, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double a = 0.0, b = 0.0, cc = 0.0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+        a = fmax(a, part[3 * k]);
+        b += part[3 * k + 1];
+        cc = fmax(cc, part[3 * k + 2]);
+    }
+    smax[threadIdx.x] = a; ssum[threadIdx.x] = b; sinc[threadIdx.x] = cc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            sinc[threadIdx.x] = fmax(sinc[threadIdx.x], sinc[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out->maxn = smax[0];
+        out->l2 = sqrt(ssum[0] / (double)(N * 3));
+        out->incn = sinc[0];
+        *count = 0;
+    }
+}
+
+void launch_norms(Ctx *c) {
+    int nb = nblocks_for(c->N, 256);
+    hipLaunchKernelGGL(k_norms, dim3(nb), dim3(256), 0, c->stream, c->N, c->R, c->sol, c->head, c->old_head,
+                       c->red.part, c->red.count + 7, c->norms);
+}
+
+}  // namespace sw2

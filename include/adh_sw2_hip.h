@@ -1,0 +1,173 @@
+/*
+ * adh_sw2_hip.h -- C ABI of the MI355X (gfx950) SW2 Newton-step library.
+ *
+ * The library replaces the reference's legacy serial SW2 Newton path:
+ * the five callbacks fe_newton receives (src/fe/fe_newton.c:220-224, bound in
+ * src/fe/sw2/fe_sw2_solve.c:45-49) and the linear-solve pair fe_newton calls
+ * (src/fe/fe_newton.c:547-557).  The reference passes SSUPER_MODEL*; a C shim
+ * in the reference forwards the arrays those structs
+ * hold -- plain pointers and sizes -- to the entry points below.  All state
+ * and the linear system stay resident in GPU memory between calls; host
+ * arrays are written only where an output pointer is non-NULL.
+ *
+ * Layouts are the reference's: per-node velocity as interleaved (x,y)
+ * (SVECT2D vel[N], src/structs/ssw_2d.h), 3 DOF per node ordered (u, v, h)
+ * with equations (x-momentum, y-momentum, continuity)
+ * (src/fe/sw2/fe_sw2_inc.c:47-51), 3x3 diagonal blocks diagonal[9*N]
+ * and off-diagonal blocks in SPARSE_VECT row order (src/structs/ssolve.h:16-21,
+ * block value order src/fe/fe_assemble_matrix.c:200-208).
+ *
+ * Return codes: ADH_SW2_OK (0) or a negative error; the solve and Newton
+ * calls additionally report the reference's YES (1) / NO (-3) verdicts in
+ * their out-parameters.  adh_sw2_hip_last_error() returns a message for the
+ * last failure on the calling thread.  Handles are not thread-safe; one
+ * handle per model, any number of handles per process.
+ */
+#ifndef ADH_SW2_HIP_H
+#define ADH_SW2_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADH_SW2_OK 0
+#define ADH_SW2_EINVAL -1
+#define ADH_SW2_EHIP -2
+#define ADH_SW2_ENOMEM -4
+#define ADH_SW2_EUNSUPPORTED -5
+
+#define ADH_SW2_YES 1   /* src/include/define.h:169 */
+#define ADH_SW2_NO (-3) /* src/include/define.h:170 */
+
+/* Mesh, strings and model constants: the fields of SGRID, SELEM_2D/1D,
+ * STR_VALUE, SMAT_SW and SMODEL that the SW2 path reads (see the
+ * reference's src/structs).  Element geometry (djac, grad_shp, 1D djac and
+ * outward normal) may be passed as computed by the reference's sgrid set-up
+ * or left NULL, in which case the library computes it with the same formulas
+ * (src/elem/element_djac_nrml_gradPhi.c:24-36, 193-274;
+ * src/structs/selem_1d.c:118-160). */
+typedef struct {
+    int nnodes, nelems2d, nelems1d, nmat;
+    const double *x, *y, *z;          /* [N] node coordinates (SNODE.x/y/z) */
+    const int *node_bc;               /* [N] bc_flag of the node's node string, 0 = none */
+    const double *node_bc_val;        /* [3N] Dirichlet u, v, h values (NULL if no Dirichlet nodes) */
+    const int *e2d;                   /* [3E] triangle node ids (SELEM_2D.nodes) */
+    const int *e2d_mat;               /* [E] material (SELEM_2D.mat) */
+    const double *e2d_ks;             /* [E] equivalent roughness height of the element's string */
+    const int *e2d_src_flag;          /* [E] 1 if the element string is BCT_WATER_SOURCE */
+    const double *e2d_src;            /* [E] water source series value */
+    const double *e2d_djac;           /* [E] optional: SELEM_2D.djac */
+    const double *e2d_grad;           /* [6E] optional: SELEM_2D.grad_shp (x0,y0,x1,y1,x2,y2) */
+    const int *e1d;                   /* [2B] boundary edge node ids (SELEM_1D.nodes) */
+    const int *e1d_adj;               /* [B] triangle adjacent to the edge (outward-normal test) */
+    const int *e1d_bc;                /* [B] 10 NB VEL, 9 NB OTW, 4 NB OUT */
+    const double *e1d_val;            /* [B] series value: VEL flux series / OTW tail elevation */
+    const double *e1d_djac;           /* [B] optional: SELEM_1D.djac */
+    const double *e1d_nrml;           /* [2B] optional: SELEM_1D.nrml */
+    const int *mat_turb;              /* [M] 0 none, 1 EVS, 2 EEV (SMAT_SW.EVSF/EEVF) */
+    const int *mat_eev_mode;          /* [M] */
+    const double *mat_eev_coef;       /* [M] */
+    const double *mat_ev;             /* [3M] user eddy viscosity xx, yy, xy */
+    const double *mat_cor_speed;      /* [M] get_coriolis_angular_speed(coriolis) */
+    double gravity, dt, tau_temporal, tau_pg, drying_lower_limit, drying_upper_limit, viscosity;
+} adh_sw2_mesh;
+
+/* Solver settings.  Defaults (adh_sw2_hip_default_opts) reproduce the
+ * reference's single-model settings (src/solver/solv_initialize.c:54-66,
+ * src/solver/solv_bcgstab.c:171-188). */
+typedef struct {
+    double tol_nonlin;   /* IP NTL (absolute max-norm) */
+    double inc_nonlin;   /* IP ITL */
+    int max_nonlin_it;   /* IP NIT */
+    int max_lin_it;      /* 500 */
+    int max_cuts;        /* 5 line-search cuts */
+    double lin_rtol;     /* BiCGSTAB stop: ||r||inf <= lin_atol + lin_rtol*||M^-1 b||inf */
+    double lin_atol;
+    int precision;       /* 0: FP64 Krylov; 1: FP32 matrix + Krylov, FP64 residual and update */
+    int preconditioner;  /* 2: 3x3 block Jacobi */
+    int check_every;     /* Krylov iterations between host convergence checks */
+} adh_sw2_opts;
+
+typedef struct {
+    int converged;       /* fe_newton's YES/NO as 1/0 */
+    int iterations;      /* Newton iterations */
+    int lin_iterations;  /* total BiCGSTAB iterations */
+    int solv_fail;
+    int nhist;           /* entries written below (initial residual first) */
+    int hist_cap;        /* capacity of the history arrays (may be 0) */
+    double *hist_max, *hist_l2, *hist_inc;
+    int *hist_lin_it;
+    double ms_load, ms_setup, ms_solve, ms_resid, ms_total;  /* device time per phase */
+} adh_sw2_report;
+
+typedef struct adh_sw2_hip adh_sw2_hip;
+
+const char *adh_sw2_hip_last_error(void);
+void adh_sw2_hip_default_opts(adh_sw2_opts *o);
+
+/* Creates device-resident mesh, adjacency, sparsity pattern and work arrays.
+ * Pattern = the reference's SPARSE_VECT insertion order (spv_load,
+ * src/solver/spv_load.c:5-139, driven by fe_sw2_load's element loops). */
+int adh_sw2_hip_create(const adh_sw2_mesh *mesh, const adh_sw2_opts *opts, adh_sw2_hip **out);
+void adh_sw2_hip_destroy(adh_sw2_hip *h);
+int adh_sw2_hip_set_opts(adh_sw2_hip *h, const adh_sw2_opts *opts);
+int adh_sw2_hip_set_dt(adh_sw2_hip *h, double dt);
+
+/* Geometry as used (host copies; any pointer may be NULL). */
+int adh_sw2_hip_geometry(adh_sw2_hip *h, double *djac, double *grad, double *djac1d, double *nrml1d);
+/* Off-diagonal sparsity: nnz blocks; row_ptr[N+1], col[nnz] in insertion order. */
+int adh_sw2_hip_pattern_nnz(adh_sw2_hip *h);
+int adh_sw2_hip_pattern(adh_sw2_hip *h, int *row_ptr, int *col);
+
+/* SSW_2D state (head, vel, old_*, older_*); vel arrays interleaved (x,y). */
+int adh_sw2_hip_set_state(adh_sw2_hip *h, const double *head, const double *vel, const double *old_head,
+                          const double *old_vel, const double *older_head, const double *older_vel);
+int adh_sw2_hip_get_state(adh_sw2_hip *h, double *head, double *vel, double *old_head, double *old_vel,
+                          double *older_head, double *older_vel);
+
+/* fe_sw2 state shift older <- old <- current (src/fe/sw2/fe_sw2.c:42-50). */
+int adh_sw2_hip_shift(adh_sw2_hip *h);
+/* fe_sw2_init (src/fe/sw2/fe_sw2_init.c:24-92) incl. fe_sw2_wdflag; bc_mask[3N] optional. */
+int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag);
+/* fe_sw2_resid (src/fe/sw2/fe_sw2_resid.c:29-216); residual[3N] optional. */
+int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual);
+/* fe_sw2_load (src/fe/sw2/fe_sw2_load.c:34-269) incl. the diagonal floor;
+ * diagonal[9N], offdiag[9*nnz] and residual[3N] (after the floor's residual
+ * tweak) optional. */
+int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdiag, double *residual);
+/* Element Jacobians as assembled (after Dirichlet edits): [81*E] in block order
+ * (row node i, col node j) -> 9 values; for parity tests. */
+int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *em1d);
+/* fe_sw2_inc (src/fe/sw2/fe_sw2_inc.c:24-57); sol NULL = the device solution. */
+int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol);
+/* solv_linear_sys_setup (src/solver/solv_linear_sys_setup.c:15-74): Jacobi
+ * scaling in place + preconditioner set-up; scale_vect[3N] optional. */
+int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect);
+/* solv_linear_sys_solve (src/solver/solv_linear_sys_solve.c:16-69):
+ * BiCGSTAB (src/solver/solv_bcgstab.c:39-301) on the scaled system;
+ * *verdict = YES/NO, sol[3N] optional, *lin_it optional. */
+int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *lin_it, int *verdict);
+/* Loads an explicit (unscaled) system: diagonal[9N], offdiag[9*nnz] in the
+ * pattern above, rhs[3N]; sol is zeroed.  For solver-only use and tests. */
+int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, const double *offdiag, const double *rhs);
+/* fe_newton with the SW2 callbacks (src/fe/fe_newton.c:214-912):
+ * init, residual, [load, setup, solve, inc, residual, line search]*.
+ * *verdict = YES/NO; on NO the state is reset by init (as :883) and the
+ * caller cuts dt (tc_scale). rep optional. */
+int adh_sw2_hip_newton(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict);
+/* One time step as the reference's time loop runs it (fe_main -> fe_sw2 ->
+ * fe_sw2_solve, retried with dt*0.25 on failure, src/main/adh_func.c:535-549). */
+int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int max_retries);
+
+/* Batched ensemble: n independent models advanced one time step each; the
+ * element, assembly and Krylov phases of all models share launches. */
+int adh_sw2_hip_timestep_batch(adh_sw2_hip **hs, int n, adh_sw2_report *reps, int *verdicts);
+
+/* Host-side utility with the reference's arithmetic: correctly rounded log
+ * used by the friction law on the device (exported for tests). */
+double adh_sw2_hip_cr_log(double x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
