@@ -284,7 +284,7 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     // ---- device arrays
     hipError_t he = hipStreamCreate(&c.stream);
     if (he != hipSuccess) { delete h; return fail(ADH_SW2_EHIP, "hipStreamCreate failed"); }
-    for (int i = 0; i < 8; i++) hipEventCreate(&c.ev[i]);
+    for (int i = 0; i < 8; i++) (void)hipEventCreate(&c.ev[i]);
     std::vector<int> zeros_i(N, 0);
     std::vector<double> zeros_d;
 #define UP(dst, src, n)                                    \
@@ -385,13 +385,13 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
 extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     if (!h) return;
     Ctx &c = h->c;
-    if (c.stream) hipStreamSynchronize(c.stream);
-    for (void *p : h->allocs) hipFree(p);
-    if (c.h_ksc) hipHostFree(c.h_ksc);
-    if (c.h_norms) hipHostFree(c.h_norms);
-    if (c.h_flag) hipHostFree(c.h_flag);
-    for (int i = 0; i < 8; i++) if (c.ev[i]) hipEventDestroy(c.ev[i]);
-    if (c.stream) hipStreamDestroy(c.stream);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (c.h_ksc) (void)hipHostFree(c.h_ksc);
+    if (c.h_norms) (void)hipHostFree(c.h_norms);
+    if (c.h_flag) (void)hipHostFree(c.h_flag);
+    for (int i = 0; i < 8; i++) if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
     delete h;
 }
 
@@ -495,7 +495,7 @@ extern "C" int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag) {
     launch_init(&c, dmask);
     int rc = sync_check(c);
     if (!rc && bc_mask) rc = hipMemcpy(bc_mask, dmask, sizeof(int) * 3 * c.N, hipMemcpyDeviceToHost) == hipSuccess ? 0 : fail(ADH_SW2_EHIP, "copy bc_mask");
-    if (dmask) hipFree(dmask);
+    if (dmask) (void)hipFree(dmask);
     if (!rc && wd_flag) HIPCHK(hipMemcpy(wd_flag, c.wd_flag, sizeof(int) * c.E, hipMemcpyDeviceToHost));
     return rc;
 }
@@ -597,9 +597,81 @@ extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, co
     return 0;
 }
 
+extern "C" int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *offdiag, double *rhs) {
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (diagonal) HIPCHK(hipMemcpy(diagonal, c.diag, sizeof(double) * 9 * c.N, hipMemcpyDeviceToHost));
+    if (offdiag && c.nnz) HIPCHK(hipMemcpy(offdiag, c.offd, sizeof(double) * 9 * c.nnz, hipMemcpyDeviceToHost));
+    if (rhs) HIPCHK(hipMemcpy(rhs, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_probe_math(const double *x, int n, double *log_out, double *sqrt_out, double *div_out) {
+    if (!x || n <= 0 || !log_out || !sqrt_out || !div_out) return fail(ADH_SW2_EINVAL, "bad probe arguments");
+    double *dx = nullptr, *dl = nullptr, *ds = nullptr, *dd = nullptr;
+    crlog::tables *dt = nullptr;
+    crlog::tables T;
+    crlog::make_tables(&T);
+    int rc = 0;
+    if (hipMalloc(&dx, sizeof(double) * n) != hipSuccess || hipMalloc(&dl, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&ds, sizeof(double) * n) != hipSuccess || hipMalloc(&dd, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&dt, sizeof(T)) != hipSuccess) {
+        rc = fail(ADH_SW2_ENOMEM, "probe alloc");
+    } else if (hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(dt, &T, sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(ADH_SW2_EHIP, "probe upload");
+    } else {
+        launch_probe(dx, n, dt, dl, ds, dd);
+        if (hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(log_out, dl, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(sqrt_out, ds, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(div_out, dd, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(ADH_SW2_EHIP, "probe run");
+    }
+    if (dx) (void)hipFree(dx);
+    if (dl) (void)hipFree(dl);
+    if (ds) (void)hipFree(ds);
+    if (dd) (void)hipFree(dd);
+    if (dt) (void)hipFree(dt);
+    return rc;
+}
+
+extern "C" int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms) {
+    if (!h || !avg_ms || reps < 1 || which < 0 || which > 3) return fail(ADH_SW2_EINVAL, "bad arguments");
+    Ctx &c = h->c;
+    if (which == 0) {
+        KScal k{};
+        k.rho = 1.0; k.rhop = 1.0; k.alpha = 1.0; k.omega = 1.0; k.max_it = 1 << 30; k.done = 0;
+        HIPCHK(hipMemcpy(c.ksc, &k, sizeof(KScal), hipMemcpyHostToDevice));
+    }
+    auto launch = [&]() {
+        if (which == 0) launch_k1_only(&c);
+        else if (which == 1) launch_jac2d_only(&c);
+        else if (which == 2) launch_resid2d_only(&c);
+        else launch_assemble_only(&c);
+    };
+    launch();  // warm-up
+    HIPCHK(hipStreamSynchronize(c.stream));
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, c.stream));
+    for (int i = 0; i < reps; i++) launch();
+    HIPCHK(hipEventRecord(b, c.stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIPCHK(hipGetLastError());
+    *avg_ms = ms / reps;
+    return 0;
+}
+
 static float elapsed(Ctx &c, int a, int b) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c.ev[a], c.ev[b]);
+    (void)hipEventElapsedTime(&ms, c.ev[a], c.ev[b]);
     return ms;
 }
 
@@ -627,34 +699,34 @@ static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
     const adh_sw2_opts &o = c.opts;
     int rc;
     Norms nm;
-    hipEventRecord(c.ev[0], c.stream);
+    (void)hipEventRecord(c.ev[0], c.stream);
     launch_init(&c, nullptr);
-    hipEventRecord(c.ev[1], c.stream);
+    (void)hipEventRecord(c.ev[1], c.stream);
     launch_resid(&c);
     launch_norms(&c);
-    hipEventRecord(c.ev[2], c.stream);
+    (void)hipEventRecord(c.ev[2], c.stream);
     if ((rc = read_norms(c, &nm))) return rc;
     if (rep) rep->ms_resid += elapsed(c, 1, 2);
     hist(rep, nm, 0);
     int it = 0, solv_flag = ADH_SW2_YES, keep = 0;
     do {
         int lin_it = 0, vd = ADH_SW2_YES;
-        hipEventRecord(c.ev[3], c.stream);
+        (void)hipEventRecord(c.ev[3], c.stream);
         launch_jacobian(&c);
         launch_assemble(&c);
         launch_zero(&c, c.sol, 3 * (size_t)c.N);
-        hipEventRecord(c.ev[4], c.stream);
+        (void)hipEventRecord(c.ev[4], c.stream);
         launch_setup(&c);
-        hipEventRecord(c.ev[5], c.stream);
+        (void)hipEventRecord(c.ev[5], c.stream);
         if ((rc = krylov(c, &lin_it, &vd))) return rc;
-        hipEventRecord(c.ev[6], c.stream);
+        (void)hipEventRecord(c.ev[6], c.stream);
         solv_flag = vd;
         launch_inc(&c, 1.0);
         launch_scale_sol(&c, -1.0);
         double old_l2 = nm.l2;
         launch_resid(&c);
         launch_norms(&c);
-        hipEventRecord(c.ev[7], c.stream);
+        (void)hipEventRecord(c.ev[7], c.stream);
         if ((rc = read_norms(c, &nm))) return rc;
         if (rep) {
             rep->ms_load += elapsed(c, 3, 4);

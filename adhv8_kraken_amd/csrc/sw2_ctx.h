@@ -97,6 +97,11 @@ void launch_krylov_iters(Ctx *c, int k);
 void launch_krylov_epilogue(Ctx *c);
 void launch_geometry(Ctx *c);
 void launch_zero(Ctx *c, double *p, size_t n);
+void launch_k1_only(Ctx *c);
+void launch_jac2d_only(Ctx *c);
+void launch_resid2d_only(Ctx *c);
+void launch_assemble_only(Ctx *c);
+void launch_probe(const double *x, int n, const crlog::tables *t, double *lg, double *sq, double *dv);
 
 int nblocks_for(int n, int bs);
 

@@ -171,6 +171,25 @@ void launch_assemble(Ctx *c) {
                        c->u, c->v);
 }
 
+void launch_jac2d_only(Ctx *c) {
+    ElemData D = c->ed;
+    D.first_elem = nullptr;
+    hipLaunchKernelGGL(k_jac2d, dim3(nblocks_for(9 * c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->loc_bc[0],
+                       c->loc_bc[1], c->loc_bc[2], c->EM);
+}
+
+void launch_resid2d_only(Ctx *c) {
+    ElemData D = c->ed;
+    D.first_elem = c->first_elem;
+    hipLaunchKernelGGL(k_resid2d, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->ER);
+}
+
+void launch_assemble_only(Ctx *c) {
+    int nslots = c->N + c->nnz;
+    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(9 * nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
+                       c->slot_ptr, c->slot_src, c->EM, c->diag, c->offd);
+}
+
 // ------------------------------------------------------- init / wet-dry flags
 // fe_sw2_init (fe_sw2_init.c:33-79)
 __global__ void k_init_nodes(int N, const double *old_head, const double *old_u, const double *old_v, double *head,
@@ -272,6 +291,20 @@ __global__ void k_zero(size_t n, double *x) {
 
 void launch_zero(Ctx *c, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero, dim3(nblocks_for((int)n, 256)), dim3(256), 0, c->stream, n, p);
+}
+
+// rounding probe: device cr_log, sqrt and division of each input
+__global__ void k_probe(const double *x, int n, const crlog::tables *t, double *lg, double *sq, double *dv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = x[i];
+    lg[i] = crlog::cr_log_t(a, t);
+    sq[i] = sqrt(a);
+    dv[i] = 1.0 / a;
+}
+
+void launch_probe(const double *x, int n, const crlog::tables *t, double *lg, double *sq, double *dv) {
+    hipLaunchKernelGGL(k_probe, dim3(nblocks_for(n, 256)), dim3(256), 0, 0, x, n, t, lg, sq, dv);
 }
 
 // ------------------------------------------------------------------- norms

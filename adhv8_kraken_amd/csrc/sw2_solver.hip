@@ -419,6 +419,16 @@ static void iters(Ctx *c, int k) {
     }
 }
 
+void launch_k1_only(Ctx *c) {
+    int nb = nblocks_for(c->N, 256);
+    if (c->opts.precision == 1)
+        hipLaunchKernelGGL(k_bcg_k1<float>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<float>(c), c->row_ptr,
+                           c->col, c->red.part, c->red.count + 2, c->ksc);
+    else
+        hipLaunchKernelGGL(k_bcg_k1<double>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<double>(c), c->row_ptr,
+                           c->col, c->red.part, c->red.count + 2, c->ksc);
+}
+
 void launch_krylov_prologue(Ctx *c) {
     if (c->opts.precision == 1) prologue<float>(c);
     else prologue<double>(c);

@@ -1,0 +1,304 @@
+"""Python host mirror of the reference's SW2 Newton-path interface.
+
+`SW2Model` wraps one device-resident model of libadh_sw2_hip.so (the C ABI in
+include/adh_sw2_hip.h) with methods named after the reference functions they
+replace:
+
+    fe_sw2          -> shift()                (src/fe/sw2/fe_sw2.c:30)
+    fe_sw2_init     -> init()                 (src/fe/sw2/fe_sw2_init.c:24)
+    fe_sw2_resid    -> resid()                (src/fe/sw2/fe_sw2_resid.c:29)
+    fe_sw2_load     -> load()                 (src/fe/sw2/fe_sw2_load.c:34)
+    fe_sw2_inc      -> inc()                  (src/fe/sw2/fe_sw2_inc.c:24)
+    solv_linear_sys_setup -> linear_sys_setup() (src/solver/solv_linear_sys_setup.c:15)
+    solv_linear_sys_solve -> linear_sys_solve() (src/solver/solv_linear_sys_solve.c:16)
+    fe_newton       -> newton()               (src/fe/fe_newton.c:214)
+
+The library is the only compute path: if libadh_sw2_hip.so is missing or no
+GPU is visible, construction raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libadh_sw2_hip.so")
+
+YES, NO = 1, -3
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+
+class Mesh(C.Structure):
+    _fields_ = [("nnodes", C.c_int), ("nelems2d", C.c_int), ("nelems1d", C.c_int), ("nmat", C.c_int),
+                ("x", PD), ("y", PD), ("z", PD), ("node_bc", PI), ("node_bc_val", PD),
+                ("e2d", PI), ("e2d_mat", PI), ("e2d_ks", PD), ("e2d_src_flag", PI), ("e2d_src", PD),
+                ("e2d_djac", PD), ("e2d_grad", PD),
+                ("e1d", PI), ("e1d_adj", PI), ("e1d_bc", PI), ("e1d_val", PD), ("e1d_djac", PD), ("e1d_nrml", PD),
+                ("mat_turb", PI), ("mat_eev_mode", PI), ("mat_eev_coef", PD), ("mat_ev", PD),
+                ("mat_cor_speed", PD),
+                ("gravity", C.c_double), ("dt", C.c_double), ("tau_temporal", C.c_double), ("tau_pg", C.c_double),
+                ("drying_lower_limit", C.c_double), ("drying_upper_limit", C.c_double),
+                ("viscosity", C.c_double)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("tol_nonlin", C.c_double), ("inc_nonlin", C.c_double), ("max_nonlin_it", C.c_int),
+                ("max_lin_it", C.c_int), ("max_cuts", C.c_int), ("lin_rtol", C.c_double), ("lin_atol", C.c_double),
+                ("precision", C.c_int), ("preconditioner", C.c_int), ("check_every", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("lin_iterations", C.c_int),
+                ("solv_fail", C.c_int), ("nhist", C.c_int), ("hist_cap", C.c_int), ("hist_max", PD),
+                ("hist_l2", PD), ("hist_inc", PD), ("hist_lin_it", PI), ("ms_load", C.c_double),
+                ("ms_setup", C.c_double), ("ms_solve", C.c_double), ("ms_resid", C.c_double),
+                ("ms_total", C.c_double)]
+
+
+class SW2Error(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads libadh_sw2_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SW2Error(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (make -C csrc)")
+        L = C.CDLL(LIB_PATH)
+        L.adh_sw2_hip_last_error.restype = C.c_char_p
+        L.adh_sw2_hip_default_opts.argtypes = [C.POINTER(Opts)]
+        L.adh_sw2_hip_create.argtypes = [C.POINTER(Mesh), C.POINTER(Opts), C.POINTER(C.c_void_p)]
+        L.adh_sw2_hip_destroy.argtypes = [C.c_void_p]
+        L.adh_sw2_hip_set_opts.argtypes = [C.c_void_p, C.POINTER(Opts)]
+        L.adh_sw2_hip_set_dt.argtypes = [C.c_void_p, C.c_double]
+        L.adh_sw2_hip_geometry.argtypes = [C.c_void_p, PD, PD, PD, PD]
+        L.adh_sw2_hip_pattern_nnz.argtypes = [C.c_void_p]
+        L.adh_sw2_hip_pattern.argtypes = [C.c_void_p, PI, PI]
+        L.adh_sw2_hip_set_state.argtypes = [C.c_void_p] + [PD] * 6
+        L.adh_sw2_hip_get_state.argtypes = [C.c_void_p] + [PD] * 6
+        L.adh_sw2_hip_shift.argtypes = [C.c_void_p]
+        L.adh_sw2_hip_init.argtypes = [C.c_void_p, PI, PI]
+        L.adh_sw2_hip_resid.argtypes = [C.c_void_p, PD]
+        L.adh_sw2_hip_load.argtypes = [C.c_void_p, PD, PD, PD]
+        L.adh_sw2_hip_elem_jacobians.argtypes = [C.c_void_p, PD, PD]
+        L.adh_sw2_hip_inc.argtypes = [C.c_void_p, PD]
+        L.adh_sw2_hip_linear_sys_setup.argtypes = [C.c_void_p, PD]
+        L.adh_sw2_hip_linear_sys_solve.argtypes = [C.c_void_p, PD, PI, PI]
+        L.adh_sw2_hip_set_system.argtypes = [C.c_void_p, PD, PD, PD]
+        L.adh_sw2_hip_newton.argtypes = [C.c_void_p, C.POINTER(Report), PI]
+        L.adh_sw2_hip_timestep.argtypes = [C.c_void_p, C.POINTER(Report), PI, C.c_int]
+        L.adh_sw2_hip_timestep_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Report), PI]
+        L.adh_sw2_hip_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, PD]
+        L.adh_sw2_hip_get_system.argtypes = [C.c_void_p, PD, PD, PD]
+        L.adh_sw2_hip_probe_math.argtypes = [PD, C.c_int, PD, PD, PD]
+        L.adh_sw2_hip_cr_log.argtypes = [C.c_double]
+        L.adh_sw2_hip_cr_log.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SW2Error(f"libadh_sw2_hip error {rc}: {lib().adh_sw2_hip_last_error().decode()}")
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().adh_sw2_hip_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(PD)
+
+
+def _pi(a):
+    return None if a is None else a.ctypes.data_as(PI)
+
+
+class SW2Model:
+    """One SW2 model resident on the GPU (see module docstring)."""
+
+    def __init__(self, problem, opts: Opts | None = None):
+        L = lib()
+        p = problem
+        self.p = p
+        self._keep = []
+
+        def k(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        m = Mesh()
+        m.nnodes, m.nelems2d, m.nelems1d, m.nmat = p.nnodes, p.nelems2d, p.nelems1d, p.nmat
+        m.x, m.y, m.z = _pd(k(p.x, np.float64)), _pd(k(p.y, np.float64)), _pd(k(p.z, np.float64))
+        m.node_bc = _pi(k(p.node_bc, np.int32))
+        m.node_bc_val = _pd(k(p.node_bc_val, np.float64))
+        m.e2d = _pi(k(p.e2d, np.int32))
+        m.e2d_mat = _pi(k(p.e2d_mat, np.int32))
+        m.e2d_ks = _pd(k(p.e2d_ks, np.float64))
+        m.e2d_src_flag = _pi(k(p.e2d_src_flag, np.int32))
+        m.e2d_src = _pd(k(p.e2d_src, np.float64))
+        if p.nelems1d:
+            m.e1d = _pi(k(p.e1d, np.int32))
+            m.e1d_adj = _pi(k(p.e1d_adj, np.int32))
+            m.e1d_bc = _pi(k(p.e1d_bc, np.int32))
+            m.e1d_val = _pd(k(p.e1d_val, np.float64))
+        m.mat_turb = _pi(k(p.mat_turb, np.int32))
+        m.mat_eev_mode = _pi(k(p.mat_eev_mode, np.int32))
+        m.mat_eev_coef = _pd(k(p.mat_eev_coef, np.float64))
+        m.mat_ev = _pd(k(p.mat_ev, np.float64))
+        m.mat_cor_speed = _pd(k(p.mat_cor_speed, np.float64))
+        m.gravity, m.dt, m.tau_temporal, m.tau_pg = p.g, p.dt, p.tau_temporal, p.tau_pg
+        m.drying_lower_limit, m.drying_upper_limit, m.viscosity = p.drying_lower, p.drying_upper, p.viscosity
+        self.opts = opts if opts is not None else default_opts()
+        h = C.c_void_p()
+        _check(L.adh_sw2_hip_create(C.byref(m), C.byref(self.opts), C.byref(h)))
+        self.h = h
+        self.L = L
+        self._keep = []  # the library copied everything it needs
+        self.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.adh_sw2_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def N(self):
+        return self.p.nnodes
+
+    def set_opts(self, opts: Opts):
+        self.opts = opts
+        _check(self.L.adh_sw2_hip_set_opts(self.h, C.byref(opts)))
+
+    def set_dt(self, dt):
+        _check(self.L.adh_sw2_hip_set_dt(self.h, dt))
+
+    def set_state(self, head, vel, old_head, old_vel, older_head, older_vel):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (head, vel, old_head, old_vel, older_head, older_vel)]
+        _check(self.L.adh_sw2_hip_set_state(self.h, *[_pd(x) for x in a]))
+
+    def get_state(self):
+        N = self.N
+        out = [np.empty(N), np.empty(2 * N), np.empty(N), np.empty(2 * N), np.empty(N), np.empty(2 * N)]
+        _check(self.L.adh_sw2_hip_get_state(self.h, *[_pd(x) for x in out]))
+        return out
+
+    def geometry(self):
+        p = self.p
+        dj, gr = np.empty(p.nelems2d), np.empty(6 * p.nelems2d)
+        dj1, nr1 = np.empty(max(1, p.nelems1d)), np.empty(2 * max(1, p.nelems1d))
+        _check(self.L.adh_sw2_hip_geometry(self.h, _pd(dj), _pd(gr), _pd(dj1), _pd(nr1)))
+        return dj, gr, dj1[: p.nelems1d], nr1[: 2 * p.nelems1d]
+
+    def pattern(self):
+        nnz = self.L.adh_sw2_hip_pattern_nnz(self.h)
+        rp = np.empty(self.N + 1, dtype=np.int32)
+        col = np.empty(max(1, nnz), dtype=np.int32)
+        _check(self.L.adh_sw2_hip_pattern(self.h, _pi(rp), _pi(col)))
+        return rp, col[:nnz]
+
+    def shift(self):
+        _check(self.L.adh_sw2_hip_shift(self.h))
+
+    def init(self):
+        bc = np.empty(3 * self.N, dtype=np.int32)
+        wd = np.empty(self.p.nelems2d, dtype=np.int32)
+        _check(self.L.adh_sw2_hip_init(self.h, _pi(bc), _pi(wd)))
+        return bc, wd
+
+    def resid(self):
+        r = np.empty(3 * self.N)
+        _check(self.L.adh_sw2_hip_resid(self.h, _pd(r)))
+        return r
+
+    def load(self):
+        nnz = self.L.adh_sw2_hip_pattern_nnz(self.h)
+        diag, offd, r = np.empty(9 * self.N), np.empty(9 * max(1, nnz)), np.empty(3 * self.N)
+        _check(self.L.adh_sw2_hip_load(self.h, _pd(diag), _pd(offd), _pd(r)))
+        return diag, offd[: 9 * nnz], r
+
+    def elem_jacobians(self):
+        em = np.empty(81 * self.p.nelems2d)
+        em1 = np.empty(36 * max(1, self.p.nelems1d))
+        _check(self.L.adh_sw2_hip_elem_jacobians(self.h, _pd(em), _pd(em1)))
+        return em, em1[: 36 * self.p.nelems1d]
+
+    def inc(self, sol=None):
+        s = None if sol is None else np.ascontiguousarray(sol, dtype=np.float64)
+        _check(self.L.adh_sw2_hip_inc(self.h, _pd(s)))
+
+    def linear_sys_setup(self):
+        s = np.empty(3 * self.N)
+        _check(self.L.adh_sw2_hip_linear_sys_setup(self.h, _pd(s)))
+        return s
+
+    def linear_sys_solve(self):
+        x = np.empty(3 * self.N)
+        it, vd = C.c_int(0), C.c_int(0)
+        _check(self.L.adh_sw2_hip_linear_sys_solve(self.h, _pd(x), C.byref(it), C.byref(vd)))
+        return vd.value, x, it.value
+
+    def set_system(self, diag, offd, rhs):
+        d, o, r = (np.ascontiguousarray(a, dtype=np.float64) for a in (diag, offd, rhs))
+        _check(self.L.adh_sw2_hip_set_system(self.h, _pd(d), _pd(o), _pd(r)))
+
+    def get_system(self):
+        nnz = self.L.adh_sw2_hip_pattern_nnz(self.h)
+        diag, offd, r = np.empty(9 * self.N), np.empty(9 * max(1, nnz)), np.empty(3 * self.N)
+        _check(self.L.adh_sw2_hip_get_system(self.h, _pd(diag), _pd(offd), _pd(r)))
+        return diag, offd[: 9 * nnz], r
+
+    def time_kernel(self, which: int, reps: int = 20) -> float:
+        """Average launch duration in ms (0 SpMV+M^-1, 1 FD Jacobian, 2 residual, 3 assembly)."""
+        ms = C.c_double(0.0)
+        _check(self.L.adh_sw2_hip_time_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def _report(self, cap):
+        r = Report()
+        self._h = [np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap, dtype=np.int32)]
+        r.hist_cap = cap
+        r.hist_max, r.hist_l2, r.hist_inc, r.hist_lin_it = (_pd(self._h[0]), _pd(self._h[1]), _pd(self._h[2]),
+                                                            _pi(self._h[3]))
+        return r
+
+    def _rdict(self, r, vd):
+        n = min(r.nhist, r.hist_cap)
+        return dict(verdict=vd, converged=r.converged, iterations=r.iterations, lin_its=r.lin_iterations,
+                    hist_max=self._h[0][:n].copy(), hist_l2=self._h[1][:n].copy(), hist_inc=self._h[2][:n].copy(),
+                    hist_lin_it=self._h[3][:n].copy(), ms_load=r.ms_load, ms_setup=r.ms_setup,
+                    ms_solve=r.ms_solve, ms_resid=r.ms_resid, ms_total=r.ms_total)
+
+    def newton(self, hist_cap=256):
+        r = self._report(hist_cap)
+        vd = C.c_int(0)
+        _check(self.L.adh_sw2_hip_newton(self.h, C.byref(r), C.byref(vd)))
+        return self._rdict(r, vd.value)
+
+    def timestep(self, max_retries=0, hist_cap=256):
+        r = self._report(hist_cap)
+        vd = C.c_int(0)
+        _check(self.L.adh_sw2_hip_timestep(self.h, C.byref(r), C.byref(vd), max_retries))
+        return self._rdict(r, vd.value)
+
+
+def cr_log(x: float) -> float:
+    return lib().adh_sw2_hip_cr_log(x)

@@ -163,9 +163,23 @@ int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int 
  * element, assembly and Krylov phases of all models share launches. */
 int adh_sw2_hip_timestep_batch(adh_sw2_hip **hs, int n, adh_sw2_report *reps, int *verdicts);
 
-/* Host-side utility with the reference's arithmetic: correctly rounded log
- * used by the friction law on the device (exported for tests). */
+/* Reads the device linear system as it currently stands (after load it is
+ * unscaled, after linear_sys_setup it is Jacobi-scaled); any pointer may be NULL. */
+int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *offdiag, double *rhs);
+
+/* Average device duration (ms) of one launch of a phase kernel, timed with
+ * HIP events on the library's stream over `reps` back-to-back launches on
+ * the model's current data: 0 = SpMV + block-Jacobi apply + dot (the BiCGSTAB
+ * K1 kernel, in the configured precision), 1 = finite-difference element
+ * Jacobian, 2 = element residual, 3 = ordered block assembly.  Clobbers the
+ * Krylov work vectors; no counterpart in the reference (measurement hook). */
+int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms);
+
+/* Correctly rounded log used by the friction law, evaluated on the host. */
 double adh_sw2_hip_cr_log(double x);
+/* Evaluates the device's log (cr_log) and sqrt on n inputs -- a check that
+ * the GPU's rounding matches the host's; no counterpart in the reference. */
+int adh_sw2_hip_probe_math(const double *x, int n, double *log_out, double *sqrt_out, double *div_out);
 
 #ifdef __cplusplus
 }

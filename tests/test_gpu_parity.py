@@ -1,0 +1,194 @@
+"""GPU parity: libadh_sw2_hip.so on the device against the CPU oracle.
+
+Bar (integer/index work and the element arithmetic): bit-exact.  The oracle
+runs with its correctly rounded log (the device log is correctly rounded;
+glibc's misrounds ~7e-4 of inputs -- see test_oracle_pins.py).  Krylov
+iterates differ from the oracle's by the order of dot-product sums only
+(tree vs sequential), so solutions and Newton histories are compared with a
+tolerance written in each test.
+"""
+import numpy as np
+import pytest
+
+from gpu_helpers import bits_equal, nmismatch, pair, say, stepped_problem
+from adhv8_kraken_amd import mesh
+from adhv8_kraken_amd.sw2 import SW2Model, default_opts, YES
+from oracle.oracle import opts as oopts, set_log_mode
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    p1 = mesh.channel_config1()
+    f = mesh.floodplain(20, 10)
+    b = mesh.bumpy_channel(60, 30, seed=11, dt=5.0)
+    return {"channel-40x20": p1, "floodplain-20x10": f, "bumpy-60x30": b}
+
+
+CASES = _cases()
+
+
+@pytest.fixture(scope="module")
+def stepped():
+    out = {}
+    for k, p in CASES.items():
+        out[k] = stepped_problem(p)
+    return out
+
+
+def test_device_rounding_probe():
+    """Device cr_log equals the host's and binary128's; device sqrt and 1/x
+    equal the host's (IEEE correctly rounded)."""
+    import ctypes as C
+    from adhv8_kraken_amd.sw2 import lib
+    say("probe: device log/sqrt/div on 2^20 inputs")
+    rng = np.random.default_rng(3)
+    n = 1 << 20
+    bits = (rng.integers(0, 1 << 52, n, dtype=np.int64) | ((rng.integers(900, 1150, n, dtype=np.int64)) << 52))
+    x = bits.view(np.float64).copy()
+    x[: 1000] = 1.0 + 29.7 * rng.uniform(0.01, 200.0, 1000)  # the friction law's log arguments
+    lg, sq, dv = np.empty(n), np.empty(n), np.empty(n)
+    PD = C.POINTER(C.c_double)
+    L = lib()
+    L.adh_sw2_hip_probe_math.argtypes = [PD, C.c_int, PD, PD, PD]
+    assert L.adh_sw2_hip_probe_math(x.ctypes.data_as(PD), n, lg.ctypes.data_as(PD), sq.ctypes.data_as(PD),
+                                    dv.ctypes.data_as(PD)) == 0
+    host_log = np.array([L.adh_sw2_hip_cr_log(float(v)) for v in x[:20000]])
+    assert bits_equal(lg[:20000], host_log)
+    assert bits_equal(sq, np.sqrt(x))
+    assert bits_equal(dv, 1.0 / x)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_geometry_and_pattern(name):
+    p = CASES[name]
+    say(f"geometry/pattern: {name}")
+    o, g = pair(p)
+    for a, b in zip(o.geometry(), g.geometry()):
+        assert bits_equal(a, b)
+    o.shift(); o.init(); o.resid(); o.load()
+    rp_o, col_o = o.pattern()
+    rp_g, col_g = g.pattern()
+    assert np.array_equal(rp_o, rp_g) and np.array_equal(col_o, col_g)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_resid_load_setup_bitexact(name, stepped):
+    """init -> resid -> load -> linear_sys_setup on identical states: residual,
+    wet/dry flags, bc mask, post-residual velocities, assembled 3x3 blocks,
+    element Jacobians, scale vector and scaled system are bit-identical."""
+    p = CASES[name]
+    st = stepped[name]
+    say(f"resid/load/setup: {name}")
+    o, g = pair(p, st)
+    o.shift(); g.shift()
+    bc_o = o.init()
+    bc_g, wd_g = g.init()
+    assert np.array_equal(bc_o, bc_g)
+    assert np.array_equal(o.wdflag(), wd_g)
+    r_o = o.resid()
+    r_g = g.resid()
+    assert nmismatch(r_o, r_g) == 0
+    for a, b in zip(o.get_state(), g.get_state()):
+        assert bits_equal(a, b)
+    d_o, f_o = o.load()
+    d_g, f_g, rr_g = g.load()
+    assert nmismatch(d_o, d_g) == 0
+    assert nmismatch(f_o, f_g) == 0
+    assert bits_equal(o.residual(), rr_g)
+    em, _ = g.elem_jacobians()
+    for ie in range(0, p.nelems2d, max(1, p.nelems2d // 200)):
+        assert bits_equal(o.elem_jacobian(ie), em[81 * ie: 81 * ie + 81])
+    s_o = o.linear_setup(oopts(prec=2))
+    s_g = g.linear_sys_setup()
+    assert bits_equal(s_o, s_g)
+
+
+def test_dry_node_velocity_rule():
+    """Velocities at nodes with old_head <= 0 are seen only by the lowest
+    element touching the node, then zeroed (fe_sw2_resid.c:85-93)."""
+    p = mesh.floodplain(20, 10)
+    rng = np.random.default_rng(1)
+    N = p.nnodes
+    h0 = p.head0.copy()
+    vel = rng.uniform(-0.2, 0.2, 2 * N)
+    st = (h0, vel, h0, vel, h0, vel)
+    say("dry-node velocity rule: floodplain 20x10 with random velocities")
+    o, g = pair(p, st)
+    o.init(); g.init()
+    assert nmismatch(o.resid(), g.resid()) == 0
+    for a, b in zip(o.get_state(), g.get_state()):
+        assert bits_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["channel-40x20", "bumpy-60x30"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_linear_solve(name, precision, stepped):
+    """BiCGSTAB with 3x3 block-Jacobi on the same scaled system as the oracle.
+    FP64: iteration count within 1 and solution within 1e-9 relative (dot
+    sums are reordered).  FP32: solution within 1e-4 relative of FP64."""
+    p = CASES[name]
+    say(f"linear solve: {name} precision={precision}")
+    o, g = pair(p, stepped[name], default_opts(precision=precision))
+    o.shift(); g.shift(); o.init(); g.init(); o.resid(); g.resid(); o.load(); g.load()
+    oo = oopts(prec=2)
+    o.linear_setup(oo)
+    g.linear_sys_setup()
+    ok_o, x_o, it_o = o.linear_solve(oo)
+    vd, x_g, it_g = g.linear_sys_solve()
+    assert vd == YES and ok_o == 1
+    rel = np.abs(x_g - x_o).max() / np.abs(x_o).max()
+    say(f"  iterations oracle={it_o} gpu={it_g} rel.diff={rel:.3e}")
+    if precision == 0:
+        assert abs(it_o - it_g) <= 1
+        assert rel < 1e-9
+    else:
+        assert rel < 1e-4
+
+
+@pytest.mark.parametrize("name", ["channel-40x20", "bumpy-60x30"])
+def test_newton_history(name):
+    """One time step of Newton (block-Jacobi BiCGSTAB on both sides): same
+    iteration count; residual history within 1e-8 relative to R0 (H7: the
+    only difference is the order of dot/l2 sums)."""
+    p = CASES[name]
+    say(f"newton: {name}")
+    o, g = pair(p)
+    o.shift(); g.shift(); o.init(); g.init()
+    r0 = np.abs(o.resid()).max()
+    tol = 1e-8 * r0
+    go = default_opts(tol_nonlin=tol, max_nonlin_it=30)
+    g.set_opts(go)
+    res_o = o.newton(oopts(tol_nonlin=tol, prec=2, max_nonlin_it=30))
+    res_g = g.newton()
+    say(f"  oracle its={res_o['iterations']} hist={res_o['hist_max']}")
+    say(f"  gpu    its={res_g['iterations']} hist={res_g['hist_max']}")
+    assert res_g["verdict"] == YES and res_o["ok"] == 1
+    assert res_g["iterations"] == res_o["iterations"]
+    assert np.allclose(res_g["hist_max"], res_o["hist_max"], rtol=0, atol=1e-8 * r0)
+    h_o = o.get_state()
+    h_g = g.get_state()
+    for a, b in zip(h_o, h_g):
+        assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(a).max())
+
+
+def test_newton_fp32_refined():
+    """FP32 matrix/preconditioner/Krylov with FP64 residual and update reaches
+    the same FP64 nonlinear tolerance; state within 1e-7 relative of FP64."""
+    p = CASES["bumpy-60x30"]
+    say("newton fp32 vs fp64: bumpy-60x30")
+    g64 = SW2Model(p)
+    g32 = SW2Model(p, default_opts(precision=1))
+    g64.shift(); g64.init()
+    r0 = np.abs(g64.resid()).max()
+    for g in (g64, g32):
+        g.set_opts(default_opts(tol_nonlin=1e-8 * r0, max_nonlin_it=30, precision=g.opts.precision))
+    r64 = g64.newton()
+    g32.shift()
+    r32 = g32.newton()
+    say(f"  fp64 its={r64['iterations']} fp32 its={r32['iterations']} final {r64['hist_max'][-1]:.3e} "
+        f"{r32['hist_max'][-1]:.3e}")
+    assert r64["verdict"] == YES and r32["verdict"] == YES
+    assert r32["hist_max"][-1] <= 1e-8 * r0
+    s64, s32 = g64.get_state(), g32.get_state()
+    assert np.abs(s64[0] - s32[0]).max() <= 1e-7 * np.abs(s64[0]).max()
