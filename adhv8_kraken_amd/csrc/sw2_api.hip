@@ -170,7 +170,7 @@ static int alloc_solver(adh_sw2_hip *h) {
         (rc = dalloc(h, &c.fAp, n3)) || (rc = dalloc(h, &c.fs, n3)) || (rc = dalloc(h, &c.fAs, n3)) ||
         (rc = dalloc(h, &c.fq, n3)) || (rc = dalloc(h, &c.fu, n3)))
         return rc;
-    if ((rc = dalloc(h, &c.diag32, 9 * (size_t)c.N)) || (rc = dalloc(h, &c.offd32, 9 * (size_t)c.nnz)) ||
+    if ((rc = dalloc(h, &c.diag32, 9 * (size_t)c.N)) || (rc = dalloc(h, &c.offd32, 9 * c.nsell)) ||
         (rc = dalloc(h, &c.dinv32, 9 * (size_t)c.N)))
         return rc;
     return 0;
@@ -258,6 +258,33 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
         for (size_t k = 0; k < r.size(); k++) if (r[k] == b) return N + c.h_row_ptr[a] + (int)k;
         return -1;
     };
+    // SELL-64 slots for the off-diagonal blocks (see sw2_ctx.h)
+    c.nslice = (N + 63) / 64;
+    std::vector<int> sell_ptr(c.nslice + 1, 0);
+    for (int s = 0; s < c.nslice; s++) {
+        int kmax = 0;
+        for (int n = 64 * s; n < N && n < 64 * s + 64; n++) {
+            int len = c.h_row_ptr[n + 1] - c.h_row_ptr[n];
+            if (len > kmax) kmax = len;
+        }
+        sell_ptr[s + 1] = sell_ptr[s] + kmax;
+    }
+    c.nsell = 64 * (size_t)sell_ptr[c.nslice];
+    std::vector<int> sell_col(c.nsell > 0 ? c.nsell : 1);
+    c.h_offd_pos.assign(c.nnz > 0 ? c.nnz : 1, 0);
+    for (int s = 0; s < c.nslice; s++)
+        for (int j = 0; j < sell_ptr[s + 1] - sell_ptr[s]; j++)
+            for (int lane = 0; lane < 64; lane++) {
+                int n = 64 * s + lane;
+                size_t q = 64 * (size_t)(sell_ptr[s] + j) + lane;
+                if (n < N && j < c.h_row_ptr[n + 1] - c.h_row_ptr[n]) {
+                    int p = c.h_row_ptr[n] + j;
+                    sell_col[q] = c.h_col[p];
+                    c.h_offd_pos[p] = (int)q;
+                } else {
+                    sell_col[q] = n < N ? n : N - 1;
+                }
+            }
     const int nslots = N + c.nnz;
     std::vector<int> slot_ptr(nslots + 1, 0);
     for (int e = 0; e < E; e++)
@@ -336,14 +363,19 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     UP(n2b, n2b.data(), n2b.size());
     UP(row_ptr, c.h_row_ptr.data(), N + 1);
     UP(col, c.h_col.data(), c.nnz);
+    UP(sell_ptr, sell_ptr.data(), c.nslice + 1);
+    UP(sell_col, sell_col.data(), c.nsell);
+    UP(offd_pos, c.h_offd_pos.data(), c.nnz);
     UP(slot_ptr, slot_ptr.data(), nslots + 1);
     UP(slot_src, slot_src.data(), slot_src.size());
+    UP(pre, (const double *)nullptr, PRE_STRIDE * (size_t)E);
+    c.pre_valid = 0;
     UP(ER, (const double *)nullptr, 9 * (size_t)E + 6 * (size_t)B);
     UP(EM, (const double *)nullptr, 81 * (size_t)E + 36 * (size_t)B);
     UP(R, (const double *)nullptr, 3 * (size_t)N);
     UP(sol, (const double *)nullptr, 3 * (size_t)N);
     UP(diag, (const double *)nullptr, 9 * (size_t)N);
-    UP(offd, (const double *)nullptr, 9 * (size_t)c.nnz);
+    UP(offd, (const double *)nullptr, 9 * c.nsell);
     UP(scale, (const double *)nullptr, 3 * (size_t)N);
     UP(dinv, (const double *)nullptr, 9 * (size_t)N);
     UP(ksc, (const KScal *)nullptr, 1);
@@ -370,7 +402,7 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     D.older_head = c.older_head; D.older_u = c.older_u; D.older_v = c.older_v;
     D.node_bc = c.node_bc; D.first_elem = c.first_elem;
     D.e2d = c.e2d; D.djac = c.djac; D.grad = c.grad; D.mat = c.mat; D.ks = c.ks; D.src_flag = c.src_flag;
-    D.src = c.src; D.wd_flag = c.wd_flag;
+    D.src = c.src; D.wd_flag = c.wd_flag; D.pre = c.pre;
     D.e1d = c.e1d; D.djac1d = c.djac1d; D.nrml1d = c.nrml1d; D.bc1d = c.bc1d; D.val1d = c.val1d;
     D.mat_turb = c.mat_turb; D.mat_eev_mode = c.mat_eev_mode; D.mat_eev_coef = c.mat_eev_coef;
     D.mat_ev = c.mat_ev; D.mat_cor = c.mat_cor;
@@ -460,6 +492,7 @@ extern "C" int adh_sw2_hip_set_state(adh_sw2_hip *h, const double *head, const d
     if ((rc = set_pair(c, c.head, c.u, c.v, head, vel))) return rc;
     if ((rc = set_pair(c, c.old_head, c.old_u, c.old_v, old_head, old_vel))) return rc;
     if ((rc = set_pair(c, c.older_head, c.older_u, c.older_v, older_head, older_vel))) return rc;
+    c.pre_valid = 0;
     return 0;
 }
 
@@ -510,6 +543,23 @@ extern "C" int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual) {
     return 0;
 }
 
+// off-diagonal blocks between the reference's row order (host) and SELL-64 (device)
+static int offd_to_host(Ctx &c, double *out) {
+    std::vector<double> sv(9 * c.nsell);
+    HIPCHK(hipMemcpy(sv.data(), c.offd, sizeof(double) * sv.size(), hipMemcpyDeviceToHost));
+    for (int p = 0; p < c.nnz; p++)
+        for (int e = 0; e < 9; e++) out[9 * (size_t)p + e] = sv[sell_vidx((size_t)c.h_offd_pos[p], e)];
+    return 0;
+}
+
+static int offd_from_host(Ctx &c, const double *in) {
+    std::vector<double> sv(9 * c.nsell, 0.0);
+    for (int p = 0; p < c.nnz; p++)
+        for (int e = 0; e < 9; e++) sv[sell_vidx((size_t)c.h_offd_pos[p], e)] = in[9 * (size_t)p + e];
+    HIPCHK(hipMemcpy(c.offd, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdiag, double *residual) {
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
@@ -518,7 +568,7 @@ extern "C" int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdia
     int rc = sync_check(c);
     if (rc) return rc;
     if (diagonal) HIPCHK(hipMemcpy(diagonal, c.diag, sizeof(double) * 9 * c.N, hipMemcpyDeviceToHost));
-    if (offdiag && c.nnz) HIPCHK(hipMemcpy(offdiag, c.offd, sizeof(double) * 9 * c.nnz, hipMemcpyDeviceToHost));
+    if (offdiag && c.nnz && (rc = offd_to_host(c, offdiag))) return rc;
     if (residual) HIPCHK(hipMemcpy(residual, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
     return 0;
 }
@@ -591,7 +641,8 @@ extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, co
     if (!h || !diagonal || !rhs || (!offdiag && h->c.nnz)) return fail(ADH_SW2_EINVAL, "null argument");
     Ctx &c = h->c;
     HIPCHK(hipMemcpy(c.diag, diagonal, sizeof(double) * 9 * c.N, hipMemcpyHostToDevice));
-    if (c.nnz) HIPCHK(hipMemcpy(c.offd, offdiag, sizeof(double) * 9 * c.nnz, hipMemcpyHostToDevice));
+    int rc;
+    if (c.nnz && (rc = offd_from_host(c, offdiag))) return rc;
     HIPCHK(hipMemcpy(c.R, rhs, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.N));
     return 0;
@@ -602,7 +653,8 @@ extern "C" int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
     if (diagonal) HIPCHK(hipMemcpy(diagonal, c.diag, sizeof(double) * 9 * c.N, hipMemcpyDeviceToHost));
-    if (offdiag && c.nnz) HIPCHK(hipMemcpy(offdiag, c.offd, sizeof(double) * 9 * c.nnz, hipMemcpyDeviceToHost));
+    int rc;
+    if (offdiag && c.nnz && (rc = offd_to_host(c, offdiag))) return rc;
     if (rhs) HIPCHK(hipMemcpy(rhs, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
     return 0;
 }

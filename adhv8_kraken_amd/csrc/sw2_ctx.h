@@ -61,12 +61,25 @@ struct Ctx {
     int *row_ptr, *col;    // off-diagonal blocks, insertion order
     int *slot_ptr, *slot_src;  // N diagonal slots then nnz off-diagonal slots
     std::vector<int> h_row_ptr, h_col;
+    // Off-diagonal blocks in a wave-sliced ELL layout (SELL-64): block rows
+    // 64s..64s+63 form slice s with K_s = sell_ptr[s+1]-sell_ptr[s] block
+    // slots per row (rows shorter than K_s are padded with zero blocks whose
+    // column is the row itself).  Slot q = 64*(sell_ptr[s] + j) + lane holds
+    // the j-th block of row 64s+lane (the reference's insertion order);
+    // its 9 values sit at sell_vidx(q, e) so that a wavefront reading entry e
+    // of slot j touches 64 consecutive values.
+    int nslice;
+    size_t nsell;          // 64 * sell_ptr[nslice]
+    int *sell_ptr, *sell_col, *offd_pos;  // [nslice+1], [nsell], [nnz] (CSR block -> slot)
+    std::vector<int> h_offd_pos;
     // work
+    double *pre;           // [PRE_STRIDE * E] old/older-state element terms
+    int pre_valid;         // pre matches the current old/older state
     double *ER;            // element residuals [9E + 6B]
     double *EM;            // element matrices [81E + 36B]
     double *R;             // residual / rhs [3N]
     double *sol;           // [3N]
-    double *diag, *offd;   // [9N], [9 nnz]
+    double *diag, *offd;   // [9N] node-major, [9 nsell] SELL-64 values
     double *scale;         // [3N]
     double *dinv;          // [9N]
     // FP32 system for mixed precision
@@ -82,7 +95,14 @@ struct Ctx {
     ElemData ed;
 };
 
+// value index of entry e (0..8) of SELL slot q
+__host__ __device__ __forceinline__ size_t sell_vidx(size_t q, int e) {
+    size_t lane = q & 63;
+    return 9 * (q - lane) + 64 * (size_t)e + lane;
+}
+
 // launchers (sw2_kernels.hip / sw2_solver.hip)
+void launch_old_terms(Ctx *c);
 void launch_resid(Ctx *c);
 void launch_jacobian(Ctx *c);
 void launch_assemble(Ctx *c);

@@ -70,6 +70,7 @@ struct ElemData {
     const int *src_flag;
     const double *src;
     const int *wd_flag;
+    const double *pre;         // [PRE_STRIDE * E] old/older-state terms (elem_old_terms)
     // boundary edges
     const int *e1d;            // [2B]
     const double *djac1d;      // [B]
@@ -275,11 +276,16 @@ HDF void wd_apply(const v3 *x, const double *h, const v2 *gp, v2 vconst, const v
     }
 }
 
+// Register-resident selection of element [k] of a 3-array by a runtime index
+// (avoids dynamically indexed private arrays, which would live in scratch).
+template <typename T>
+HDF T pick3(const T *a, int k) { return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]); }
+
 // fe_sw2_wet_dry_wrapper (src/fe/fe_sw2_wet_dry_wrapper.c:106-306).  Flags
 // select which optional inputs exist (grad_phi, v_wet_dry, f_wet_dry).
 template <int FN, bool HAS_GRAD, bool HAS_VWD, bool HAS_FWD, bool REDIST>
-HD double wd_wrapper(d3 *elem_rhs, const v3 *x, const double *h, const v2 *grad_phi, v2 vconst, const v2 *v_wet_dry,
-                     const double *f_wet_dry, double djac, double dt, double c) {
+HDF double wd_wrapper(d3 *elem_rhs, const v3 *x, const double *h, const v2 *grad_phi, v2 vconst, const v2 *v_wet_dry,
+                      const double *f_wet_dry, double djac, double dt, double c) {
     int total_vtx = 0, passcode = 0;
     for (int i = 0; i < 3; i++) {
         int is_dry = (h[i] <= 0.0) ? 1 : 0;
@@ -299,17 +305,22 @@ HD double wd_wrapper(d3 *elem_rhs, const v3 *x, const double *h, const v2 *grad_
     v3 nx[3];
     double nf[3] = {0.0, 0.0, 0.0};
     v2 nv[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    const double ho = pick3(h, only);
+    const v3 xo = pick3(x, only);
+    const v2 vo = HAS_VWD ? pick3(v_wet_dry, only) : v2{0.0, 0.0};
+    const double fo = HAS_FWD ? pick3(f_wet_dry, only) : 0.0;
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         if (i != only) {
-            double xsi = h[i] / (h[i] - h[only]);
-            nx[i].x = x[i].x + xsi * (x[only].x - x[i].x);   // FIXED_POS macro.h:756
-            nx[i].y = x[i].y + xsi * (x[only].y - x[i].y);
+            double xsi = h[i] / (h[i] - ho);
+            nx[i].x = x[i].x + xsi * (xo.x - x[i].x);   // FIXED_POS macro.h:756
+            nx[i].y = x[i].y + xsi * (xo.y - x[i].y);
             nx[i].z = x[i].z;
             if (HAS_VWD) {
-                nv[i].x = v_wet_dry[i].x + xsi * (v_wet_dry[only].x - v_wet_dry[i].x);
-                nv[i].y = v_wet_dry[i].y + xsi * (v_wet_dry[only].y - v_wet_dry[i].y);
+                nv[i].x = v_wet_dry[i].x + xsi * (vo.x - v_wet_dry[i].x);
+                nv[i].y = v_wet_dry[i].y + xsi * (vo.y - v_wet_dry[i].y);
             }
-            if (HAS_FWD) nf[i] = f_wet_dry[i] + xsi * (f_wet_dry[only] - f_wet_dry[i]);
+            if (HAS_FWD) nf[i] = f_wet_dry[i] + xsi * (fo - f_wet_dry[i]);
         } else {
             nx[i] = x[i];
             new_h[i] = h[i];
@@ -337,19 +348,24 @@ HD double wd_wrapper(d3 *elem_rhs, const v3 *x, const double *h, const v2 *grad_
     wd_apply<FN>(nx, new_h, ng, vconst, HAS_VWD ? nv : nullptr, nf, djac, dt, c, nr);
     double factor = (total_vtx >> 1) - (total_vtx % 2);
     if (REDIST) {  // RE_DISTRIBUTE macro.h:789-805
+        d3 no = pick3(nr, only);
+#pragma unroll
         for (int i = 0; i < 3; i++) {
             if (i == only) continue;
-            double a0x = nx[i].x - x[only].x, a0y = nx[i].y - x[only].y;
-            double a1x = x[i].x - x[only].x, a1y = x[i].y - x[only].y;
+            double a0x = nx[i].x - xo.x, a0y = nx[i].y - xo.y;
+            double a1x = x[i].x - xo.x, a1y = x[i].y - xo.y;
             double l0 = a0x * a0x + a0y * a0y, l1 = a1x * a1x + a1y * a1y;
             double xsi = sqrt(l0 / l1);
-            nr[only].c_eq += (1.0 - xsi) * nr[i].c_eq;
-            nr[only].x_eq += (1.0 - xsi) * nr[i].x_eq;
-            nr[only].y_eq += (1.0 - xsi) * nr[i].y_eq;
+            no.c_eq += (1.0 - xsi) * nr[i].c_eq;
+            no.x_eq += (1.0 - xsi) * nr[i].x_eq;
+            no.y_eq += (1.0 - xsi) * nr[i].y_eq;
             nr[i].c_eq *= xsi;
             nr[i].x_eq *= xsi;
             nr[i].y_eq *= xsi;
         }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if (i == only) nr[i] = no;
     }
     for (int i = 0; i < 3; i++) {
         elem_rhs[i].c_eq += factor * nr[i].c_eq;
@@ -373,11 +389,14 @@ HD double wd_factor(const v3 *x, const double *h, double djac) {
     if (total_vtx != 1) passcode = (7 - passcode);
     int only = passcode / 2;
     v2 nx[3];
+    const double ho = pick3(h, only);
+    const v3 xo = pick3(x, only);
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         if (i != only) {
-            double xsi = h[i] / (h[i] - h[only]);
-            nx[i].x = x[i].x + xsi * (x[only].x - x[i].x);
-            nx[i].y = x[i].y + xsi * (x[only].y - x[i].y);
+            double xsi = h[i] / (h[i] - ho);
+            nx[i].x = x[i].x + xsi * (xo.x - x[i].x);
+            nx[i].y = x[i].y + xsi * (xo.y - x[i].y);
         } else {
             nx[i].x = x[i].x;
             nx[i].y = x[i].y;
@@ -430,13 +449,76 @@ HDF void sw2_temporal(const v3 *nds, double djac, double dll, const double *h, c
     }
 }
 
+// Per-element terms of the body residual that depend only on the old and
+// older states (fixed during a Newton solve), computed once per solve with
+// the same code and consumed by body_resid in the reference's order.
+// Layout of the 24 doubles per triangle:
+//   [0..8]   t(n) temporal contribution: continuity c0 c1 c2, x0 x1 x2, y0 y1 y2
+//   [9..17]  t(n-1) temporal contribution, same layout
+//   [18]     wet/dry factor of the old depths (SUPG length scale)
+//   [19..21] minus the wet/dry-averaged old u, v, h integrals (SUPG d/dt seeds)
+constexpr int PRE_STRIDE = 24;
+
+HDF void sw2_temporal_terms(const v3 *nds, double djac, double dll, const double *h, const v2 *vel, double wdf,
+                            double dt_factor, double *out9) {
+    double con = dt_factor * wdf;
+    d3 rhs[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    double uh[3], vh[3], rx[3] = {0., 0., 0.}, ry[3] = {0., 0., 0.};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        uh[i] = vel[i].x * dmax(h[i], dll);
+        vh[i] = vel[i].y * dmax(h[i], dll);
+    }
+    wd_wrapper<WD_CONT_TEMPORAL, false, false, false, true>(rhs, nds, h, nullptr, v2{0., 0.}, nullptr, nullptr, djac,
+                                                             1.0, dt_factor);
+    int_tri_phi_f_lump(djac, con, uh, rx);
+    int_tri_phi_f_lump(djac, con, vh, ry);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        out9[i] = rhs[i].c_eq;
+        out9[3 + i] = rx[i];
+        out9[6 + i] = ry[i];
+    }
+}
+
+HD void elem_old_terms(const ElemData &D, int ie, double *pre) {
+    const int nd[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
+    const double djac = D.djac[ie], alpha = D.alpha;
+    v3 nds[3];
+    double hold[3], holder[3];
+    v2 vold[3], volder[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        int n = nd[i];
+        nds[i].x = D.x[n]; nds[i].y = D.y[n]; nds[i].z = D.z[n];
+        hold[i] = D.old_head[n];
+        holder[i] = D.older_head[n];
+        vold[i].x = D.old_u[n]; vold[i].y = D.old_v[n];
+        volder[i].x = D.older_u[n]; volder[i].y = D.older_v[n];
+    }
+    const double factor_old = wd_factor(nds, hold, djac);
+    const double factor_older = wd_factor(nds, holder, djac);
+    sw2_temporal_terms(nds, djac, D.dll, hold, vold, factor_old, (-1.) * (1. + alpha), pre);
+    sw2_temporal_terms(nds, djac, D.dll, holder, volder, factor_older, (alpha / 2.), pre + 9);
+    pre[18] = factor_old;
+    d3 dfdt[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    wd_wrapper<WD_INTEGRATE_F, false, true, true, false>(dfdt, nds, hold, nullptr, v2{0., 0.}, vold, hold, djac, 1, 1);
+    pre[19] = -dfdt[0].x_eq;
+    pre[20] = -dfdt[0].y_eq;
+    pre[21] = -dfdt[0].c_eq;
+    pre[22] = 0.;
+    pre[23] = 0.;
+}
+
 // ------------------------------------------------------------ body residual
 // fe_sw2_body_resid (src/fe/sw2/fe_sw2_body_resid.c:45-1267), triangles.
 // dry_rule: apply the reference's in-loop zeroing of dry-node velocities
 // (fe_sw2_resid.c:85-93): a node with old_head <= 0 keeps its velocity only
-// for the lowest-index element that touches it.
-HD void body_resid(const ElemData &D, int ie, double pert, int pnode, int pvar, int psign, bool dry_rule,
-                   d3 *elem_rhs) {
+// for the lowest-index element that touches it.  Old/older-state terms come
+// from `pre` (elem_old_terms); the reference's wet/dry average of the old
+// state (:248) feeds only debug output and is not formed.
+HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, int pnode, int pvar, int psign,
+                   bool dry_rule, d3 *elem_rhs) {
     int PRESSURE_FLAG = 1;
     const int nd[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
     v2 gs[3];
@@ -451,29 +533,25 @@ HD void body_resid(const ElemData &D, int ie, double pert, int pnode, int pvar, 
 
     double node_z[3];
     v3 nds[3];
-    double h[3], hold[3], holder[3];
-    v2 vel[3], vold[3], volder[3];
+    double h[3];
+    v2 vel[3];
+#pragma unroll
     for (int i = 0; i < 3; i++) {
         int n = nd[i];
         nds[i].x = D.x[n]; nds[i].y = D.y[n]; nds[i].z = D.z[n];
         node_z[i] = nds[i].z;
         h[i] = D.head[n];
-        hold[i] = D.old_head[n];
-        holder[i] = D.older_head[n];
         double un = D.u[n], vn = D.v[n];
-        if (dry_rule && hold[i] <= 0. && D.first_elem[n] != ie) { un = 0.; vn = 0.; }
+        if (dry_rule && D.old_head[n] <= 0. && D.first_elem[n] != ie) { un = 0.; vn = 0.; }
         vel[i].x = un; vel[i].y = vn;
-        vold[i].x = D.old_u[n]; vold[i].y = D.old_v[n];
-        volder[i].x = D.older_u[n]; volder[i].y = D.older_v[n];
     }
-    if (pvar == PERTURB_H) {
-        h[pnode] += psign * pert;
-    } else if (pvar == PERTURB_U) {
-        PRESSURE_FLAG = 0;
-        vel[pnode].x += psign * pert;
-    } else if (pvar == PERTURB_V) {
-        PRESSURE_FLAG = 0;
-        vel[pnode].y += psign * pert;
+    if (pvar == PERTURB_U || pvar == PERTURB_V) PRESSURE_FLAG = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (i != pnode) continue;
+        if (pvar == PERTURB_H) h[i] += psign * pert;
+        else if (pvar == PERTURB_U) vel[i].x += psign * pert;
+        else if (pvar == PERTURB_V) vel[i].y += psign * pert;
     }
     // baroclinic density and lid pressures are zero for the supported decks
     const double elem_density[3] = {0., 0., 0.};
@@ -499,14 +577,12 @@ HD void body_resid(const ElemData &D, int ie, double pert, int pnode, int pvar, 
     const double vst_i = elem_u_avg / dmax(vmag, 1.E-8);
     const double vst_j = elem_v_avg / dmax(vmag, 1.E-8);
 
-    double factor, factor_old, factor_older;
-    double h_wd = elem_h_avg, hold_wd;
-    v2 vwd, vold_wd;
+    double factor;
+    const double factor_old = pre[18];
+    double h_wd = elem_h_avg;
+    v2 vwd;
     wd_average(nds, h, vel, djac, &h_wd, &vwd);
-    wd_average(nds, hold, vold, djac, &hold_wd, &vold_wd);
     factor = wd_factor(nds, h, djac);
-    factor_old = wd_factor(nds, hold, djac);
-    factor_older = wd_factor(nds, holder, djac);
     const double vwd_mag = sqrt(vwd.x * vwd.x + vwd.y * vwd.y);
 
     const double h_fric = dmax(h_wd, SMALL);
@@ -583,8 +659,16 @@ HD void body_resid(const ElemData &D, int ie, double pert, int pnode, int pvar, 
 
     // temporal :480-490
     sw2_temporal(nds, djac, dll, h, vel, factor, (1. + alpha / 2.), elem_rhs, -3);
-    sw2_temporal(nds, djac, dll, hold, vold, factor_old, (-1.) * (1. + alpha), elem_rhs, wd_flag);
-    sw2_temporal(nds, djac, dll, holder, volder, factor_older, (alpha / 2.), elem_rhs, wd_flag);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {   // t(n) and t(n-1) terms, fe_sw2_temporal with the element's wd_flag
+        const double *t = pre + 9 * k;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (wd_flag != 2) elem_rhs[i].c_eq += t[i];
+            elem_rhs[i].x_eq += t[3 + i];
+            elem_rhs[i].y_eq += t[6 + i];
+        }
+    }
 
     // diffusion :504-539
     {
@@ -691,10 +775,7 @@ HD void body_resid(const ElemData &D, int ie, double pert, int pnode, int pvar, 
         }
         d3 dfdt[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
         d3 adv[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-        factor_old = wd_wrapper<WD_INTEGRATE_F, false, true, true, false>(dfdt, nds, hold, nullptr, v2{0., 0.}, vold, hold,
-                                                                           djac, 1, 1);
-        double dudt = -dfdt[0].x_eq, dvdt = -dfdt[0].y_eq, dhdt = -dfdt[0].c_eq;
-        for (int i = 0; i < 3; i++) { dfdt[i].x_eq = 0.; dfdt[i].y_eq = 0.; dfdt[i].c_eq = 0.; }
+        double dudt = pre[19], dvdt = pre[20], dhdt = pre[21];   // -(old-state integrals), :1012-1013
         factor = wd_wrapper<WD_INTEGRATE_F, false, true, true, false>(dfdt, nds, h, nullptr, v2{0., 0.}, vel, h, djac, 1, 1);
         dudt += dfdt[0].x_eq; dvdt += dfdt[0].y_eq; dhdt += dfdt[0].c_eq;
         factor = wd_wrapper<WD_GLS_CONVECTION, true, true, true, false>(adv, nds, h, gs, v2{0., 0.}, vel, h, djac, dt, 1);
@@ -755,9 +836,14 @@ HD void boundary_resid(const ElemData &D, int ie, double pert, int pnode, int pv
         if (dry_zero && D.old_head[n] <= 0.) { un = 0.; vn = 0.; }
         vel[i].x = un; vel[i].y = vn;
     }
-    if (pvar == PERTURB_H) h[pnode] += psign * pert;
-    if (pvar == PERTURB_U) { vel[pnode].x += psign * pert; PRESSURE_FLAG = 0; }
-    if (pvar == PERTURB_V) { vel[pnode].y += psign * pert; PRESSURE_FLAG = 0; }
+    if (pvar == PERTURB_U || pvar == PERTURB_V) PRESSURE_FLAG = 0;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        if (i != pnode) continue;
+        if (pvar == PERTURB_H) h[i] += psign * pert;
+        if (pvar == PERTURB_U) vel[i].x += psign * pert;
+        if (pvar == PERTURB_V) vel[i].y += psign * pert;
+    }
     double u[2] = {vel[0].x, vel[1].x}, v[2] = {vel[0].y, vel[1].y};
     double h_avg = one_2 * (h[0] + h[1]);
     if (h_avg < SMALL) h_avg = SMALL;
@@ -857,8 +943,9 @@ HD void jac_column(const ElemData &D, int ie, int j, int var, double *col) {
     int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
     d3 P[3], M[3];
     if (NN == 3) {
-        body_resid(D, ie, eps, j, pv, +1, false, P);
-        body_resid(D, ie, eps, j, pv, -1, false, M);
+        const double *pre = D.pre + PRE_STRIDE * (size_t)ie;
+        body_resid(D, pre, ie, eps, j, pv, +1, false, P);
+        body_resid(D, pre, ie, eps, j, pv, -1, false, M);
     } else {
         boundary_resid(D, ie, eps, j, pv, +1, false, P);
         boundary_resid(D, ie, eps, j, pv, -1, false, M);
@@ -877,33 +964,34 @@ HD void jac_column(const ElemData &D, int ie, int j, int var, double *col) {
 // the node-string flags come from nodes 0..NN-1 of the mesh; `loc_bc` holds them.
 template <int NN>
 HDF void jac_column_db(const ElemData &D, const int *nodes, const int *loc_bc, int j, int var, double *col) {
+#pragma unroll
     for (int k = 0; k < NN; k++) {
-        bool dry = D.old_head[nodes[k]] <= 0.;
-        int nb = loc_bc[k];
-        // sequence of edits for local node k, applied in the reference's order
-        int edits[3];
-        int ne = 0;
-        if (dry) edits[ne++] = 0;                     // mom
-        if (nb == BCT_PRS_DIR) edits[ne++] = 1;       // dacont
-        else if (nb == BCT_VEL_DIR) edits[ne++] = 0;
-        else if (nb == BCT_VEL_PRS_DIR) { edits[ne++] = 1; edits[ne++] = 0; }
-        for (int q = 0; q < ne; q++) {
-            if (edits[q] == 0) {  // mom: zero rows k (x,y) all cols; zero cols k (u,v) all rows; unit diag
-                col[3 * k + 0] = 0.;
-                col[3 * k + 1] = 0.;
-                if (j == k && var < 2) {
-                    for (int i = 0; i < NN; i++) { col[3 * i] = 0.; col[3 * i + 1] = 0.; col[3 * i + 2] = 0.; }
-                    if (var == 0) col[3 * k + 0] = 1.;
-                    else col[3 * k + 1] = 1.;
-                }
-            } else {              // dacont: zero row k (c) all cols; zero col k (h) all rows; unit diag
-                col[3 * k + 2] = 0.;
-                if (j == k && var == 2) {
-                    for (int i = 0; i < NN; i++) { col[3 * i] = 0.; col[3 * i + 1] = 0.; col[3 * i + 2] = 0.; }
-                    col[3 * k + 2] = 1.;
-                }
+        // mom: zero rows k (x,y) in all columns, zero columns (k,u),(k,v), unit diagonal
+        auto mom = [&]() {
+            col[3 * k + 0] = 0.;
+            col[3 * k + 1] = 0.;
+            if (j == k && var < 2) {
+#pragma unroll
+                for (int i = 0; i < NN; i++) { col[3 * i] = 0.; col[3 * i + 1] = 0.; col[3 * i + 2] = 0.; }
+                if (var == 0) col[3 * k + 0] = 1.;
+                else col[3 * k + 1] = 1.;
             }
-        }
+        };
+        // dacont: zero row (k,c) in all columns, zero column (k,h), unit diagonal
+        auto dacont = [&]() {
+            col[3 * k + 2] = 0.;
+            if (j == k && var == 2) {
+#pragma unroll
+                for (int i = 0; i < NN; i++) { col[3 * i] = 0.; col[3 * i + 1] = 0.; col[3 * i + 2] = 0.; }
+                col[3 * k + 2] = 1.;
+            }
+        };
+        // the reference's order of edits for local node k
+        if (D.old_head[nodes[k]] <= 0.) mom();
+        const int nb = loc_bc[k];
+        if (nb == BCT_PRS_DIR) dacont();
+        else if (nb == BCT_VEL_DIR) mom();
+        else if (nb == BCT_VEL_PRS_DIR) { dacont(); mom(); }
     }
 }
 

@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(128) k_resid2d(ElemData D, int E, double *ER) 
     int ie = blockIdx.x * blockDim.x + threadIdx.x;
     if (ie >= E) return;
     d3 r[3];
-    body_resid(D, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
+    body_resid(D, D.pre + PRE_STRIDE * (size_t)ie, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
     double *o = ER + 9 * (size_t)ie;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -78,6 +78,7 @@ __global__ void k_resid_gather(int N, const int *n2e_ptr, const int *n2e, const 
 }
 
 void launch_resid(Ctx *c) {
+    if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = c->first_elem;
     if (c->E) hipLaunchKernelGGL(k_resid2d, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->ER);
@@ -123,6 +124,7 @@ __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1
 }
 
 void launch_jacobian(Ctx *c) {
+    if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = nullptr;
     if (c->E)
@@ -135,15 +137,15 @@ void launch_jacobian(Ctx *c) {
 
 // Ordered block assembly: thread per (slot, entry).  slot_src lists 9-double
 // element-block offsets in EM in ascending element order, 2D before 1D.
-__global__ void k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_src, const double *EM,
-                           double *diag, double *offd) {
+__global__ void k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_src, const int *offd_pos,
+                           const double *EM, double *diag, double *offd) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nslots * 9) return;
     int s = (int)(t / 9), k = (int)(t % 9);
     double a = 0.0;
     for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) a += EM[9 * (size_t)slot_src[p] + k];
     if (s < N) diag[9 * (size_t)s + k] = a;
-    else offd[9 * (size_t)(s - N) + k] = a;
+    else offd[sell_vidx((size_t)offd_pos[s - N], k)] = a;
 }
 
 // diagonal floor and residual tweak, fe_sw2_load.c:218-236
@@ -166,12 +168,13 @@ __global__ void k_diag_floor(int N, double *diag, double *R, const double *u, co
 void launch_assemble(Ctx *c) {
     int nslots = c->N + c->nnz;
     hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(9 * nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
-                       c->slot_ptr, c->slot_src, c->EM, c->diag, c->offd);
+                       c->slot_ptr, c->slot_src, c->offd_pos, c->EM, c->diag, c->offd);
     hipLaunchKernelGGL(k_diag_floor, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->diag, c->R,
                        c->u, c->v);
 }
 
 void launch_jac2d_only(Ctx *c) {
+    if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = nullptr;
     hipLaunchKernelGGL(k_jac2d, dim3(nblocks_for(9 * c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->loc_bc[0],
@@ -179,6 +182,7 @@ void launch_jac2d_only(Ctx *c) {
 }
 
 void launch_resid2d_only(Ctx *c) {
+    if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = c->first_elem;
     hipLaunchKernelGGL(k_resid2d, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->ER);
@@ -187,7 +191,7 @@ void launch_resid2d_only(Ctx *c) {
 void launch_assemble_only(Ctx *c) {
     int nslots = c->N + c->nnz;
     hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(9 * nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
-                       c->slot_ptr, c->slot_src, c->EM, c->diag, c->offd);
+                       c->slot_ptr, c->slot_src, c->offd_pos, c->EM, c->diag, c->offd);
 }
 
 // ------------------------------------------------------- init / wet-dry flags
@@ -235,6 +239,20 @@ __global__ void k_wd_flag(int E, const int *e2d, const double *head, const doubl
     wd_flag[e] = f;
 }
 
+// old/older-state element terms, once per Newton solve (sw2_elem.h elem_old_terms)
+__global__ void __launch_bounds__(128) k_old_terms(ElemData D, int E, double *pre) {
+    int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= E) return;
+    elem_old_terms(D, ie, pre + PRE_STRIDE * (size_t)ie);
+}
+
+void launch_old_terms(Ctx *c) {
+    if (c->E)
+        hipLaunchKernelGGL(k_old_terms, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, c->ed, c->E, c->pre);
+    c->pre_valid = 1;
+}
+
+
 void launch_init(Ctx *c, int *bc_mask_dev) {
     hipLaunchKernelGGL(k_init_nodes, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->old_head,
                        c->old_u, c->old_v, c->head, c->u, c->v, c->node_bc, c->node_bc_val, bc_mask_dev, c->wd_tmp);
@@ -244,6 +262,7 @@ void launch_init(Ctx *c, int *bc_mask_dev) {
         hipLaunchKernelGGL(k_wd_flag, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
                            c->old_head, c->wd_tmp, c->wd_flag);
     }
+    launch_old_terms(c);
 }
 
 // fe_sw2 (fe_sw2.c:42-50)
@@ -256,6 +275,7 @@ __global__ void k_shift(int N, double *h, double *u, double *v, double *oh, doub
 }
 
 void launch_shift(Ctx *c) {
+    c->pre_valid = 0;
     hipLaunchKernelGGL(k_shift, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->head, c->u, c->v,
                        c->old_head, c->old_u, c->old_v, c->older_head, c->older_u, c->older_v);
 }

@@ -26,7 +26,8 @@ constexpr double SOLV_TOL = SMALL;  // src/include/define.h:177
 
 // ------------------------------------------------------------- Jacobi scaling
 // spv_jacobi 3-DOF branch (src/solver/spv_jacobi.c:222-316)
-__global__ void k_scale_rows(int N, const double *diag, const double *offd, const int *row_ptr, double *sv) {
+__global__ void k_scale_rows(int N, const double *diag, const double *offd, const int *row_ptr, const int *sell_ptr,
+                             double *sv) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const double *D = diag + 9 * (size_t)n;
@@ -36,18 +37,22 @@ __global__ void k_scale_rows(int N, const double *diag, const double *offd, cons
         for (int k = 0; k < 3; k++) if (fabs(D[3 * j + k]) > m) m = fabs(D[3 * j + k]);
         s[j] = m;
     }
-    for (int b = row_ptr[n]; b < row_ptr[n + 1]; b++) {
-        const double *a = offd + 9 * (size_t)b;
+    const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
+    const int len = row_ptr[n + 1] - row_ptr[n];
+    for (int b = 0; b < len; b++) {
+        const size_t q = q0 + 64 * (size_t)b;
         for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++)
-                if (fabs(a[3 * j + k]) > s[j]) s[j] = fabs(a[3 * j + k]);
+            for (int k = 0; k < 3; k++) {
+                double a = fabs(offd[sell_vidx(q, 3 * j + k)]);
+                if (a > s[j]) s[j] = a;
+            }
     }
     const double small_factor = 1.0 / sqrt(SOLV_TOL);
     for (int j = 0; j < 3; j++) sv[3 * n + j] = s[j] < SOLV_TOL ? small_factor : 1.0 / sqrt(fabs(s[j]));
 }
 
-__global__ void k_scale_apply(int N, double *diag, double *offd, const int *row_ptr, const int *col, const double *sv,
-                              double *R, double *sol) {
+__global__ void k_scale_apply(int N, double *diag, double *offd, const int *row_ptr, const int *sell_ptr,
+                              const int *sell_col, const double *sv, double *R, double *sol) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double rf[3] = {sv[3 * n], sv[3 * n + 1], sv[3 * n + 2]};
@@ -58,11 +63,13 @@ __global__ void k_scale_apply(int N, double *diag, double *offd, const int *row_
     double *D = diag + 9 * (size_t)n;
     for (int j = 0; j < 3; j++)
         for (int k = 0; k < 3; k++) D[3 * j + k] *= rf[j] * rf[k];
-    for (int b = row_ptr[n]; b < row_ptr[n + 1]; b++) {
-        double *a = offd + 9 * (size_t)b;
-        const double *sc = sv + 3 * (size_t)col[b];
+    const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
+    const int len = row_ptr[n + 1] - row_ptr[n];
+    for (int b = 0; b < len; b++) {
+        const size_t q = q0 + 64 * (size_t)b;
+        const double *sc = sv + 3 * (size_t)sell_col[q];
         for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) a[3 * j + k] *= rf[j] * sc[k];
+            for (int k = 0; k < 3; k++) offd[sell_vidx(q, 3 * j + k)] *= rf[j] * sc[k];
     }
 }
 
@@ -105,13 +112,14 @@ __global__ void k_to_f32(size_t n, const double *a, float *b) {
 
 void launch_setup(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
-    hipLaunchKernelGGL(k_scale_rows, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr, c->scale);
-    hipLaunchKernelGGL(k_scale_apply, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr, c->col,
-                       c->scale, c->R, c->sol);
+    hipLaunchKernelGGL(k_scale_rows, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
+                       c->sell_ptr, c->scale);
+    hipLaunchKernelGGL(k_scale_apply, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
+                       c->sell_ptr, c->sell_col, c->scale, c->R, c->sol);
     bool f32 = c->opts.precision == 1;
     hipLaunchKernelGGL(k_bj_setup, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->dinv, f32 ? c->dinv32 : nullptr);
     if (f32) {
-        size_t n9 = 9 * (size_t)c->N, m9 = 9 * (size_t)c->nnz;
+        size_t n9 = 9 * (size_t)c->N, m9 = 9 * c->nsell;
         hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for((int)n9, 256)), dim3(256), 0, c->stream, n9, c->diag, c->diag32);
         if (m9)
             hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for((int)m9, 256)), dim3(256), 0, c->stream, m9, c->offd,
@@ -188,23 +196,28 @@ __device__ __forceinline__ bool reduce_last(double (&v)[K], const int (&op)[K], 
 // y = M^-1 (A x) for block row n; A = diag + offd in the reference's
 // accumulation order (solv_amult.c:99-123: diagonal block, then the row's
 // blocks in insertion order summed into a separate partial).
+// Off-diagonal blocks come from the SELL-64 slice of this row's wavefront:
+// every lane runs the slice's K block slots (zero padding past the row's
+// length), and entry e of slot j is one coalesced 64-value read per wave.
 template <typename T, typename XF>
-__device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *offd, const int *row_ptr, const int *col,
-                                               const T *dinv, XF xf, T (&y)[3]) {
+__device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *offd, const int *sell_ptr,
+                                               const int *sell_col, const T *dinv, XF xf, T (&y)[3]) {
     const T *D = diag + 9 * (size_t)n;
     T x0 = xf(3 * n), x1 = xf(3 * n + 1), x2 = xf(3 * n + 2);
     T a0 = D[0] * x0 + D[1] * x1 + D[2] * x2;
     T a1 = D[3] * x0 + D[4] * x1 + D[5] * x2;
     T a2 = D[6] * x0 + D[7] * x1 + D[8] * x2;
     T p0 = 0, p1 = 0, p2 = 0;
-    int b0 = row_ptr[n], b1 = row_ptr[n + 1];
-    for (int b = b0; b < b1; b++) {
-        const T *a = offd + 9 * (size_t)b;
-        int cc = 3 * col[b];
+    const int s = n >> 6, lane = n & 63;
+    const int k0 = sell_ptr[s], k1 = sell_ptr[s + 1];
+    for (int b = k0; b < k1; b++) {
+        const size_t q = 64 * (size_t)b + lane;
+        const T *a = offd + 9 * 64 * (size_t)b + lane;
+        int cc = 3 * sell_col[q];
         T y0 = xf(cc), y1 = xf(cc + 1), y2 = xf(cc + 2);
-        p0 += a[0] * y0 + a[1] * y1 + a[2] * y2;
-        p1 += a[3] * y0 + a[4] * y1 + a[5] * y2;
-        p2 += a[6] * y0 + a[7] * y1 + a[8] * y2;
+        p0 += a[0] * y0 + a[64] * y1 + a[128] * y2;
+        p1 += a[192] * y0 + a[256] * y1 + a[320] * y2;
+        p2 += a[384] * y0 + a[448] * y1 + a[512] * y2;
     }
     a0 += p0; a1 += p1; a2 += p2;
     const T *M = dinv + 9 * (size_t)n;
@@ -257,13 +270,13 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<T> V, const double 
 
 // prologue part 2: r = b' - M^-1 A u, q = r, p = r, rho = (r,q), ||r||inf  (:138-188)
 template <typename T>
-__global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *row_ptr, const int *col, double *part,
+__global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (n < N) {
         T y[3];
-        amult_prec_row<T>(n, V.diag, V.offd, row_ptr, col, V.dinv, VecX<T>{V.u}, y);
+        amult_prec_row<T>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.u}, y);
         for (int j = 0; j < 3; j++) {
             T r = V.b[3 * n + j] + (T)(-1.0) * y[j];
             V.r[3 * n + j] = r;
@@ -288,14 +301,14 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *ro
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *row_ptr, const int *col, double *part,
+__global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < N) {
         T y[3];
-        amult_prec_row<T>(n, V.diag, V.offd, row_ptr, col, V.dinv, VecX<T>{V.p}, y);
+        amult_prec_row<T>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.p}, y);
         for (int j = 0; j < 3; j++) {
             V.Ap[3 * n + j] = y[j];
             v[0] += (double)V.q[3 * n + j] * (double)y[j];
@@ -310,7 +323,7 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *ro
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *row_ptr, const int *col, double *part,
+__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,7 +332,7 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *ro
     if (n < N) {
         VecS<T> sf{V.r, V.Ap, nalpha};
         T y[3];
-        amult_prec_row<T>(n, V.diag, V.offd, row_ptr, col, V.dinv, sf, y);
+        amult_prec_row<T>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, sf, y);
         for (int j = 0; j < 3; j++) {
             T s = sf(3 * n + j);
             V.s[3 * n + j] = s;
@@ -401,7 +414,7 @@ static void prologue(Ctx *c) {
     KVecs<T> V = vecs<T>(c);
     hipLaunchKernelGGL(k_bcg_p1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, c->red.count + 0,
                        c->ksc);
-    hipLaunchKernelGGL(k_bcg_p2<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->row_ptr, c->col, c->red.part,
+    hipLaunchKernelGGL(k_bcg_p2<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col, c->red.part,
                        c->red.count + 1, c->ksc);
 }
 
@@ -410,9 +423,9 @@ static void iters(Ctx *c, int k) {
     int nb = nblocks_for(c->N, 256);
     KVecs<T> V = vecs<T>(c);
     for (int i = 0; i < k; i++) {
-        hipLaunchKernelGGL(k_bcg_k1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->row_ptr, c->col, c->red.part,
+        hipLaunchKernelGGL(k_bcg_k1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col, c->red.part,
                            c->red.count + 2, c->ksc);
-        hipLaunchKernelGGL(k_bcg_k2<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->row_ptr, c->col, c->red.part,
+        hipLaunchKernelGGL(k_bcg_k2<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col, c->red.part,
                            c->red.count + 3, c->ksc);
         hipLaunchKernelGGL(k_bcg_k3<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 4,
                            c->ksc);
@@ -422,11 +435,11 @@ static void iters(Ctx *c, int k) {
 void launch_k1_only(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     if (c->opts.precision == 1)
-        hipLaunchKernelGGL(k_bcg_k1<float>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<float>(c), c->row_ptr,
-                           c->col, c->red.part, c->red.count + 2, c->ksc);
+        hipLaunchKernelGGL(k_bcg_k1<float>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<float>(c), c->sell_ptr,
+                           c->sell_col, c->red.part, c->red.count + 2, c->ksc);
     else
-        hipLaunchKernelGGL(k_bcg_k1<double>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<double>(c), c->row_ptr,
-                           c->col, c->red.part, c->red.count + 2, c->ksc);
+        hipLaunchKernelGGL(k_bcg_k1<double>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<double>(c), c->sell_ptr,
+                           c->sell_col, c->red.part, c->red.count + 2, c->ksc);
 }
 
 void launch_krylov_prologue(Ctx *c) {

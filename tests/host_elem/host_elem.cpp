@@ -2,6 +2,7 @@
 // for the host, so the CPU suite can compare it with the oracle bit for bit
 // before any GPU run.  Not part of the product library.
 #include <cstring>
+#include <vector>
 #include "../../adhv8_kraken_amd/csrc/sw2_elem.h"
 
 using namespace sw2;
@@ -13,6 +14,7 @@ extern "C" {
 
 struct HostElem {
     ElemData D;
+    std::vector<double> pre;
 };
 
 void *he_create(const double *x, const double *y, const double *z, const double *head, const double *u,
@@ -22,7 +24,7 @@ void *he_create(const double *x, const double *y, const double *z, const double 
                 const double *src, const int *wd_flag, const int *e1d, const double *djac1d, const double *nrml1d,
                 const int *bc1d, const double *val1d, const int *mat_turb, const int *mat_eev_mode,
                 const double *mat_eev_coef, const double *mat_ev, const double *mat_cor, double g, double dt,
-                double alpha, double tau_pg, double dll, double dul, double visc) {
+                double alpha, double tau_pg, double dll, double dul, double visc, int nelems) {
     if (!ready) { crlog::make_tables(&T); ready = true; }
     HostElem *h = new HostElem();
     ElemData &D = h->D;
@@ -33,6 +35,9 @@ void *he_create(const double *x, const double *y, const double *z, const double 
     D.mat_turb = mat_turb; D.mat_eev_mode = mat_eev_mode; D.mat_eev_coef = mat_eev_coef; D.mat_ev = mat_ev;
     D.mat_cor = mat_cor; D.g = g; D.dt = dt; D.alpha = alpha; D.tau_pg = tau_pg; D.dll = dll; D.dul = dul;
     D.visc = visc; D.logt = &T;
+    h->pre.assign(PRE_STRIDE * (size_t)nelems, 0.0);
+    for (int ie = 0; ie < nelems; ie++) elem_old_terms(D, ie, h->pre.data() + PRE_STRIDE * (size_t)ie);
+    D.pre = h->pre.data();
     return h;
 }
 
@@ -40,7 +45,8 @@ void he_destroy(void *p) { delete (HostElem *)p; }
 
 void he_body(void *p, int ie, double pert, int pn, int pv, int ps, int dry_rule, double *out) {
     d3 r[3];
-    body_resid(((HostElem *)p)->D, ie, pert, pn, pv, ps, dry_rule != 0, r);
+    const ElemData &D = ((HostElem *)p)->D;
+    body_resid(D, D.pre + PRE_STRIDE * (size_t)ie, ie, pert, pn, pv, ps, dry_rule != 0, r);
     memcpy(out, r, sizeof(r));
 }
 

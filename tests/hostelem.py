@@ -27,7 +27,7 @@ def lib():
         L = C.CDLL(LIB)
         L.he_create.restype = C.c_void_p
         L.he_create.argtypes = [PD] * 12 + [PI, PI, PI, PD, PD, PI, PD, PI, PD, PI, PI, PD, PD, PI, PD, PI, PI, PD,
-                                            PD, PD] + [C.c_double] * 7
+                                            PD, PD] + [C.c_double] * 7 + [C.c_int]
         L.he_destroy.argtypes = [C.c_void_p]
         L.he_body.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
         L.he_boundary.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
@@ -67,7 +67,7 @@ class HostElem:
             args.append(a.ctypes.data_as(PD if a.dtype == np.float64 else PI))
         self.L = lib()
         self.h = self.L.he_create(*args, p.g, p.dt, p.tau_temporal, p.tau_pg, p.drying_lower, p.drying_upper,
-                                  p.viscosity)
+                                  p.viscosity, int(p.nelems2d))
         self.p = p
 
     def __del__(self):

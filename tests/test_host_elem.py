@@ -1,0 +1,79 @@
+"""The library's element code (csrc/sw2_elem.h) compiled for the host must
+reproduce the oracle's element residuals, boundary residuals and element
+Jacobians bit for bit, before any GPU run.  CPU only."""
+import numpy as np
+import pytest
+
+from adhv8_kraken_amd import mesh
+from hostelem import HostElem
+from oracle.oracle import Oracle, opts, set_log_mode
+
+
+def _stepped(p, newton_its=2):
+    o = Oracle(p)
+    o.shift()
+    o.init()
+    R0 = np.abs(o.resid()).max()
+    o.newton(opts(tol_nonlin=1e-30 * R0, max_nonlin_it=newton_its, prec=1, nblock=1))
+    # start the next step so old != older and the state is off the time stencil's fixed point
+    o.shift()
+    o.init()
+    o.newton(opts(tol_nonlin=1e-30 * R0, max_nonlin_it=1, prec=1, nblock=1))
+    return o
+
+
+def _compare(o, p, elems):
+    st = o.get_state()
+    wd = o.wdflag()
+    dj, gr, dj1, nr1 = o.geometry()
+    H = HostElem(p, st, wd, dj, gr, dj1, nr1)
+    bad = []
+    for ie in elems:
+        if not np.array_equal(o.body_resid(ie).view(np.int64), H.body(ie).view(np.int64)):
+            bad.append(("body", ie))
+        for pv in (1, 2, 5):
+            for pn in range(3):
+                for ps in (1, -1):
+                    a = o.body_resid(ie, 1e-7, pn, pv, ps)
+                    b = H.body(ie, 1e-7, pn, pv, ps)
+                    if not np.array_equal(a.view(np.int64), b.view(np.int64)):
+                        bad.append(("pert", ie, pn, pv, ps))
+        if not np.array_equal(o.elem_jacobian(ie).view(np.int64), H.elem_jac(ie).view(np.int64)):
+            bad.append(("jac", ie))
+    for ie in range(p.nelems1d):
+        for pv in (-1, 1, 2, 5):
+            pert = 1e-7 if pv > 0 else 0.0
+            a = o.boundary_resid(ie, pert, 0, pv, 1)
+            b = H.boundary(ie, pert, 0, pv, 1)
+            if not np.array_equal(a.view(np.int64), b.view(np.int64)):
+                bad.append(("edge", ie, pv))
+    return bad, wd
+
+
+@pytest.fixture(autouse=True)
+def _cr_log():
+    set_log_mode(1)
+
+
+def test_wet_channel_bit_exact():
+    p = mesh.channel_config1()
+    o = _stepped(p)
+    bad, _ = _compare(o, p, range(0, p.nelems2d, 3))
+    assert not bad, bad[:10]
+
+
+def test_wet_dry_floodplain_bit_exact():
+    """Partially wet triangles go through the wet/dry clipping wrapper and
+    shock capturing (fe_sw2_wet_dry_wrapper.c:106-306)."""
+    p = mesh.floodplain(20, 10, nsteps=1)
+    o = _stepped(p, newton_its=3)
+    bad, wd = _compare(o, p, range(p.nelems2d))
+    assert np.count_nonzero(wd) > 0
+    assert not bad, bad[:10]
+
+
+def test_bumpy_channel_varied_roughness_bit_exact():
+    p = mesh.bumpy_channel(30, 12, seed=9)
+    o = _stepped(p)
+    bad, _ = _compare(o, p, range(0, p.nelems2d, 2))
+    assert not bad, bad[:10]

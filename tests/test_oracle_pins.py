@@ -1,0 +1,306 @@
+"""Pins of the CPU oracle against what mathematics and physics fix.
+
+Each test checks one oracle function against an independent statement of
+the same quantity (closed form, quadrature, exact arithmetic, invariant or
+dense linear algebra), chosen so that a dropped term, a wrong sign or a
+transposed operand in the oracle fails it.  CPU only.
+"""
+import math
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.sparse
+import scipy.sparse.linalg
+
+from adhv8_kraken_amd import mesh
+from oracle.oracle import Oracle, lib, opts, set_log_mode
+
+
+def _blocks(o):
+    """Assembled system as scipy CSR (rows u,v,h per node; block entry (r,c)
+    at value index 3r+c, reference fe_assemble_matrix.c:200-208)."""
+    N = o.N
+    diag, offd = o.matrix()
+    rp, col = o.pattern()
+    rows, cols, vals = [], [], []
+    for i in range(N):
+        for r in range(3):
+            for c in range(3):
+                rows.append(3 * i + r)
+                cols.append(3 * i + c)
+                vals.append(diag[9 * i + 3 * r + c])
+        for p in range(rp[i], rp[i + 1]):
+            j = col[p]
+            for r in range(3):
+                for c in range(3):
+                    rows.append(3 * i + r)
+                    cols.append(3 * j + c)
+                    vals.append(offd[9 * p + 3 * r + c])
+    return scipy.sparse.csr_matrix((vals, (rows, cols)), shape=(3 * N, 3 * N))
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    return mesh.channel_config1()
+
+
+# ------------------------------------------------------------------ geometry
+def test_geometry_closed_forms():
+    """djac = triangle area, P1 gradients = rows of inv([1 x y]) (element_djac_nrml_gradPhi.c:193-274);
+    edge djac = length and unit outward normal (selem_1d.c:118-150)."""
+    p = mesh.bumpy_channel(12, 6, seed=11)
+    rng = np.random.default_rng(3)
+    p.x = p.x + rng.uniform(-2.0, 2.0, p.nnodes)  # skew every triangle
+    p.y = p.y + rng.uniform(-2.0, 2.0, p.nnodes)
+    o = Oracle(p)
+    dj, gr, dj1, nr1 = o.geometry()
+    e = p.e2d.reshape(-1, 3)
+    for ie in range(p.nelems2d):
+        X = np.stack([np.ones(3), p.x[e[ie]], p.y[e[ie]]], axis=1)
+        area = 0.5 * np.linalg.det(X)
+        assert dj[ie] == pytest.approx(area, rel=1e-12)
+        G = np.linalg.inv(X)  # phi_i = G[0,i] + G[1,i] x + G[2,i] y
+        for i in range(3):
+            assert gr[6 * ie + 2 * i] == pytest.approx(G[1, i], rel=1e-10, abs=1e-14)
+            assert gr[6 * ie + 2 * i + 1] == pytest.approx(G[2, i], rel=1e-10, abs=1e-14)
+    ed = p.e1d.reshape(-1, 2)
+    for b in range(p.nelems1d):
+        n0, n1 = ed[b]
+        L = math.hypot(p.x[n1] - p.x[n0], p.y[n1] - p.y[n0])
+        assert dj1[b] == pytest.approx(L, rel=1e-12)
+        nx, ny = nr1[2 * b], nr1[2 * b + 1]
+        assert nx * nx + ny * ny == pytest.approx(1.0, rel=1e-12)
+        assert nx * (p.x[n1] - p.x[n0]) + ny * (p.y[n1] - p.y[n0]) == pytest.approx(0.0, abs=1e-9)
+        adj = e[p.e1d_adj[b]]
+        cx, cy = p.x[adj].mean(), p.y[adj].mean()
+        mx, my = 0.5 * (p.x[n0] + p.x[n1]), 0.5 * (p.y[n0] + p.y[n1])
+        assert nx * (mx - cx) + ny * (my - cy) > 0.0
+
+
+# -------------------------------------------------------------- math helpers
+def _exact_log(x):
+    getcontext().prec = 50
+    return float(Decimal(x).ln())
+
+
+def test_correctly_rounded_log():
+    """Oracle log in correctly rounded mode equals ln computed in 50-digit
+    decimal arithmetic, rounded once to double."""
+    L = lib()
+    set_log_mode(1)
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([np.exp(rng.uniform(-30, 30, 3000)), 1.0 + rng.uniform(-1e-3, 1e-3, 1000),
+                         [1.0, 2.0, 0.5, 10.0, 1e-300, 1e300, np.nextafter(1.0, 2.0), np.nextafter(1.0, 0.0)]])
+    bad = [x for x in xs if L.o_log(float(x)) != _exact_log(float(x))]
+    assert not bad, bad[:5]
+
+
+def test_drag_coef_is_depth_averaged_log_law():
+    """fr_bedshstr_drag_coef.c:18-48 is c_f = 2 u*^2 / U^2 for the log profile
+    u(z) = (u*/0.4) ln(z/z0), z0 = ks/29.7, averaged over z0..z0+h; checked by
+    quadrature, including the clamp at ks > 29.7 h."""
+    L = lib()
+    set_log_mode(1)
+    for h, ks in [(1.0, 0.01), (0.3, 2e-4), (2.5, 0.5), (0.05, 0.9), (1.0, 1e-9), (1e-9, 0.1)]:
+        hh, kk = max(h, 1e-8), max(ks, 1e-8)
+        if kk > 29.7 * hh:
+            hh = kk / 29.7
+        z0 = kk / 29.7
+        U = scipy.integrate.quad(lambda z: math.log(z / z0) / 0.4, z0, z0 + hh, epsabs=0, epsrel=1e-13,
+                                 limit=200)[0] / hh
+        cf = 2.0 / (U * U)
+        assert L.o_drag_coef(h, ks) == pytest.approx(cf, rel=1e-10)
+
+
+# ------------------------------------------------------------ residual pins
+def test_lake_at_rest_balances_pressure_and_bed_slope():
+    """Still water over a bumpy bed: pressure, bed slope and boundary
+    pressure cancel (fe_sw2_body_resid.c:761-862), so the momentum residual
+    vanishes to roundoff away from the tailwater edge, whose pressure term
+    takes the bed of the edge's first node for both ends
+    (fe_sw2_boundary_resid.c:375-376)."""
+    p = mesh.bumpy_channel(40, 20, seed=7)
+    wse = 0.5
+    p.head0 = wse - p.z
+    p.e1d_val[:] = np.where(p.e1d_bc == mesh.BCT_VEL_NEU, 0.0, wse)
+    o = Oracle(p)
+    o.shift()
+    o.init()
+    R = o.resid()
+    east = p.x == p.x.max()
+    scale = 9.8 * np.abs(p.head0).max() ** 2 * 10.0
+    assert np.abs(R[0::3][~east]).max() < 1e-12 * scale
+    assert np.abs(R[1::3][~east]).max() < 1e-12 * scale
+    assert np.abs(R[2::3]).max() < 1e-12 * scale
+    # the tailwater-edge quirk leaves a momentum imbalance there on a sloped bed
+    assert np.abs(R[0::3][east]).max() > 1e-6 * scale
+
+
+def test_mass_balance_one_step(cfg1):
+    """After a converged step the stored volume change equals dt times the
+    net boundary inflow: the 10 m^3/s NB VEL inflow minus the tailwater
+    outflow integrated with the P1 trapezoid rule."""
+    p = cfg1
+    o = Oracle(p)
+    o.shift()
+    o.init()
+    R0 = o.resid()
+    res = o.newton(opts(tol_nonlin=1e-12 * np.abs(R0).max(), prec=1, nblock=1))
+    assert res["converged"]
+    h, v, oh, _, _, _ = o.get_state()
+    dj, gr, dj1, nr1 = o.geometry()
+    e = p.e2d.reshape(-1, 3)
+    vol = lambda hh: np.sum(dj * hh[e].sum(1) / 3.0)
+    ed = p.e1d.reshape(-1, 2)
+    qout = 0.0
+    for b in np.where(p.e1d_bc == mesh.BCT_PRS_NEU)[0]:
+        n0, n1 = ed[b]
+        nx, ny = nr1[2 * b], nr1[2 * b + 1]
+        f0 = h[n0] * (v[2 * n0] * nx + v[2 * n0 + 1] * ny)
+        f1 = h[n1] * (v[2 * n1] * nx + v[2 * n1 + 1] * ny)
+        qout += dj1[b] * 0.5 * (f0 + f1)
+    qin = 0.05 * 200.0
+    dV = vol(h) - vol(oh)
+    assert abs(dV - p.dt * (qin - qout)) < 1e-9 * p.dt * qin
+
+
+def test_jacobian_is_directional_derivative_of_residual(cfg1):
+    """The assembled FD Jacobian (fe_sw2_load.c:34-269) applied to a random
+    direction equals -dR/dt along it (R = -sum of element residuals,
+    fe_sw2_resid.c:29-216), checked with a central difference of the global
+    residual."""
+    p = cfg1
+    o = Oracle(p)
+    o.shift()
+    o.init()
+    R0 = o.resid()
+    o.newton(opts(tol_nonlin=1e-3 * np.abs(R0).max(), prec=1, nblock=1, max_nonlin_it=1))
+    st = o.get_state()
+    o.resid()
+    o.load()
+    A = _blocks(o)
+    rng = np.random.default_rng(1)
+    N = p.nnodes
+    d = rng.standard_normal(3 * N)
+    d[2::3] *= 1e-2
+    t = 1e-5
+
+    def R_at(s):
+        h = st[0] + s * d[2::3]
+        vel = st[1].copy()
+        vel[0::2] += s * d[0::3]
+        vel[1::2] += s * d[1::3]
+        o.set_state(h, vel, *st[2:])
+        return o.resid()
+
+    fd = -(R_at(t) - R_at(-t)) / (2 * t)
+    o.set_state(*st)
+    Jd = A @ d
+    assert np.linalg.norm(Jd - fd) < 1e-5 * np.linalg.norm(fd)
+    # a transposed block layout fails the same check
+    Jt = _blocks_transposed(A, N) @ d
+    assert np.linalg.norm(Jt - fd) > 1e-2 * np.linalg.norm(fd)
+
+
+def _blocks_transposed(A, N):
+    A = A.tocoo()
+    r, c = A.row, A.col
+    rr = 3 * (r // 3) + (c % 3)
+    cc = 3 * (c // 3) + (r % 3)
+    return scipy.sparse.csr_matrix((A.data, (rr, cc)), shape=A.shape)
+
+
+def test_pattern_is_mesh_adjacency(cfg1):
+    """Off-diagonal blocks = distinct node pairs sharing a triangle or an edge
+    element (spv_load.c:5-139); the structured flume has nnzb = N + 2 edges
+    and 6 off-diagonal blocks on interior rows."""
+    p = cfg1
+    o = _loaded(p)
+    rp, col = o.pattern()
+    N = p.nnodes
+    nbr = [set() for _ in range(N)]
+    for t in p.e2d.reshape(-1, 3):
+        for a in t:
+            for b in t:
+                if a != b:
+                    nbr[a].add(b)
+    for i in range(N):
+        got = list(col[rp[i]:rp[i + 1]])
+        assert len(got) == len(set(got))
+        assert set(got) == nbr[i]
+    npx, npy = p.meta["npx"], p.meta["npy"]
+    edges = (npx - 1) * npy + npx * (npy - 1) + (npx - 1) * (npy - 1)
+    assert rp[N] == 2 * edges
+    interior = [i * npy + j for i in range(1, npx - 1) for j in range(1, npy - 1)]
+    assert all(rp[k + 1] - rp[k] == 6 for k in interior)
+
+
+# ------------------------------------------------------------ linear algebra
+def _loaded(p):
+    o = Oracle(p)
+    o.shift()
+    o.init()
+    o.resid()
+    o.load()
+    return o
+
+
+def test_jacobi_scaling_closed_form(cfg1):
+    """spv_jacobi.c:222-317: s_k = 1/sqrt(max_j |A_kj|), and the stored
+    system becomes S A S with b scaled by S (spv_vscale.c:66-90)."""
+    o = _loaded(cfg1)
+    A = _blocks(o)
+    b = o.residual()
+    s = o.linear_setup(opts(prec=2))
+    mx = np.asarray(np.abs(A).max(axis=1).todense()).ravel()
+    want = np.where(mx < np.finfo(float).eps, 1.0 / math.sqrt(np.finfo(float).eps), 1.0 / np.sqrt(mx))
+    np.testing.assert_allclose(s, want, rtol=1e-15)
+    As = _blocks(o)
+    S = scipy.sparse.diags(s)
+    assert abs(As - S @ A @ S).max() <= 1e-15 * abs(As).max()
+    np.testing.assert_allclose(o.rhs(), s * b, rtol=1e-15)
+
+
+def test_profile_lu_solve_matches_dense_solve(cfg1):
+    """With one block (OP BLK 1) the skyline LU is an exact factorisation, so
+    the preconditioned BiCGSTAB returns the dense solution of A x = b."""
+    o = _loaded(cfg1)
+    A = _blocks(o).toarray()
+    b = o.residual().copy()
+    o.linear_setup(opts(prec=1, nblock=1))
+    ok, x, it = o.linear_solve(opts(prec=1, nblock=1, lin_rtol=1e-14, lin_atol=0.0))
+    assert ok == 1
+    xd = np.linalg.solve(A, b)
+    assert np.abs(x - xd).max() < 1e-9 * np.abs(xd).max()
+
+
+def test_block_jacobi_bicgstab_reaches_its_stopping_rule(cfg1):
+    """BiCGSTAB with the 3x3 block-Jacobi preconditioner stops once
+    ||r||_inf <= atol + rtol ||M^-1 b||_inf (solv_bcgstab.c:39-301); the
+    returned x satisfies the unscaled system to that accuracy."""
+    o = _loaded(cfg1)
+    A = _blocks(o)
+    b = o.residual().copy()
+    o.linear_setup(opts(prec=2))
+    ok, x, it = o.linear_solve(opts(prec=2, lin_rtol=1e-12, lin_atol=0.0))
+    assert ok == 1 and 1 <= it < 500
+    xd = scipy.sparse.linalg.spsolve(A.tocsc(), b)
+    assert np.abs(x - xd).max() < 1e-8 * np.abs(xd).max()
+
+
+def test_newton_converges_quadratically(cfg1):
+    """fe_newton.c:214-912 with an accurate solve: the max-norm history
+    falls superlinearly to 1e-8 of the initial residual."""
+    o = Oracle(cfg1)
+    o.shift()
+    o.init()
+    R0 = np.abs(o.resid()).max()
+    res = o.newton(opts(tol_nonlin=1e-10 * R0, prec=1, nblock=1))
+    hm = res["hist_max"]
+    assert res["converged"] and res["iterations"] <= 6
+    assert hm[-1] <= 1e-10 * R0
+    # each of the first steps at least squares the relative residual (within a constant)
+    rel = hm / hm[0]
+    assert rel[2] < 10.0 * rel[1] ** 1.5
