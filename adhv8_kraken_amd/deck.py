@@ -1,0 +1,398 @@
+"""Readers for the reference's SW2 input decks: `.2dm` mesh, `.bc` cards and
+`.hot` initial conditions, turned into an `SW2Problem` plus the run controls
+(Newton tolerances, time series, time span, test-case flag).
+
+Only the card subset that the SW2 Newton path consumes is interpreted; any
+other physics card that would change the equations (wind, waves, Coriolis by
+latitude, transport, Dirichlet strings, ...) raises `DeckError` rather than
+being silently dropped.  Formats follow:
+
+* `.2dm`: ND id x y z / TRI id n1 n2 n3 mat (src/initio/read_geo.c:50),
+* `.bc`: cards as parsed by src/initio/read_bc_{OP,IP,MP,FR,NB,SERIES,TC,TEST,STRINGS}.c,
+* `.hot`: SMS DATASET blocks IOH / IPH / IV / IPV (src/structs/ssw.c:164-545).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh as _mesh
+
+
+class DeckError(ValueError):
+    pass
+
+
+NOT_QUITE_SMALL = 1.0e-6  # src/include/define.h (IOH zero depths become -NOT_QUITE_SMALL, ssw.c:221-224)
+
+
+@dataclass
+class Series:
+    """Piecewise-linear time series (src/structs/sseries.c)."""
+
+    kind: str
+    sid: int
+    times: np.ndarray
+    values: np.ndarray  # [npoints, nvalues]
+
+    def average(self, t0: float, t1: float, col: int = 0) -> float:
+        """Mean value over [t0, t1] of the linear interpolant, the quantity
+        sseries_setall_ts_valuesAVG hands to the boundary terms
+        (src/structs/sseries.c:618)."""
+        t, v = self.times, self.values[:, col]
+        if t1 <= t0:
+            return float(np.interp(t0, t, v))
+        pts = np.unique(np.concatenate([[t0, t1], t[(t > t0) & (t < t1)]]))
+        vals = np.interp(pts, t, v)
+        return float(np.sum(0.5 * (vals[1:] + vals[:-1]) * np.diff(pts)) / (t1 - t0))
+
+    def is_constant(self, col: int = 0) -> bool:
+        return bool(np.all(self.values[:, col] == self.values[0, col]))
+
+
+@dataclass
+class Deck:
+    problem: _mesh.SW2Problem
+    tol_nonlin: float
+    inc_nonlin: float
+    max_nonlin_it: int
+    max_lin_it: int
+    nblock: int
+    prec: int
+    t0: float
+    tf: float
+    dt_series: Series
+    series: dict
+    edge_series: np.ndarray      # series id per 1D element (value source of e1d_val)
+    test_case: str | None
+    density: float
+    cards: list = field(default_factory=list)
+
+    def dt_at(self, t: float) -> float:
+        return float(np.interp(t, self.dt_series.times, self.dt_series.values[:, 0]))
+
+
+# ------------------------------------------------------------------ .2dm
+def read_2dm(path):
+    nodes, tris = {}, []
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "ND":
+                nodes[int(tok[1])] = (float(tok[2]), float(tok[3]), float(tok[4]))
+            elif tok[0] == "TRI":
+                tris.append((int(tok[1]), int(tok[2]), int(tok[3]), int(tok[4]), int(tok[5])))
+            elif tok[0] in ("E3T",):
+                tris.append((int(tok[1]), int(tok[2]), int(tok[3]), int(tok[4]), int(tok[5])))
+            elif tok[0] in ("MESH2D",):
+                continue
+            elif tok[0] in ("E4Q", "QUAD", "E6T", "E8Q", "E9Q", "TET", "PRISM"):
+                raise DeckError(f"{path}: element type {tok[0]} is not a linear triangle")
+    n = len(nodes)
+    if sorted(nodes) != list(range(1, n + 1)):
+        raise DeckError(f"{path}: node ids are not 1..{n}")
+    xyz = np.array([nodes[i] for i in range(1, n + 1)], dtype=np.float64)
+    tris.sort()
+    if [t[0] for t in tris] != list(range(1, len(tris) + 1)):
+        raise DeckError(f"{path}: element ids are not 1..{len(tris)}")
+    e2d = np.array([[t[1] - 1, t[2] - 1, t[3] - 1] for t in tris], dtype=np.int32)
+    mat = np.array([t[4] - 1 for t in tris], dtype=np.int32)
+    return xyz, e2d, mat
+
+
+# ------------------------------------------------------------------ .hot
+def read_hot(path, nnodes):
+    """Returns {name: array}; scalar sets are [N], vector sets [N, k]."""
+    out = {}
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = 0
+    while i < len(lines):
+        tok = lines[i].split()
+        if tok and tok[0] in ("BEGSCL", "BEGVEC"):
+            vec = tok[0] == "BEGVEC"
+            name, nd = None, None
+            i += 1
+            while i < len(lines):
+                t = lines[i].split()
+                if t and t[0] == "ND":
+                    nd = int(t[1])
+                elif t and t[0] == "NAME":
+                    name = t[1].strip('"').upper()
+                elif t and t[0] == "TS":
+                    i += 1
+                    break
+                i += 1
+            if nd != nnodes:
+                raise DeckError(f"{path}: dataset {name} has {nd} values for {nnodes} nodes")
+            rows = []
+            while i < len(lines) and len(rows) < nd:
+                t = lines[i].split()
+                if t:
+                    rows.append([float(v) for v in t])
+                i += 1
+            arr = np.array(rows, dtype=np.float64)
+            out[name] = arr if vec else arr[:, 0]
+            continue
+        i += 1
+    return out
+
+
+# ------------------------------------------------------------------ .bc
+_IGNORED = {("OP", "SW2"), ("OP", "TRN"), ("OP", "INC"), ("MP", "ML"), ("MP", "SRT"), ("SOUT", None),
+            ("SERIES", "AWRITE"), ("SERIES", "OUTPUT")}
+
+
+def _series_card(tok, lines, i):
+    kind = tok[1]
+    sid = int(tok[2])
+    npts = int(tok[3])
+    rows = []
+    j = i + 1
+    while len(rows) < npts:
+        t = lines[j].split("#")[0].split("!")[0].split()
+        j += 1
+        if not t:
+            continue
+        rows.append([float(v) for v in t])
+    if kind in ("AWRITE", "OUTPUT"):
+        return None, j
+    r = np.array(rows, dtype=np.float64)
+    return Series(kind, sid, r[:, 0].copy(), r[:, 1:].copy()), j
+
+
+def read_deck(stem):
+    """Builds the SW2 problem of <stem>.2dm / <stem>.bc / <stem>.hot."""
+    xyz, e2d, mat = read_2dm(stem + ".2dm")
+    N, E = xyz.shape[0], e2d.shape[0]
+    with open(stem + ".bc") as f:
+        lines = f.read().splitlines()
+
+    op = dict(nblock=1, prec=1, tau_pg=0.5, tau_temporal=0.0)
+    ip = dict(tol_nonlin=None, inc_nonlin=None, max_nonlin_it=None, max_lin_it=500)
+    mp = dict(g=9.8, density=1000.0, viscosity=1.0e-6, muc=1.0, dtl=0.0)
+    mat_string = {}
+    ev = {}
+    manning = {}
+    series = {}
+    nb = {}       # string -> (kind, series id)
+    edges = []    # (n1, n2, string)
+    t0 = tf = None
+    test_case = None
+    cards = []
+    i = 0
+    while i < len(lines):
+        raw = lines[i].split("#")[0]
+        tok = raw.split()
+        if not tok or tok[0].startswith("!") or tok[0].startswith("#"):
+            i += 1
+            continue
+        c0 = tok[0].upper()
+        c1 = tok[1].upper() if len(tok) > 1 else None
+        cards.append(" ".join(tok))
+        if c0 == "END":
+            break
+        if c0 == "SERIES":
+            s, i = _series_card([c0, c1] + tok[2:], lines, i)
+            if s is not None:
+                series[s.sid] = s
+            continue
+        if (c0, c1) in _IGNORED or (c0, None) in _IGNORED:
+            pass
+        elif c0 == "OP":
+            if c1 == "BLK":
+                op["nblock"] = int(tok[2])
+            elif c1 == "PRE":
+                op["prec"] = int(tok[2])
+            elif c1 == "TPG":
+                op["tau_pg"] = float(tok[2])
+            elif c1 == "TEM":
+                op["tau_temporal"] = float(tok[2])
+            else:
+                raise DeckError(f"unsupported card: {raw.strip()}")
+        elif c0 == "IP":
+            key = {"NTL": "tol_nonlin", "ITL": "inc_nonlin", "NIT": "max_nonlin_it", "MIT": None}.get(c1, "?")
+            if key == "?":
+                raise DeckError(f"unsupported card: {raw.strip()}")
+            if key == "max_nonlin_it":
+                ip[key] = int(tok[2])
+            elif key:
+                ip[key] = float(tok[2])
+            # IP MIT is parsed into the model but the single-model solve keeps
+            # its own limit of 500 (src/solver/solv_initialize.c:54)
+        elif c0 == "MTS":
+            mat_string[int(tok[1]) - 1] = int(tok[2])
+        elif c0 == "MP":
+            if c1 == "EVS":
+                ev[int(tok[2]) - 1] = (float(tok[3]), float(tok[4]), float(tok[5]))
+            elif c1 == "MUC":
+                mp["muc"] = float(tok[2])
+            elif c1 == "MU":
+                mp["viscosity"] = float(tok[2])
+            elif c1 == "RHO":
+                mp["density"] = float(tok[2])
+            elif c1 == "G":
+                mp["g"] = float(tok[2])
+            elif c1 == "DTL":
+                mp["dtl"] = max(float(tok[2]), 0.0)
+            else:
+                raise DeckError(f"unsupported card: {raw.strip()}")
+        elif c0 == "FR":
+            if c1 != "MNG":
+                raise DeckError(f"unsupported card: {raw.strip()}")
+            manning[int(tok[2])] = float(tok[3])
+        elif c0 == "NB":
+            if c1 not in ("OTW", "VEL", "SOURCE", "OUT"):
+                raise DeckError(f"unsupported card: {raw.strip()}")
+            nb[int(tok[2])] = (c1, int(tok[3]) if len(tok) > 3 else None)
+        elif c0 == "EGS":
+            edges.append((int(tok[1]) - 1, int(tok[2]) - 1, int(tok[3])))
+        elif c0 == "TC":
+            if c1 == "T0":
+                t0 = float(tok[2])
+            elif c1 == "TF":
+                tf = float(tok[2])
+            else:
+                raise DeckError(f"unsupported card: {raw.strip()}")
+        elif c0 == "TEST":
+            test_case = c1
+        else:
+            raise DeckError(f"unsupported card: {raw.strip()}")
+        i += 1
+
+    # materials: one per string id that MTS maps (material m -> string s)
+    nmat = int(mat.max()) + 1
+    for m in range(nmat):
+        mat_string.setdefault(m, m + 1)
+    ks = np.zeros(E, dtype=np.float64)
+    src_flag = np.zeros(E, dtype=np.int32)
+    src = np.zeros(E, dtype=np.float64)
+    for m in range(nmat):
+        s = mat_string[m]
+        sel = mat == m
+        if s in manning:
+            ks[sel] = _mesh.manning_to_ks(manning[s], mp["muc"], mp["g"])
+        if s in nb and nb[s][0] == "SOURCE":
+            ser = series[nb[s][1]]
+            if not ser.is_constant():
+                raise DeckError("time-varying NB SOURCE series are not supported")
+            src_flag[sel] = 1
+            src[sel] = ser.values[0, 0]
+
+    mat_turb = np.array([_mesh.TURB_EVS if m in ev else _mesh.TURB_NONE for m in range(nmat)], dtype=np.int32)
+    mat_ev = np.array([c for m in range(nmat) for c in ev.get(m, (0.0, 0.0, 0.0))], dtype=np.float64)
+
+    # 1D boundary elements from EGS cards, each with its adjacent triangle
+    owner = {}
+    for ie, t in enumerate(e2d):
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            owner.setdefault((min(a, b), max(a, b)), ie)
+    e1d, e1d_adj, e1d_bc, e1d_val, e1d_ser = [], [], [], [], []
+    kinds = {"OTW": _mesh.BCT_PRS_NEU, "VEL": _mesh.BCT_VEL_NEU, "OUT": _mesh.BCT_OUTFLOW}
+    for (a, b, s) in edges:
+        if s not in nb or nb[s][0] == "SOURCE":
+            continue  # an edge string without a natural condition is a wall
+        kind, sid = nb[s]
+        key = (min(a, b), max(a, b))
+        if key not in owner:
+            raise DeckError(f"EGS {a + 1} {b + 1}: not an edge of the mesh")
+        e1d.append((a, b))
+        e1d_adj.append(owner[key])
+        e1d_bc.append(kinds[kind])
+        e1d_ser.append(sid if sid is not None else -1)
+        e1d_val.append(0.0)
+    B = len(e1d)
+
+    hot = read_hot(stem + ".hot", N)
+    if "IOH" not in hot:
+        raise DeckError("hotstart has no IOH")
+    head = hot["IOH"].copy()
+    head[head == 0.0] = -NOT_QUITE_SMALL
+    vel = np.zeros(2 * N)
+    if "IV" in hot:
+        vel[0::2] = hot["IV"][:, 0]
+        vel[1::2] = hot["IV"][:, 1]
+    old_vel = vel.copy()
+    if "IPV" in hot:
+        old_vel[0::2] = hot["IPV"][:, 0]
+        old_vel[1::2] = hot["IPV"][:, 1]
+    old_head = hot["IPH"].copy() if "IPH" in hot else head.copy()
+
+    dt_series = next((s for s in series.values() if s.kind == "DT"), None)
+    if dt_series is None or t0 is None or tf is None:
+        raise DeckError("deck needs SERIES DT, TC T0 and TC TF")
+
+    p = _mesh.SW2Problem(
+        name=stem.rsplit("/", 1)[-1],
+        x=np.ascontiguousarray(xyz[:, 0]), y=np.ascontiguousarray(xyz[:, 1]), z=np.ascontiguousarray(xyz[:, 2]),
+        e2d=np.ascontiguousarray(e2d.ravel()), e2d_mat=mat, e2d_ks=ks, e2d_src_flag=src_flag, e2d_src=src,
+        e1d=np.ascontiguousarray(np.array(e1d, dtype=np.int32).reshape(-1)),
+        e1d_adj=np.array(e1d_adj, dtype=np.int32), e1d_bc=np.array(e1d_bc, dtype=np.int32),
+        e1d_val=np.array(e1d_val, dtype=np.float64),
+        node_bc=np.zeros(N, dtype=np.int32), node_bc_val=np.zeros(3 * N, dtype=np.float64),
+        mat_turb=mat_turb, mat_eev_mode=np.zeros(nmat, dtype=np.int32), mat_eev_coef=np.zeros(nmat),
+        mat_ev=mat_ev, mat_cor_speed=np.zeros(nmat),
+        g=mp["g"], dt=dt_series.values[0, 0], tau_temporal=op["tau_temporal"], tau_pg=op["tau_pg"],
+        drying_lower=mp["dtl"], drying_upper=mp["dtl"], viscosity=mp["viscosity"],
+        head0=head, vel0=vel, nsteps=0,
+        meta=dict(old_head=old_head, old_vel=old_vel, deck=stem),
+    )
+    deck = Deck(problem=p,
+                tol_nonlin=ip["tol_nonlin"] if ip["tol_nonlin"] is not None else -9999999.9,
+                inc_nonlin=ip["inc_nonlin"] if ip["inc_nonlin"] is not None else -9999999.9,
+                max_nonlin_it=ip["max_nonlin_it"] if ip["max_nonlin_it"] is not None else 10,
+                max_lin_it=ip["max_lin_it"], nblock=op["nblock"], prec=op["prec"], t0=t0, tf=tf,
+                dt_series=dt_series, series=series, edge_series=np.array(e1d_ser, dtype=np.int32),
+                test_case=test_case, density=mp["density"], cards=cards)
+    set_edge_values(deck, t0, t0 + p.dt)
+    return deck
+
+
+def set_edge_values(deck: Deck, t0: float, t1: float):
+    """Boundary values for the step [t0, t1]: series averages per edge."""
+    p = deck.problem
+    for b in range(p.nelems1d):
+        sid = int(deck.edge_series[b])
+        if sid >= 0:
+            p.e1d_val[b] = deck.series[sid].average(t0, t1)
+
+
+def step_times(deck: Deck):
+    """dt sequence of the reference's time control without failures
+    (src/tools/tc_end.c:5-50): dt from the DT series, doubled after success
+    but capped by the series, exact hit of TF and a half step when within
+    two steps of it."""
+    t = deck.t0
+    dt = deck.dt_at(t)
+    if t + dt >= deck.tf:
+        dt = deck.tf - t
+    out = []
+    while True:
+        out.append((t, dt))
+        t += dt
+        if t >= deck.tf:
+            return out
+        dt *= 2.0
+        dt_new = deck.dt_at(t)
+        if dt_new < dt:
+            dt = dt_new
+        if t + dt >= deck.tf:
+            dt = deck.tf - t
+        elif t + 2.0 * dt >= deck.tf:
+            dt = 0.5 * (deck.tf - t)
+
+
+def testcase_error_nb(head, vel, nnodes):
+    """write_testcase_error_nb (src/initio/test_cases.c:341-388): mean
+    absolute deviation from the 45-degree 0.1 m/s inflow and from h = 1."""
+    a = 0.1 / math.sqrt(2.0)
+    ex = ey = eh = 0.0
+    for i in range(nnodes):  # sequential sums in node order, as the reference
+        ex += abs(vel[2 * i] - a)
+        ey += abs(vel[2 * i + 1] - a)
+        eh += abs(head[i] - 1.0)
+    return ex / nnodes, ey / nnodes, eh / nnodes

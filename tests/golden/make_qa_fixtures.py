@@ -1,0 +1,61 @@
+"""Writes tests/golden/qa_<case>.npz from the reference's SW2 QA decks.
+
+Each fixture holds the deck's data as read by adhv8_kraken_amd.deck (mesh,
+materials, boundary edges and values, initial state, Newton controls and
+the time-step sequence) together with the figures of the case's committed
+error.out.last, which the reference's test-case writer
+(src/initio/test_cases.c) produced at the end of the run.
+
+Run here, where the reference tree is mounted:
+    python tests/golden/make_qa_fixtures.py [/root/reference]
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from adhv8_kraken_amd import deck  # noqa: E402
+
+CASES = {
+    "angle_nb": "qa/sw2/angle_nb/tri/angle_nb",
+}
+
+FIELDS = ["x", "y", "z", "e2d", "e2d_mat", "e2d_ks", "e2d_src_flag", "e2d_src", "e1d", "e1d_adj", "e1d_bc",
+          "e1d_val", "node_bc", "node_bc_val", "mat_turb", "mat_eev_mode", "mat_eev_coef", "mat_ev",
+          "mat_cor_speed", "head0", "vel0"]
+SCALARS = ["g", "dt", "tau_temporal", "tau_pg", "drying_lower", "drying_upper", "viscosity"]
+
+
+def expected_errors(path):
+    """Every floating-point figure of error.out.last, in file order."""
+    txt = open(path).read()
+    return np.array([float(v) for v in re.findall(r"[-+]?\d+\.\d+e[-+]\d+", txt)], dtype=np.float64), txt
+
+
+def main(ref):
+    for name, rel in CASES.items():
+        stem = os.path.join(ref, rel)
+        d = deck.read_deck(stem)
+        p = d.problem
+        out = {f: getattr(p, f) for f in FIELDS}
+        out.update({s: np.float64(getattr(p, s)) for s in SCALARS})
+        out["old_head"] = p.meta["old_head"]
+        out["old_vel"] = p.meta["old_vel"]
+        out["steps"] = np.array(deck.step_times(d), dtype=np.float64)
+        out["newton"] = np.array([d.tol_nonlin, d.inc_nonlin, d.max_nonlin_it, d.max_lin_it, d.prec, d.nblock],
+                                 dtype=np.float64)
+        vals, txt = expected_errors(os.path.join(os.path.dirname(stem), "error.out.last"))
+        out["expected"] = vals
+        out["expected_text"] = np.array(txt)
+        out["test_case"] = np.array(d.test_case or "")
+        dst = os.path.join(HERE, f"qa_{name}.npz")
+        np.savez_compressed(dst, **out)
+        print(dst, os.path.getsize(dst), "bytes")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
