@@ -3,8 +3,8 @@
 (Newton tolerances, time series, time span, test-case flag).
 
 Only the card subset that the SW2 Newton path consumes is interpreted; any
-other physics card that would change the equations (wind, waves, Coriolis by
-latitude, transport, Dirichlet strings, ...) raises `DeckError` rather than
+other physics card that would change the equations (wind, waves, baroclinic
+terms, transport, Dirichlet strings, ...) raises `DeckError` rather than
 being silently dropped.  Formats follow:
 
 * `.2dm`: ND id x y z / TRI id n1 n2 n3 mat (src/initio/read_geo.c:50),
@@ -23,6 +23,11 @@ from . import mesh as _mesh
 
 class DeckError(ValueError):
     pass
+
+
+class _XY:
+    def __init__(self, xyz):
+        self.x, self.y = xyz[:, 0], xyz[:, 1]
 
 
 NOT_QUITE_SMALL = 1.0e-6  # src/include/define.h (IOH zero depths become -NOT_QUITE_SMALL, ssw.c:221-224)
@@ -143,6 +148,9 @@ def read_hot(path, nnodes):
 
 
 # ------------------------------------------------------------------ .bc
+# card groups the reference's .bc parser dispatches on (src/initio/read_bc.c)
+_CARD_GROUPS = {"OP", "IP", "MP", "FR", "NB", "DB", "EGS", "NDS", "MTS", "SERIES", "TC", "TEST", "SOUT", "END", "CN",
+                "PC", "NDP", "FLX", "FST", "WER", "WRS", "FLP", "FGT", "SLS", "SDR", "BR", "OFF", "ON", "MID"}
 _IGNORED = {("OP", "SW2"), ("OP", "TRN"), ("OP", "INC"), ("MP", "ML"), ("MP", "SRT"), ("SOUT", None),
             ("SERIES", "AWRITE"), ("SERIES", "OUTPUT")}
 
@@ -177,12 +185,15 @@ def read_deck(stem):
     mp = dict(g=9.8, density=1000.0, viscosity=1.0e-6, muc=1.0, dtl=0.0)
     mat_string = {}
     ev = {}
+    eev = {}      # material -> (coef, mode)
+    cor = {}      # material -> latitude in degrees
     manning = {}
     series = {}
     nb = {}       # string -> (kind, series id)
     edges = []    # (n1, n2, string)
     t0 = tf = None
     test_case = None
+    test_args = []
     cards = []
     i = 0
     while i < len(lines):
@@ -229,6 +240,15 @@ def read_deck(stem):
         elif c0 == "MP":
             if c1 == "EVS":
                 ev[int(tok[2]) - 1] = (float(tok[3]), float(tok[4]), float(tok[5]))
+                eev.pop(int(tok[2]) - 1, None)
+            elif c1 == "EEV":
+                mode = int(tok[4])
+                if mode not in (1, 2, 3):
+                    raise DeckError(f"EEV mode must be 1, 2 or 3: {raw.strip()}")
+                eev[int(tok[2]) - 1] = (float(tok[3]), mode)
+                ev.pop(int(tok[2]) - 1, None)
+            elif c1 == "COR":
+                cor[int(tok[2]) - 1] = float(tok[3])
             elif c1 == "MUC":
                 mp["muc"] = float(tok[2])
             elif c1 == "MU":
@@ -260,8 +280,10 @@ def read_deck(stem):
                 raise DeckError(f"unsupported card: {raw.strip()}")
         elif c0 == "TEST":
             test_case = c1
-        else:
+            test_args = [float(v) for v in tok[2:]]
+        elif c0 in _CARD_GROUPS:
             raise DeckError(f"unsupported card: {raw.strip()}")
+        # any other first word is not a card and the reference's parser skips it
         i += 1
 
     # materials: one per string id that MTS maps (material m -> string s)
@@ -283,8 +305,14 @@ def read_deck(stem):
             src_flag[sel] = 1
             src[sel] = ser.values[0, 0]
 
-    mat_turb = np.array([_mesh.TURB_EVS if m in ev else _mesh.TURB_NONE for m in range(nmat)], dtype=np.int32)
+    mat_turb = np.array([_mesh.TURB_EVS if m in ev else _mesh.TURB_EEV if m in eev else _mesh.TURB_NONE
+                         for m in range(nmat)], dtype=np.int32)
     mat_ev = np.array([c for m in range(nmat) for c in ev.get(m, (0.0, 0.0, 0.0))], dtype=np.float64)
+    mat_eev_coef = np.array([eev[m][0] if m in eev else 0.0 for m in range(nmat)], dtype=np.float64)
+    mat_eev_mode = np.array([eev[m][1] if m in eev else 0 for m in range(nmat)], dtype=np.int32)
+    # angular speed per material (src/tools/coriolis.c:3-5)
+    mat_cor_speed = np.array([_mesh.coriolis_speed(cor[m]) if m in cor else 0.0 for m in range(nmat)],
+                             dtype=np.float64)
 
     # 1D boundary elements from EGS cards, each with its adjacent triangle
     owner = {}
@@ -307,20 +335,31 @@ def read_deck(stem):
         e1d_val.append(0.0)
     B = len(e1d)
 
+    # hotstart (src/structs/ssw.c:164-545): depths default to 0 when the file
+    # holds no IOH; zero depths read from IOH become -NOT_QUITE_SMALL; without
+    # IPH/IPV the old and older states copy the current one, with them the
+    # older state still copies the current one (ssw.c:510-542)
     hot = read_hot(stem + ".hot", N)
-    if "IOH" not in hot:
-        raise DeckError("hotstart has no IOH")
-    head = hot["IOH"].copy()
-    head[head == 0.0] = -NOT_QUITE_SMALL
+    head = np.zeros(N)
+    if "IOH" in hot:
+        head = hot["IOH"].copy()
+        head[head == 0.0] = -NOT_QUITE_SMALL
     vel = np.zeros(2 * N)
     if "IV" in hot:
         vel[0::2] = hot["IV"][:, 0]
         vel[1::2] = hot["IV"][:, 1]
+    old_head = hot["IPH"].copy() if "IPH" in hot else head.copy()
+    older_head = head.copy()
     old_vel = vel.copy()
     if "IPV" in hot:
         old_vel[0::2] = hot["IPV"][:, 0]
         old_vel[1::2] = hot["IPV"][:, 1]
-    old_head = hot["IPH"].copy() if "IPH" in hot else head.copy()
+    older_vel = vel.copy()
+    if test_case == "DAM":
+        # testcase_prep_dam2d runs after the hotstart (src/main/adh_func.c:294-298)
+        from . import qa
+        head = qa.dam2d_initial_head(_XY(xyz),
+                                     *test_args[:3])
 
     dt_series = next((s for s in series.values() if s.kind == "DT"), None)
     if dt_series is None or t0 is None or tf is None:
@@ -334,12 +373,13 @@ def read_deck(stem):
         e1d_adj=np.array(e1d_adj, dtype=np.int32), e1d_bc=np.array(e1d_bc, dtype=np.int32),
         e1d_val=np.array(e1d_val, dtype=np.float64),
         node_bc=np.zeros(N, dtype=np.int32), node_bc_val=np.zeros(3 * N, dtype=np.float64),
-        mat_turb=mat_turb, mat_eev_mode=np.zeros(nmat, dtype=np.int32), mat_eev_coef=np.zeros(nmat),
-        mat_ev=mat_ev, mat_cor_speed=np.zeros(nmat),
+        mat_turb=mat_turb, mat_eev_mode=mat_eev_mode, mat_eev_coef=mat_eev_coef,
+        mat_ev=mat_ev, mat_cor_speed=mat_cor_speed,
         g=mp["g"], dt=dt_series.values[0, 0], tau_temporal=op["tau_temporal"], tau_pg=op["tau_pg"],
         drying_lower=mp["dtl"], drying_upper=mp["dtl"], viscosity=mp["viscosity"],
         head0=head, vel0=vel, nsteps=0,
-        meta=dict(old_head=old_head, old_vel=old_vel, deck=stem),
+        meta=dict(old_head=old_head, old_vel=old_vel, older_head=older_head, older_vel=older_vel, deck=stem,
+                  latitude=[cor.get(m, 0.0) for m in range(nmat)]),
     )
     deck = Deck(problem=p,
                 tol_nonlin=ip["tol_nonlin"] if ip["tol_nonlin"] is not None else -9999999.9,
@@ -348,6 +388,7 @@ def read_deck(stem):
                 max_lin_it=ip["max_lin_it"], nblock=op["nblock"], prec=op["prec"], t0=t0, tf=tf,
                 dt_series=dt_series, series=series, edge_series=np.array(e1d_ser, dtype=np.int32),
                 test_case=test_case, density=mp["density"], cards=cards)
+    deck.test_args = test_args
     set_edge_values(deck, t0, t0 + p.dt)
     return deck
 
@@ -360,39 +401,3 @@ def set_edge_values(deck: Deck, t0: float, t1: float):
         if sid >= 0:
             p.e1d_val[b] = deck.series[sid].average(t0, t1)
 
-
-def step_times(deck: Deck):
-    """dt sequence of the reference's time control without failures
-    (src/tools/tc_end.c:5-50): dt from the DT series, doubled after success
-    but capped by the series, exact hit of TF and a half step when within
-    two steps of it."""
-    t = deck.t0
-    dt = deck.dt_at(t)
-    if t + dt >= deck.tf:
-        dt = deck.tf - t
-    out = []
-    while True:
-        out.append((t, dt))
-        t += dt
-        if t >= deck.tf:
-            return out
-        dt *= 2.0
-        dt_new = deck.dt_at(t)
-        if dt_new < dt:
-            dt = dt_new
-        if t + dt >= deck.tf:
-            dt = deck.tf - t
-        elif t + 2.0 * dt >= deck.tf:
-            dt = 0.5 * (deck.tf - t)
-
-
-def testcase_error_nb(head, vel, nnodes):
-    """write_testcase_error_nb (src/initio/test_cases.c:341-388): mean
-    absolute deviation from the 45-degree 0.1 m/s inflow and from h = 1."""
-    a = 0.1 / math.sqrt(2.0)
-    ex = ey = eh = 0.0
-    for i in range(nnodes):  # sequential sums in node order, as the reference
-        ex += abs(vel[2 * i] - a)
-        ey += abs(vel[2 * i + 1] - a)
-        eh += abs(head[i] - 1.0)
-    return ex / nnodes, ey / nnodes, eh / nnodes

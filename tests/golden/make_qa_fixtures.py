@@ -22,6 +22,9 @@ from adhv8_kraken_amd import deck  # noqa: E402
 
 CASES = {
     "angle_nb": "qa/sw2/angle_nb/tri/angle_nb",
+    "angle_coriolis": "qa/sw2/angle_coriolis/tri/2D-coriolis_X",
+    "angle_dam": "qa/sw2/angle_dam/tri/angle_dam",
+    "angle_source": "qa/sw2/angle_source/tri/angle_source",
 }
 
 FIELDS = ["x", "y", "z", "e2d", "e2d_mat", "e2d_ks", "e2d_src_flag", "e2d_src", "e1d", "e1d_adj", "e1d_bc",
@@ -43,9 +46,16 @@ def main(ref):
         p = d.problem
         out = {f: getattr(p, f) for f in FIELDS}
         out.update({s: np.float64(getattr(p, s)) for s in SCALARS})
-        out["old_head"] = p.meta["old_head"]
-        out["old_vel"] = p.meta["old_vel"]
-        out["steps"] = np.array(deck.step_times(d), dtype=np.float64)
+        for k in ("old_head", "old_vel", "older_head", "older_vel"):
+            out[k] = p.meta[k]
+        for sid in set(d.edge_series.tolist()) - {-1}:
+            if not d.series[sid].is_constant():
+                raise SystemExit(f"{name}: boundary series {sid} varies in time")
+        out["t0_tf"] = np.array([d.t0, d.tf])
+        out["dt_series"] = np.stack([d.dt_series.times, d.dt_series.values[:, 0]])
+        out["test_args"] = np.array(getattr(d, "test_args", []), dtype=np.float64)
+        out["density"] = np.float64(d.density)
+        out["latitude"] = np.array(p.meta["latitude"], dtype=np.float64)
         out["newton"] = np.array([d.tol_nonlin, d.inc_nonlin, d.max_nonlin_it, d.max_lin_it, d.prec, d.nblock],
                                  dtype=np.float64)
         vals, txt = expected_errors(os.path.join(os.path.dirname(stem), "error.out.last"))

@@ -1,51 +1,64 @@
-"""End-to-end golden vectors: the reference's SW2 QA cases (qa/sw2/*/tri),
-whose committed error.out.last holds the analytic-error figures the
-reference printed at the end of its run (src/initio/test_cases.c).
+"""End-to-end golden vectors: the reference's SW2 QA cases (qa/sw2/*/tri).
+Each case's committed error.out.last holds the analytic-error figures that
+the reference's test-case writer (src/initio/test_cases.c) printed at the
+end of its run; tests/golden/qa_<case>.npz holds the deck converted to
+arrays, and adhv8_kraken_amd.qa restates the writers.
 
-angle_nb: 45-degree channel, NB VEL outflow and NB OTW tailwater, ten
-1000 s steps, Newton to 1e-10 absolute (IP NTL) with a full skyline-LU
-preconditioner (OP BLK 1).  The printed figures are mean absolute
-deviations of u, v from 0.1/sqrt(2) m/s and of h from 1 m."""
+  angle_nb        45-degree channel, NB VEL + NB OTW, 10 steps, NTL 1e-10
+  angle_coriolis  Coriolis at 45 N with EEV turbulence, 175 steps, NIT 4 (first step cut twice)
+  angle_dam       dry-bed dam break with wetting and drying, 401 steps
+  angle_source    rainfall (NB SOURCE) on a closed basin, 200 steps, mass error
+
+The oracle runs the reference's algorithm (skyline-LU preconditioner with
+one block, OP BLK 1).  The library keeps its own 3x3 block-Jacobi BiCGSTAB
+and is run with a 1e-12 relative linear tolerance, so each Newton step
+solves its system as exactly as the reference's LU does and only rounding
+separates the two Newton sequences.
+"""
 import numpy as np
 import pytest
 
 import qa_cases
 
-# The figures in error.out.last carry the run's own Newton tolerance
-# (1e-10 absolute on residuals of order 1e2): agreement to 1e-9 relative
-# means the same converged state, not just a similar one.
-ORACLE_RTOL = 1e-9
-# The library solves each Newton system with block-Jacobi BiCGSTAB instead of
-# the reference's exact skyline LU, to a relative linear tolerance of 1e-12,
-# so its converged state differs from the reference's by the Newton tolerance.
-LIBRARY_RTOL = 1e-7
+# Per case: figures compared (indices into the error.out.last numbers, and
+# into our error vector) and the relative tolerances.  The oracle
+# tolerances are set an order above the agreement measured when the fixtures
+# were made; they reflect each deck's Newton tolerance (angle_coriolis
+# stops at 1e-4 absolute and its figure is a small difference of slopes, so
+# its last digits follow rounding), not a looseness of the comparison.
+CASES = {
+    #               ours idx   ref idx        oracle rtol  library rtol
+    "angle_nb": ([0, 1, 2], [0, 2, 4], 1e-9, 1e-7),
+    "angle_coriolis": ([0], [0], 1e-5, 1e-2),
+    "angle_dam": ([0, 1, 2, 3], [0, 2, 4, 5], 3e-9, 1e-5),
+    "angle_source": ([0], [0], 1e-6, 1e-3),
+}
 
 
-def test_angle_nb_oracle_reproduces_error_out_last():
+@pytest.mark.parametrize("name", list(CASES))
+def test_oracle_reproduces_error_out_last(name):
+    ours, ref, rtol, _ = CASES[name]
+    p, ctl = qa_cases.load(name)
+    h, v, log, err = qa_cases.run_oracle(name, p, ctl)
+    np.testing.assert_allclose(err[ours], ctl["expected"][ref], rtol=rtol)
+
+
+def test_fixture_comparison_is_sharp():
+    """A velocity off by 1e-6 m/s at one node of angle_nb fails the check."""
+    from adhv8_kraken_amd import qa
     p, ctl = qa_cases.load("angle_nb")
-    h, v, its = qa_cases.run_oracle(p, ctl)
-    got = qa_cases.errors("angle_nb", p, h, v)
-    want = ctl["expected"][[0, 2, 4]]  # x-vel, y-vel, head absolute errors
-    np.testing.assert_allclose(got, want, rtol=ORACLE_RTOL)
-    assert len(its) == 10
-
-
-def test_angle_nb_fixture_detects_a_wrong_answer():
-    """The comparison is sharp: a state off by 1e-6 m/s at one node fails it."""
-    p, ctl = qa_cases.load("angle_nb")
-    h, v, _ = qa_cases.run_oracle(p, ctl)
+    h, v, log, err = qa_cases.run_oracle("angle_nb", p, ctl)
     v = v.copy()
     v[0] += 1e-6
-    got = qa_cases.errors("angle_nb", p, h, v)
-    want = ctl["expected"][[0, 2, 4]]
-    assert not np.allclose(got, want, rtol=ORACLE_RTOL, atol=0)
+    bad = qa.error_nb(p, h, v)
+    assert not np.allclose(bad, ctl["expected"][[0, 2, 4]], rtol=CASES["angle_nb"][2], atol=0)
 
 
 @pytest.mark.gpu
-def test_angle_nb_library_reproduces_error_out_last():
-    p, ctl = qa_cases.load("angle_nb")
-    print(f"[qa] angle_nb on the device: N={p.nnodes} E={p.nelems2d} steps={len(ctl['steps'])}", flush=True)
-    h, v, its = qa_cases.run_library(p, ctl, lin_rtol=1e-12, lin_atol=0.0)
-    got = qa_cases.errors("angle_nb", p, h, v)
-    want = ctl["expected"][[0, 2, 4]]
-    np.testing.assert_allclose(got, want, rtol=LIBRARY_RTOL)
+@pytest.mark.parametrize("name", list(CASES))
+def test_library_reproduces_error_out_last(name):
+    ours, ref, _, rtol = CASES[name]
+    p, ctl = qa_cases.load(name)
+    print(f"[qa] {name} on the device: N={p.nnodes} E={p.nelems2d}", flush=True)
+    h, v, log, err = qa_cases.run_library(name, p, ctl, lin_rtol=1e-12, lin_atol=0.0)
+    np.testing.assert_allclose(err[ours], ctl["expected"][ref], rtol=rtol)
