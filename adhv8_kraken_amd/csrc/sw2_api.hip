@@ -690,8 +690,11 @@ extern "C" int adh_sw2_hip_probe_math(const double *x, int n, double *log_out, d
 }
 
 extern "C" int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms) {
-    if (!h || !avg_ms || reps < 1 || which < 0 || which > 3) return fail(ADH_SW2_EINVAL, "bad arguments");
+    if (!h || !avg_ms || reps < 1 || which < 0 || which > 4) return fail(ADH_SW2_EINVAL, "bad arguments");
     Ctx &c = h->c;
+    double *copy_dst = nullptr;
+    const size_t copy_n = 81 * (size_t)c.E;  // the element-matrix slab: 648 MB at 1M triangles
+    if (which == 4) HIPCHK(hipMalloc(&copy_dst, sizeof(double) * (copy_n ? copy_n : 1)));
     if (which == 0) {
         KScal k{};
         k.rho = 1.0; k.rhop = 1.0; k.alpha = 1.0; k.omega = 1.0; k.max_it = 1 << 30; k.done = 0;
@@ -701,7 +704,8 @@ extern "C" int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, doub
         if (which == 0) launch_k1_only(&c);
         else if (which == 1) launch_jac2d_only(&c);
         else if (which == 2) launch_resid2d_only(&c);
-        else launch_assemble_only(&c);
+        else if (which == 3) launch_assemble_only(&c);
+        else launch_copy(&c, c.EM, copy_dst, copy_n);
     };
     launch();  // warm-up
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -717,6 +721,7 @@ extern "C" int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, doub
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     HIPCHK(hipGetLastError());
+    if (copy_dst) HIPCHK(hipFree(copy_dst));
     *avg_ms = ms / reps;
     return 0;
 }

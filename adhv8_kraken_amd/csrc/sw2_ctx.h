@@ -117,6 +117,7 @@ void launch_krylov_iters(Ctx *c, int k);
 void launch_krylov_epilogue(Ctx *c);
 void launch_geometry(Ctx *c);
 void launch_zero(Ctx *c, double *p, size_t n);
+void launch_copy(Ctx *c, const double *src, double *dst, size_t n);
 void launch_k1_only(Ctx *c);
 void launch_jac2d_only(Ctx *c);
 void launch_resid2d_only(Ctx *c);

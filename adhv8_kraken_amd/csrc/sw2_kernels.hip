@@ -304,6 +304,17 @@ void launch_scale_sol(Ctx *c, double f) {
                        c->sol, f);
 }
 
+// plain streaming copy of n doubles (bandwidth calibration for the counters)
+__global__ void __launch_bounds__(256) k_copy(size_t n, const double *__restrict__ src, double *__restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy(Ctx *c, const double *src, double *dst, size_t n) {
+    if (n) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, n, src, dst);
+}
+
 __global__ void k_zero(size_t n, double *x) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = 0.0;

@@ -267,7 +267,8 @@ class SW2Model:
         return diag, offd[: 9 * nnz], r
 
     def time_kernel(self, which: int, reps: int = 20) -> float:
-        """Average launch duration in ms (0 SpMV+M^-1, 1 FD Jacobian, 2 residual, 3 assembly)."""
+        """Average launch duration in ms (0 SpMV+M^-1, 1 FD Jacobian, 2 residual, 3 assembly,
+        4 copy of the 81E-double element-matrix slab)."""
         ms = C.c_double(0.0)
         _check(self.L.adh_sw2_hip_time_kernel(self.h, which, reps, C.byref(ms)))
         return ms.value

@@ -108,6 +108,22 @@ def cpu_baseline(seconds_budget=20.0):
                        f"{its} Newton its in {dt:.2f}s; rate scaled by 40k/1M triangles")
 
 
+def kernel_mode(p):
+    from adhv8_kraken_amd.sw2 import SW2Model, default_opts
+    out = {}
+    for prec in (1, 0):
+        g = SW2Model(p, default_opts(precision=prec, max_nonlin_it=2))
+        g.shift()
+        g.timestep()
+        out[f"k1_prec{prec}_ms"] = g.time_kernel(0, reps=20)
+        out[f"jac_prec{prec}_ms"] = g.time_kernel(1, reps=2)
+        if prec == 0:
+            out["copy_ms"] = g.time_kernel(4, reps=5)
+            out["copy_bytes"] = 2 * 8 * 81 * p.nelems2d
+        g.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +132,9 @@ def main():
     ap.add_argument("--ncx", type=int, default=1000)
     ap.add_argument("--ncy", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels", action="store_true",
+                    help="profiling mode: one time step per form, then the K1 kernel, the FD Jacobian "
+                         "and the calibration copy launched back to back (for rocprofv3 counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described in "
@@ -125,6 +144,9 @@ def main():
     log(f"building the {args.ncx}x{args.ncy}-cell channel")
     p = mesh.bumpy_channel(args.ncx, args.ncy, seed=2)
     log(f"  {p.nnodes} nodes, {p.nelems2d} triangles")
+    if args.kernels:
+        kernel_mode(p)
+        return
     res = {}
     roof = {}
     for prec in (0, 1):

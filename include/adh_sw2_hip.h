@@ -171,7 +171,9 @@ int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *offdiag, do
  * HIP events on the library's stream over `reps` back-to-back launches on
  * the model's current data: 0 = SpMV + block-Jacobi apply + dot (the BiCGSTAB
  * K1 kernel, in the configured precision), 1 = finite-difference element
- * Jacobian, 2 = element residual, 3 = ordered block assembly.  Clobbers the
+ * Jacobian, 2 = element residual, 3 = ordered block assembly, 4 = a plain
+ * copy of the 81E-double element-matrix slab into a scratch buffer (read +
+ * write of 1296 B per triangle; calibrates memory counters).  Clobbers the
  * Krylov work vectors; no counterpart in the reference (measurement hook). */
 int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms);
 
