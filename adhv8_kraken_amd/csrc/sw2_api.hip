@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,17 @@ extern "C" void adh_sw2_hip_default_opts(adh_sw2_opts *o) {
     o->precision = 0;
     o->preconditioner = 2;
     o->check_every = 8;
+    o->nblock = 1;
+}
+
+static int check_opts(const adh_sw2_opts *o) {
+    if (o->precision != 0 && o->precision != 1) return fail(ADH_SW2_EINVAL, "precision must be 0 or 1");
+    if (o->preconditioner != 1 && o->preconditioner != 2)
+        return fail(ADH_SW2_EUNSUPPORTED, "preconditioner must be 1 (profile LU) or 2 (block Jacobi)");
+    if (o->preconditioner == 1 && o->precision != 0)
+        return fail(ADH_SW2_EUNSUPPORTED, "the profile-LU preconditioner runs in FP64 (precision 0)");
+    if (o->preconditioner == 1 && o->nblock < 1) return fail(ADH_SW2_EINVAL, "nblock must be >= 1");
+    return 0;
 }
 
 static crlog::tables g_host_tables;
@@ -161,7 +173,7 @@ static int validate(const adh_sw2_mesh *m) {
 static int alloc_solver(adh_sw2_hip *h) {
     Ctx &c = h->c;
     int rc;
-    size_t n3 = 3 * (size_t)c.N;
+    size_t n3 = 3 * (size_t)c.Np;
     if ((rc = dalloc(h, &c.kb, n3)) || (rc = dalloc(h, &c.kr, n3)) || (rc = dalloc(h, &c.kp, n3)) ||
         (rc = dalloc(h, &c.kAp, n3)) || (rc = dalloc(h, &c.ks_, n3)) || (rc = dalloc(h, &c.kAs, n3)) ||
         (rc = dalloc(h, &c.kq, n3)) || (rc = dalloc(h, &c.ku, n3)))
@@ -170,8 +182,8 @@ static int alloc_solver(adh_sw2_hip *h) {
         (rc = dalloc(h, &c.fAp, n3)) || (rc = dalloc(h, &c.fs, n3)) || (rc = dalloc(h, &c.fAs, n3)) ||
         (rc = dalloc(h, &c.fq, n3)) || (rc = dalloc(h, &c.fu, n3)))
         return rc;
-    if ((rc = dalloc(h, &c.diag32, 9 * (size_t)c.N)) || (rc = dalloc(h, &c.offd32, 9 * c.nsell)) ||
-        (rc = dalloc(h, &c.dinv32, 9 * (size_t)c.N)))
+    if ((rc = dalloc(h, &c.diag32, 9 * (size_t)c.Np)) || (rc = dalloc(h, &c.offd32, 9 * c.nsell)) ||
+        (rc = dalloc(h, &c.dinv32, 9 * (size_t)c.Np)))
         return rc;
     return 0;
 }
@@ -187,8 +199,12 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     adh_sw2_hip *h = new adh_sw2_hip();  // value-initialised: all device pointers start null
     Ctx &c = h->c;
     c.N = m->nnodes; c.E = m->nelems2d; c.B = m->nelems1d; c.M = m->nmat;
-    if (opts) c.opts = *opts;
-    else adh_sw2_hip_default_opts(&c.opts);
+    c.Np = (c.N + 63) & ~63;
+    if (opts) {
+        int rc0 = check_opts(opts);
+        if (rc0) { delete h; return rc0; }
+        c.opts = *opts;
+    } else adh_sw2_hip_default_opts(&c.opts);
     if (c.opts.check_every < 1) c.opts.check_every = 1;
     c.dt = m->dt;
     const int N = c.N, E = c.E, B = c.B;
@@ -268,6 +284,17 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
             if (len > kmax) kmax = len;
         }
         sell_ptr[s + 1] = sell_ptr[s] + kmax;
+    }
+    {   // pad every slice to the longest row when that costs little: the
+        // SpMV then runs a fully unrolled, branch-free slot loop
+        int kglob = 0;
+        for (int s = 0; s < c.nslice; s++) kglob = std::max(kglob, sell_ptr[s + 1] - sell_ptr[s]);
+        c.kuni = 0;
+        if (kglob >= 1 && kglob <= KUNI_MAX &&
+            (double)kglob * c.nslice <= 1.25 * (double)sell_ptr[c.nslice]) {
+            c.kuni = kglob;
+            for (int s = 0; s <= c.nslice; s++) sell_ptr[s] = s * kglob;
+        }
     }
     c.nsell = 64 * (size_t)sell_ptr[c.nslice];
     std::vector<int> sell_col(c.nsell > 0 ? c.nsell : 1);
@@ -369,15 +396,18 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     UP(slot_ptr, slot_ptr.data(), nslots + 1);
     UP(slot_src, slot_src.data(), slot_src.size());
     UP(pre, (const double *)nullptr, PRE_STRIDE * (size_t)E);
+    UP(elist_wet, (const int *)nullptr, E);
+    UP(elist_gen, (const int *)nullptr, E);
+    UP(ecount, (const int *)nullptr, 2);
     c.pre_valid = 0;
     UP(ER, (const double *)nullptr, 9 * (size_t)E + 6 * (size_t)B);
     UP(EM, (const double *)nullptr, 81 * (size_t)E + 36 * (size_t)B);
-    UP(R, (const double *)nullptr, 3 * (size_t)N);
-    UP(sol, (const double *)nullptr, 3 * (size_t)N);
-    UP(diag, (const double *)nullptr, 9 * (size_t)N);
+    UP(R, (const double *)nullptr, 3 * (size_t)c.Np);
+    UP(sol, (const double *)nullptr, 3 * (size_t)c.Np);
+    UP(diag, (const double *)nullptr, 9 * (size_t)c.Np);
     UP(offd, (const double *)nullptr, 9 * c.nsell);
-    UP(scale, (const double *)nullptr, 3 * (size_t)N);
-    UP(dinv, (const double *)nullptr, 9 * (size_t)N);
+    UP(scale, (const double *)nullptr, 3 * (size_t)c.Np);
+    UP(dinv, (const double *)nullptr, 9 * (size_t)c.Np);
     UP(ksc, (const KScal *)nullptr, 1);
     UP(norms, (const Norms *)nullptr, 1);
     c.red.maxblocks = nblocks_for(N, 256) + 8;
@@ -392,7 +422,7 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
         return fail(ADH_SW2_ENOMEM, "hipHostMalloc failed");
     }
     {   // scale_vect starts at 1.0 (fe_main.c:263)
-        std::vector<double> ones(3 * (size_t)N, 1.0);
+        std::vector<double> ones(3 * (size_t)c.Np, 1.0);
         HIPCHK(hipMemcpy(c.scale, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
     }
     ElemData &D = c.ed;
@@ -418,6 +448,7 @@ extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     if (!h) return;
     Ctx &c = h->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    band_free(&c);
     for (void *p : h->allocs) (void)hipFree(p);
     if (c.h_ksc) (void)hipHostFree(c.h_ksc);
     if (c.h_norms) (void)hipHostFree(c.h_norms);
@@ -429,8 +460,8 @@ extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
 
 extern "C" int adh_sw2_hip_set_opts(adh_sw2_hip *h, const adh_sw2_opts *o) {
     if (!h || !o) return fail(ADH_SW2_EINVAL, "null argument");
-    if (o->precision != 0 && o->precision != 1) return fail(ADH_SW2_EINVAL, "precision must be 0 or 1");
-    if (o->preconditioner != 2) return fail(ADH_SW2_EUNSUPPORTED, "preconditioner must be 2 (block Jacobi)");
+    int rc = check_opts(o);
+    if (rc) return rc;
     h->c.opts = *o;
     if (h->c.opts.check_every < 1) h->c.opts.check_every = 1;
     return 0;
@@ -533,13 +564,31 @@ extern "C" int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag) {
     return rc;
 }
 
+// node vectors [3N] / diagonal blocks [9N] between node-major (host) and the
+// device's sliced layout
+static int nodes_to_host(Ctx &c, const double *dev, double *out, int per) {
+    std::vector<double> t((size_t)per * c.Np);
+    HIPCHK(hipMemcpy(t.data(), dev, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+    for (int n = 0; n < c.N; n++)
+        for (int k = 0; k < per; k++) out[(size_t)per * n + k] = t[per == 3 ? vidx3(n, k) : didx9(n, k)];
+    return 0;
+}
+
+static int nodes_from_host(Ctx &c, double *dev, const double *in, int per) {
+    std::vector<double> t((size_t)per * c.Np, 0.0);
+    for (int n = 0; n < c.N; n++)
+        for (int k = 0; k < per; k++) t[per == 3 ? vidx3(n, k) : didx9(n, k)] = in[(size_t)per * n + k];
+    HIPCHK(hipMemcpy(dev, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual) {
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     launch_resid(&c);
     int rc = sync_check(c);
     if (rc) return rc;
-    if (residual) HIPCHK(hipMemcpy(residual, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    if (residual && (rc = nodes_to_host(c, c.R, residual, 3))) return rc;
     return 0;
 }
 
@@ -567,9 +616,9 @@ extern "C" int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdia
     launch_assemble(&c);
     int rc = sync_check(c);
     if (rc) return rc;
-    if (diagonal) HIPCHK(hipMemcpy(diagonal, c.diag, sizeof(double) * 9 * c.N, hipMemcpyDeviceToHost));
+    if (diagonal && (rc = nodes_to_host(c, c.diag, diagonal, 9))) return rc;
     if (offdiag && c.nnz && (rc = offd_to_host(c, offdiag))) return rc;
-    if (residual) HIPCHK(hipMemcpy(residual, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    if (residual && (rc = nodes_to_host(c, c.R, residual, 3))) return rc;
     return 0;
 }
 
@@ -585,18 +634,30 @@ extern "C" int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *
 extern "C" int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol) {
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
-    if (sol) HIPCHK(hipMemcpy(c.sol, sol, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice));
+    int rc;
+    if (sol && (rc = nodes_from_host(c, c.sol, sol, 3))) return rc;
     launch_inc(&c, 1.0);
     return sync_check(c);
+}
+
+// builds the profile-LU block structure when the options ask for it
+static int ensure_band(Ctx &c) {
+    if (c.opts.preconditioner != 1) return 0;
+    if (c.band_nblock == std::min(c.opts.nblock, c.N)) return 0;
+    std::string err = band_structure(&c, c.opts.nblock);
+    if (!err.empty()) return fail(ADH_SW2_EUNSUPPORTED, err);
+    return 0;
 }
 
 extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) {
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
+    int rc0 = ensure_band(c);
+    if (rc0) return rc0;
     launch_setup(&c);
     int rc = sync_check(c);
     if (rc) return rc;
-    if (scale_vect) HIPCHK(hipMemcpy(scale_vect, c.scale, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    if (scale_vect && (rc = nodes_to_host(c, c.scale, scale_vect, 3))) return rc;
     return 0;
 }
 
@@ -633,18 +694,18 @@ extern "C" int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *li
     if (rc) return rc;
     if ((rc = sync_check(c))) return rc;
     if (verdict) *verdict = vd;
-    if (sol) HIPCHK(hipMemcpy(sol, c.sol, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    if (sol && (rc = nodes_to_host(c, c.sol, sol, 3))) return rc;
     return 0;
 }
 
 extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, const double *offdiag, const double *rhs) {
     if (!h || !diagonal || !rhs || (!offdiag && h->c.nnz)) return fail(ADH_SW2_EINVAL, "null argument");
     Ctx &c = h->c;
-    HIPCHK(hipMemcpy(c.diag, diagonal, sizeof(double) * 9 * c.N, hipMemcpyHostToDevice));
     int rc;
+    if ((rc = nodes_from_host(c, c.diag, diagonal, 9))) return rc;
     if (c.nnz && (rc = offd_from_host(c, offdiag))) return rc;
-    HIPCHK(hipMemcpy(c.R, rhs, sizeof(double) * 3 * c.N, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.N));
+    if ((rc = nodes_from_host(c, c.R, rhs, 3))) return rc;
+    HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.Np));
     return 0;
 }
 
@@ -652,10 +713,10 @@ extern "C" int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
-    if (diagonal) HIPCHK(hipMemcpy(diagonal, c.diag, sizeof(double) * 9 * c.N, hipMemcpyDeviceToHost));
     int rc;
+    if (diagonal && (rc = nodes_to_host(c, c.diag, diagonal, 9))) return rc;
     if (offdiag && c.nnz && (rc = offd_to_host(c, offdiag))) return rc;
-    if (rhs) HIPCHK(hipMemcpy(rhs, c.R, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost));
+    if (rhs && (rc = nodes_to_host(c, c.R, rhs, 3))) return rc;
     return 0;
 }
 
@@ -755,6 +816,7 @@ static void hist(adh_sw2_report *r, const Norms &n, int lin_it) {
 static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
     const adh_sw2_opts &o = c.opts;
     int rc;
+    if ((rc = ensure_band(c))) return rc;
     Norms nm;
     (void)hipEventRecord(c.ev[0], c.stream);
     launch_init(&c, nullptr);
@@ -771,7 +833,7 @@ static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
         (void)hipEventRecord(c.ev[3], c.stream);
         launch_jacobian(&c);
         launch_assemble(&c);
-        launch_zero(&c, c.sol, 3 * (size_t)c.N);
+        launch_zero(&c, c.sol, 3 * (size_t)c.Np);
         (void)hipEventRecord(c.ev[4], c.stream);
         launch_setup(&c);
         (void)hipEventRecord(c.ev[5], c.stream);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string>
 #include <vector>
 
 #include "../../include/adh_sw2_hip.h"
@@ -33,6 +34,7 @@ struct Norms {
 
 struct Ctx {
     int N, E, B, M, nnz;
+    int Np;                // N rounded up to a multiple of 64 (node-slice padding)
     adh_sw2_opts opts;
     double dt;
     hipStream_t stream;
@@ -69,19 +71,26 @@ struct Ctx {
     // its 9 values sit at sell_vidx(q, e) so that a wavefront reading entry e
     // of slot j touches 64 consecutive values.
     int nslice;
+    int kuni;              // > 0: every slice holds kuni slots (sell_ptr[s] = s * kuni)
     size_t nsell;          // 64 * sell_ptr[nslice]
     int *sell_ptr, *sell_col, *offd_pos;  // [nslice+1], [nsell], [nnz] (CSR block -> slot)
     std::vector<int> h_offd_pos;
     // work
+    int *elist_wet, *elist_gen, *ecount;  // FD-Jacobian triangle lists [E], [E], counts [2]
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state
     double *ER;            // element residuals [9E + 6B]
     double *EM;            // element matrices [81E + 36B]
-    double *R;             // residual / rhs [3N]
-    double *sol;           // [3N]
-    double *diag, *offd;   // [9N] node-major, [9 nsell] SELL-64 values
-    double *scale;         // [3N]
-    double *dinv;          // [9N]
+    // Node vectors and 3x3 node blocks use the same wave-sliced layout as
+    // the off-diagonal blocks: nodes 64s..64s+63 form slice s, and component
+    // k of node n sits at vidx3(n, k) (entry e of a block at didx9(n, e)), so
+    // a wavefront touching one component of its 64 nodes reads 64
+    // consecutive values.
+    double *R;             // residual / rhs [3 Np]
+    double *sol;           // [3 Np]
+    double *diag, *offd;   // [9 Np] sliced, [9 nsell] SELL-64 values
+    double *scale;         // [3 Np]
+    double *dinv;          // [9 Np]
     // FP32 system for mixed precision
     float *diag32, *offd32, *dinv32;
     // Krylov vectors (FP64) and FP32 twins
@@ -93,13 +102,34 @@ struct Ctx {
     Norms *norms, *h_norms;
     int *h_flag;           // pinned scratch
     ElemData ed;
+    // profile-LU (OP BLK) preconditioner, built on demand for opts.nblock
+    int band_nblock;       // block count of the current band structure (0 = none)
+    int band_nb;           // blocks
+    int band_max_n;        // DOFs of the largest block (LDS of the solve)
+    size_t band_size;      // doubles in `band`
+    int *blk_n0, *blk_w, *node_blk;
+    size_t *blk_off;
+    double *band;
 };
+
+constexpr int KUNI_MAX = 10;
+constexpr int BAND_MAX_DOF = 8190;                        // LDS of one block solve (64 KB)
+constexpr size_t BAND_MAX_BYTES = (size_t)8 << 30;       // band storage limit  // longest uniform slice the unrolled SpMV is built for
 
 // value index of entry e (0..8) of SELL slot q
 __host__ __device__ __forceinline__ size_t sell_vidx(size_t q, int e) {
     size_t lane = q & 63;
     return 9 * (q - lane) + 64 * (size_t)e + lane;
 }
+
+// component k (0..2) of node n in a sliced node vector
+__host__ __device__ __forceinline__ size_t vidx3(int n, int k) {
+    size_t lane = (size_t)(n & 63);
+    return 3 * ((size_t)n - lane) + 64 * (size_t)k + lane;
+}
+
+// entry e (0..8, row-major 3x3) of node n's diagonal block
+__host__ __device__ __forceinline__ size_t didx9(int n, int e) { return sell_vidx((size_t)n, e); }
 
 // launchers (sw2_kernels.hip / sw2_solver.hip)
 void launch_old_terms(Ctx *c);
@@ -122,6 +152,10 @@ void launch_k1_only(Ctx *c);
 void launch_jac2d_only(Ctx *c);
 void launch_resid2d_only(Ctx *c);
 void launch_assemble_only(Ctx *c);
+std::string band_structure(Ctx *c, int nblock);
+void band_free(Ctx *c);
+void launch_band_setup(Ctx *c);
+void launch_band_solve(Ctx *c, double *x, const KScal *sc);
 void launch_probe(const double *x, int n, const crlog::tables *t, double *lg, double *sq, double *dv);
 
 int nblocks_for(int n, int bs);

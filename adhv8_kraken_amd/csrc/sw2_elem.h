@@ -283,9 +283,16 @@ HDF T pick3(const T *a, int k) { return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]); 
 
 // fe_sw2_wet_dry_wrapper (src/fe/fe_sw2_wet_dry_wrapper.c:106-306).  Flags
 // select which optional inputs exist (grad_phi, v_wet_dry, f_wet_dry).
-template <int FN, bool HAS_GRAD, bool HAS_VWD, bool HAS_FWD, bool REDIST>
+// WET: the caller guarantees h > 0 at all three nodes, so only the fully
+// wet branch is compiled (the element kernels run such triangles through a
+// variant without the clipping code and its registers).
+template <int FN, bool HAS_GRAD, bool HAS_VWD, bool HAS_FWD, bool REDIST, bool WET = false>
 HDF double wd_wrapper(d3 *elem_rhs, const v3 *x, const double *h, const v2 *grad_phi, v2 vconst, const v2 *v_wet_dry,
                       const double *f_wet_dry, double djac, double dt, double c) {
+    if constexpr (WET) {
+        wd_apply<FN>(x, h, grad_phi, vconst, HAS_VWD ? v_wet_dry : nullptr, f_wet_dry, djac, dt, c, elem_rhs);
+        return 1.0;
+    }
     int total_vtx = 0, passcode = 0;
     for (int i = 0; i < 3; i++) {
         int is_dry = (h[i] <= 0.0) ? 1 : 0;
@@ -377,7 +384,9 @@ HDF double wd_wrapper(d3 *elem_rhs, const v3 *x, const double *h, const v2 *grad
 }
 
 // fe_sw2_wet_dry_factor (src/fe/fe_sw2_wet_dry_wrapper.c:31-73)
+template <bool WET = false>
 HD double wd_factor(const v3 *x, const double *h, double djac) {
+    if constexpr (WET) return 1.0;
     int total_vtx = 0, passcode = 0;
     for (int i = 0; i < 3; i++) {
         int is_dry = (h[i] <= 0.0) ? 1 : 0;
@@ -413,9 +422,10 @@ HD double wd_factor(const v3 *x, const double *h, double djac) {
 }
 
 // fe_sw2_wd_average (src/fe/sw2/fe_sw2_wet_dry_integrations.c:42-61)
+template <bool WET = false>
 HDF void wd_average(const v3 *x, const double *depth, const v2 *v_wd, double djac, double *f_avg, v2 *v_avg) {
     d3 av[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-    double factor = wd_wrapper<WD_INTEGRATE_F, false, true, true, false>(av, x, depth, nullptr, v2{0., 0.}, v_wd, depth,
+    double factor = wd_wrapper<WD_INTEGRATE_F, false, true, true, false, WET>(av, x, depth, nullptr, v2{0., 0.}, v_wd, depth,
                                                                         djac, 1., 1.);
     *f_avg = 0.;
     v_avg->x = 0.;
@@ -429,6 +439,7 @@ HDF void wd_average(const v3 *x, const double *depth, const v2 *v_wd, double dja
 }
 
 // fe_sw2_temporal (src/fe/sw2/fe_sw2_body_resid.c:1288-1354, triangle branch)
+template <bool WET = false>
 HDF void sw2_temporal(const v3 *nds, double djac, double dll, const double *h, const v2 *vel, double wdf,
                       double dt_factor, d3 *elem_rhs, int wd_flag) {
     double con = dt_factor * wdf;
@@ -438,7 +449,7 @@ HDF void sw2_temporal(const v3 *nds, double djac, double dll, const double *h, c
         uh[i] = vel[i].x * dmax(h[i], dll);
         vh[i] = vel[i].y * dmax(h[i], dll);
     }
-    wd_wrapper<WD_CONT_TEMPORAL, false, false, false, true>(rhs, nds, h, nullptr, v2{0., 0.}, nullptr, nullptr, djac,
+    wd_wrapper<WD_CONT_TEMPORAL, false, false, false, true, WET>(rhs, nds, h, nullptr, v2{0., 0.}, nullptr, nullptr, djac,
                                                              1.0, dt_factor);
     int_tri_phi_f_lump(djac, con, uh, rx);
     int_tri_phi_f_lump(djac, con, vh, ry);
@@ -517,6 +528,7 @@ HD void elem_old_terms(const ElemData &D, int ie, double *pre) {
 // for the lowest-index element that touches it.  Old/older-state terms come
 // from `pre` (elem_old_terms); the reference's wet/dry average of the old
 // state (:248) feeds only debug output and is not formed.
+template <bool WET = false>
 HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, int pnode, int pvar, int psign,
                    bool dry_rule, d3 *elem_rhs) {
     int PRESSURE_FLAG = 1;
@@ -581,8 +593,8 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     const double factor_old = pre[18];
     double h_wd = elem_h_avg;
     v2 vwd;
-    wd_average(nds, h, vel, djac, &h_wd, &vwd);
-    factor = wd_factor(nds, h, djac);
+    wd_average<WET>(nds, h, vel, djac, &h_wd, &vwd);
+    factor = wd_factor<WET>(nds, h, djac);
     const double vwd_mag = sqrt(vwd.x * vwd.x + vwd.y * vwd.y);
 
     const double h_fric = dmax(h_wd, SMALL);
@@ -650,7 +662,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
 
     // advection :434-461
     for (int i = 0; i < 3; i++) { rhs[i].x_eq = 0.; rhs[i].y_eq = 0.; rhs[i].c_eq = 0.; }
-    wd_wrapper<WD_CONVECTION, true, true, false, true>(rhs, nds, h, gs, v2{0., 0.}, vel, nullptr, djac, dt, -1);
+    wd_wrapper<WD_CONVECTION, true, true, false, true, WET>(rhs, nds, h, gs, v2{0., 0.}, vel, nullptr, djac, dt, -1);
     for (int i = 0; i < 3; i++) {
         elem_rhs[i].c_eq += rhs[i].c_eq;
         elem_rhs[i].x_eq += rhs[i].x_eq;
@@ -658,7 +670,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     }
 
     // temporal :480-490
-    sw2_temporal(nds, djac, dll, h, vel, factor, (1. + alpha / 2.), elem_rhs, -3);
+    sw2_temporal<WET>(nds, djac, dll, h, vel, factor, (1. + alpha / 2.), elem_rhs, -3);
 #pragma unroll
     for (int k = 0; k < 2; k++) {   // t(n) and t(n-1) terms, fe_sw2_temporal with the element's wd_flag
         const double *t = pre + 9 * k;
@@ -744,13 +756,13 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
             for (int i = 0; i < 3; i++) { rhs[i].x_eq = 0.; rhs[i].y_eq = 0.; rhs[i].c_eq = 0.; }
             if (pass == 0) {
                 t1 = -one_2 * g * dt;
-                wd_wrapper<WD_PRESSURE, true, false, false, true>(rhs, nds, h, gs, v2{0., 0.}, nullptr, nullptr, djac, 1, t1);
+                wd_wrapper<WD_PRESSURE, true, false, false, true, WET>(rhs, nds, h, gs, v2{0., 0.}, nullptr, nullptr, djac, 1, t1);
             } else if (pass == 1) {
                 t1 = g * dt;
-                wd_wrapper<WD_BODYFORCE, false, false, false, true>(rhs, nds, h, nullptr, ggz, nullptr, nullptr, djac, 1, t1);
+                wd_wrapper<WD_BODYFORCE, false, false, false, true, WET>(rhs, nds, h, nullptr, ggz, nullptr, nullptr, djac, 1, t1);
             } else {
                 t1 = one_2 * dt * g;
-                wd_wrapper<WD_BOUNDARY_PRESSURE, false, false, false, true>(rhs, nds, h, nullptr, v2{0., 0.}, nullptr, nullptr,
+                wd_wrapper<WD_BOUNDARY_PRESSURE, false, false, false, true, WET>(rhs, nds, h, nullptr, v2{0., 0.}, nullptr, nullptr,
                                                                             djac, 1, t1);
             }
             for (int i = 0; i < 3; i++) {   // dof3_add_replace_array
@@ -776,9 +788,9 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
         d3 dfdt[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
         d3 adv[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
         double dudt = pre[19], dvdt = pre[20], dhdt = pre[21];   // -(old-state integrals), :1012-1013
-        factor = wd_wrapper<WD_INTEGRATE_F, false, true, true, false>(dfdt, nds, h, nullptr, v2{0., 0.}, vel, h, djac, 1, 1);
+        factor = wd_wrapper<WD_INTEGRATE_F, false, true, true, false, WET>(dfdt, nds, h, nullptr, v2{0., 0.}, vel, h, djac, 1, 1);
         dudt += dfdt[0].x_eq; dvdt += dfdt[0].y_eq; dhdt += dfdt[0].c_eq;
-        factor = wd_wrapper<WD_GLS_CONVECTION, true, true, true, false>(adv, nds, h, gs, v2{0., 0.}, vel, h, djac, dt, 1);
+        factor = wd_wrapper<WD_GLS_CONVECTION, true, true, true, false, WET>(adv, nds, h, gs, v2{0., 0.}, vel, h, djac, dt, 1);
         double x_pressure = g_factor * (ggh.x + ggz.x) * dt * factor + ggp.x * dt * factor;
         double y_pressure = g_factor * (ggh.y + ggz.y) * dt * factor + ggp.y * dt * factor;
         double x_friction = g * bed_stress.x * dt * factor;
@@ -935,7 +947,7 @@ HDF double num_diff_eps(double var) {
 // local node `j` (perturb_sw2_head/vel, src/fe/sw2/fe_sw2_load.c:293-483 and
 // elem_matrix_deriv, src/fe/fe_elem_utilities.c:518-525).  col[i*3 + r] is
 // d(elem_rhs[i].eq_r)/d(var_j) with eq order x, y, c.
-template <int NN>
+template <int NN, bool WET = false>
 HD void jac_column(const ElemData &D, int ie, int j, int var, double *col) {
     int n = NN == 3 ? D.e2d[3 * ie + j] : D.e1d[2 * ie + j];
     double val = var == 0 ? D.u[n] : (var == 1 ? D.v[n] : D.head[n]);
@@ -944,8 +956,8 @@ HD void jac_column(const ElemData &D, int ie, int j, int var, double *col) {
     d3 P[3], M[3];
     if (NN == 3) {
         const double *pre = D.pre + PRE_STRIDE * (size_t)ie;
-        body_resid(D, pre, ie, eps, j, pv, +1, false, P);
-        body_resid(D, pre, ie, eps, j, pv, -1, false, M);
+        body_resid<WET>(D, pre, ie, eps, j, pv, +1, false, P);
+        body_resid<WET>(D, pre, ie, eps, j, pv, -1, false, M);
     } else {
         boundary_resid(D, ie, eps, j, pv, +1, false, P);
         boundary_resid(D, ie, eps, j, pv, -1, false, M);

@@ -68,9 +68,9 @@ __global__ void k_resid_gather(int N, const int *n2e_ptr, const int *n2e, const 
         if (!dry) { rx -= o[0]; ry -= o[1]; }
         rc -= o[2];
     }
-    R[3 * n] = rx;
-    R[3 * n + 1] = ry;
-    R[3 * n + 2] = rc;
+    R[vidx3(n, 0)] = rx;
+    R[vidx3(n, 1)] = ry;
+    R[vidx3(n, 2)] = rc;
     if (dry && (n2e_ptr[n + 1] > n2e_ptr[n] || n2b_ptr[n + 1] > n2b_ptr[n])) {
         u[n] = 0.;
         v[n] = 0.;
@@ -89,14 +89,39 @@ void launch_resid(Ctx *c) {
 }
 
 // ------------------------------------------------------------------ Jacobian
-// One thread per (triangle, column): column q = 3*j + var (var 0 u, 1 v, 2 h).
-// EM[e*81 + (i*3+j)*9 + r*3 + var] = d elem_rhs[i].eq_r / d var_j.
-__global__ void __launch_bounds__(128) k_jac2d(ElemData D, int E, int b0, int b1, int b2, double *EM) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 9 * E) return;
-    int ie = t / 9, q = t % 9, j = q / 3, var = q % 3;
+// EM[e*81 + (i*3+j)*9 + r*3 + var] = d elem_rhs[i].eq_r / d var_j, column
+// q = 3*j + var (var 0 u, 1 v, 2 h).
+// The same over a list of triangles (`count` entries), two lanes per column:
+// the even lane evaluates the +eps residual and the odd lane the -eps one, so
+// each lane carries one residual evaluation; the pair forms the central
+// difference with a lane exchange.  WET lists hold triangles whose depths stay
+// positive under every perturbation and run without the wet/dry clipping code.
+template <bool WET>
+__global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list, const int *count, int b0, int b1,
+                                                    int b2, double *EM) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int li = t / 18;
+    if (li >= *count) return;  // both lanes of a pair leave together
+    const int ie = list[li];
+    const int q = (t >> 1) % 9, j = q / 3, var = q % 3;
+    const int sgn = (t & 1) ? -1 : +1;
+    const int n = D.e2d[3 * ie + j];
+    const double val = var == 0 ? D.u[n] : (var == 1 ? D.v[n] : D.head[n]);
+    const double eps = num_diff_eps(val), eps2 = 2.0 * eps;
+    const int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
+    d3 r[3];
+    body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, eps, j, pv, sgn, false, r);
     double col[9];
-    jac_column<3>(D, ie, j, var, col);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        // even lanes hold P (+eps), odd lanes M (-eps); both lanes form P - M
+        double mx = __shfl_xor(r[i].x_eq, 1, 64), my = __shfl_xor(r[i].y_eq, 1, 64),
+               mc = __shfl_xor(r[i].c_eq, 1, 64);
+        col[3 * i + 0] = (r[i].x_eq - mx) / eps2;
+        col[3 * i + 1] = (r[i].y_eq - my) / eps2;
+        col[3 * i + 2] = (r[i].c_eq - mc) / eps2;
+    }
+    if (t & 1) return;
     const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
     const int lb[3] = {b0, b1, b2};
     jac_column_db<3>(D, nodes, lb, j, var, col);
@@ -104,8 +129,10 @@ __global__ void __launch_bounds__(128) k_jac2d(ElemData D, int E, int b0, int b1
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) o[(3 * i + j) * 9 + 3 * r + var] = col[3 * i + r];
+        for (int rr = 0; rr < 3; rr++) o[(3 * i + j) * 9 + 3 * rr + var] = col[3 * i + rr];
 }
+template __global__ void k_jac2d_list<true>(ElemData, const int *, const int *, int, int, int, double *);
+template __global__ void k_jac2d_list<false>(ElemData, const int *, const int *, int, int, int, double *);
 
 __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, double *EM1) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,13 +150,63 @@ __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1
         for (int r = 0; r < 3; r++) o[(2 * i + j) * 9 + 3 * r + var] = col[3 * i + r];
 }
 
+// Splits the triangles into those whose three depths stay positive under the
+// -eps perturbation of the FD Jacobian (h - eps(h) > 0, the exact operation
+// the perturbed residual performs) and the rest.  One atomic per wavefront;
+// list order is irrelevant (each triangle writes only its own block).
+__global__ void __launch_bounds__(256) k_wet_split(int E, const int *e2d, const double *head, int *wet, int *gen,
+                                                   int *counts) {
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    bool is_wet = false;
+    if (ie < E) {
+        is_wet = true;
+        for (int i = 0; i < 3; i++) {
+            const double h = head[e2d[3 * ie + i]];
+            if (!(h + (-1.0) * num_diff_eps(h) > 0.0)) is_wet = false;
+        }
+    }
+    const unsigned long long bw = __ballot(ie < E && is_wet), bg = __ballot(ie < E && !is_wet);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int base_w = 0, base_g = 0;
+    if (lane == 0) {
+        if (bw) base_w = atomicAdd(counts, __popcll(bw));
+        if (bg) base_g = atomicAdd(counts + 1, __popcll(bg));
+    }
+    base_w = __shfl(base_w, 0, 64);
+    base_g = __shfl(base_g, 0, 64);
+    if (ie < E) {
+        if (is_wet) wet[base_w + __popcll(bw & below)] = ie;
+        else gen[base_g + __popcll(bg & below)] = ie;
+    }
+}
+
+__global__ void k_zero_int(int n, int *p) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+static void launch_jac2d_split(Ctx *c, const ElemData &D) {
+    if (!c->E) return;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, 2, c->ecount);
+    hipLaunchKernelGGL(k_wet_split, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
+                       c->elist_wet, c->elist_gen, c->ecount);
+    (void)hipMemcpyAsync(c->h_flag, c->ecount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    const int nwet = c->h_flag[0], ngen = c->h_flag[1];
+    if (nwet > 0)
+        hipLaunchKernelGGL(k_jac2d_list<true>, dim3(nblocks_for(18 * nwet, 128)), dim3(128), 0, c->stream, D,
+                           c->elist_wet, c->ecount, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->EM);
+    if (ngen > 0)
+        hipLaunchKernelGGL(k_jac2d_list<false>, dim3(nblocks_for(18 * ngen, 128)), dim3(128), 0, c->stream, D,
+                           c->elist_gen, c->ecount + 1, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->EM);
+}
+
 void launch_jacobian(Ctx *c) {
     if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = nullptr;
-    if (c->E)
-        hipLaunchKernelGGL(k_jac2d, dim3(nblocks_for(9 * c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->loc_bc[0],
-                           c->loc_bc[1], c->loc_bc[2], c->EM);
+    launch_jac2d_split(c, D);
     if (c->B)
         hipLaunchKernelGGL(k_jac1d, dim3(nblocks_for(6 * c->B, 128)), dim3(128), 0, c->stream, D, c->B, c->loc_bc[0],
                            c->loc_bc[1], c->EM + 81 * (size_t)c->E);
@@ -144,7 +221,7 @@ __global__ void k_assemble(int nslots, int N, const int *slot_ptr, const int *sl
     int s = (int)(t / 9), k = (int)(t % 9);
     double a = 0.0;
     for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) a += EM[9 * (size_t)slot_src[p] + k];
-    if (s < N) diag[9 * (size_t)s + k] = a;
+    if (s < N) diag[didx9(s, k)] = a;
     else offd[sell_vidx((size_t)offd_pos[s - N], k)] = a;
 }
 
@@ -153,16 +230,17 @@ __global__ void k_diag_floor(int N, double *diag, double *R, const double *u, co
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const double f = NOT_QUITE_SMALL;
-    double *D = diag + 9 * (size_t)n;
-    if (fabs(D[0]) < f) {
-        D[0] = f;
-        if (fabs(R[3 * n]) < f) R[3 * n] = -f * u[n];
+    double &d0 = diag[didx9(n, 0)], &d4 = diag[didx9(n, 4)], &d8 = diag[didx9(n, 8)];
+    double &r0 = R[vidx3(n, 0)], &r1 = R[vidx3(n, 1)];
+    if (fabs(d0) < f) {
+        d0 = f;
+        if (fabs(r0) < f) r0 = -f * u[n];
     }
-    if (fabs(D[4]) < f) {
-        D[4] = f;
-        if (fabs(R[3 * n + 1]) < f) R[3 * n + 1] = -f * v[n];
+    if (fabs(d4) < f) {
+        d4 = f;
+        if (fabs(r1) < f) r1 = -f * v[n];
     }
-    if (fabs(D[8]) < f) D[8] = f;
+    if (fabs(d8) < f) d8 = f;
 }
 
 void launch_assemble(Ctx *c) {
@@ -177,8 +255,7 @@ void launch_jac2d_only(Ctx *c) {
     if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = nullptr;
-    hipLaunchKernelGGL(k_jac2d, dim3(nblocks_for(9 * c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->loc_bc[0],
-                       c->loc_bc[1], c->loc_bc[2], c->EM);
+    launch_jac2d_split(c, D);
 }
 
 void launch_resid2d_only(Ctx *c) {
@@ -284,9 +361,9 @@ void launch_shift(Ctx *c) {
 __global__ void k_inc(int N, const double *sol, double *h, double *u, double *v) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    u[n] += 1. * sol[3 * n];
-    v[n] += 1. * sol[3 * n + 1];
-    h[n] += 1. * sol[3 * n + 2];
+    u[n] += 1. * sol[vidx3(n, 0)];
+    v[n] += 1. * sol[vidx3(n, 1)];
+    h[n] += 1. * sol[vidx3(n, 2)];
 }
 
 void launch_inc(Ctx *c, double) {
@@ -300,7 +377,7 @@ __global__ void k_scale_vec(size_t n, double *x, double f) {
 }
 
 void launch_scale_sol(Ctx *c, double f) {
-    hipLaunchKernelGGL(k_scale_vec, dim3(nblocks_for(3 * c->N, 256)), dim3(256), 0, c->stream, (size_t)3 * c->N,
+    hipLaunchKernelGGL(k_scale_vec, dim3(nblocks_for(3 * c->Np, 256)), dim3(256), 0, c->stream, (size_t)3 * c->Np,
                        c->sol, f);
 }
 
@@ -351,7 +428,7 @@ __global__ void __launch_bounds__(256) k_norms(int N, const double *R, const dou
     if (n < N) {
         bool inc = dmin(h[n], oh[n]) > 0;
         for (int j = 0; j < 3; j++) {
-            double r = R[3 * n + j], q = sol[3 * n + j];
+            double r = R[vidx3(n, j)], q = sol[vidx3(n, j)];
             if (inc && fabs(r) > m) m = fabs(r);
             if (inc && fabs(q) > in) in = fabs(q);
             s += r * r;

@@ -15,9 +15,10 @@
 // past convergence return at entry.  The same kernels run in FP64 or FP32
 // (matrix, preconditioner and vectors), with dot products accumulated in FP64.
 //
-// HBM layout: 3x3 diagonal blocks diag[9N] and off-diagonal blocks offd[9*nnz]
-// (block rows in the reference's insertion order), col[nnz], row_ptr[N+1],
-// vectors [3N] node-major.  SpMV reads 72 B (FP64) per block.
+// HBM layout (wave-sliced, 64 nodes per slice): diagonal blocks at
+// didx9(n, e), off-diagonal blocks in SELL-64 (sell_vidx), node vectors at
+// vidx3(n, k); every per-node load of a wavefront is one 64-wide contiguous
+// access.  SpMV reads 72 B (FP64) per block.
 #include "sw2_ctx.h"
 
 namespace sw2 {
@@ -30,11 +31,13 @@ __global__ void k_scale_rows(int N, const double *diag, const double *offd, cons
                              double *sv) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    const double *D = diag + 9 * (size_t)n;
     double s[3];
     for (int j = 0; j < 3; j++) {
         double m = 0.0;
-        for (int k = 0; k < 3; k++) if (fabs(D[3 * j + k]) > m) m = fabs(D[3 * j + k]);
+        for (int k = 0; k < 3; k++) {
+            double a = fabs(diag[didx9(n, 3 * j + k)]);
+            if (a > m) m = a;
+        }
         s[j] = m;
     }
     const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
@@ -48,26 +51,26 @@ __global__ void k_scale_rows(int N, const double *diag, const double *offd, cons
             }
     }
     const double small_factor = 1.0 / sqrt(SOLV_TOL);
-    for (int j = 0; j < 3; j++) sv[3 * n + j] = s[j] < SOLV_TOL ? small_factor : 1.0 / sqrt(fabs(s[j]));
+    for (int j = 0; j < 3; j++) sv[vidx3(n, j)] = s[j] < SOLV_TOL ? small_factor : 1.0 / sqrt(fabs(s[j]));
 }
 
 __global__ void k_scale_apply(int N, double *diag, double *offd, const int *row_ptr, const int *sell_ptr,
                               const int *sell_col, const double *sv, double *R, double *sol) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    double rf[3] = {sv[3 * n], sv[3 * n + 1], sv[3 * n + 2]};
+    double rf[3] = {sv[vidx3(n, 0)], sv[vidx3(n, 1)], sv[vidx3(n, 2)]};
     for (int j = 0; j < 3; j++) {
-        R[3 * n + j] *= rf[j];
-        sol[3 * n + j] /= rf[j];
+        R[vidx3(n, j)] *= rf[j];
+        sol[vidx3(n, j)] /= rf[j];
     }
-    double *D = diag + 9 * (size_t)n;
     for (int j = 0; j < 3; j++)
-        for (int k = 0; k < 3; k++) D[3 * j + k] *= rf[j] * rf[k];
+        for (int k = 0; k < 3; k++) diag[didx9(n, 3 * j + k)] *= rf[j] * rf[k];
     const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
     const int len = row_ptr[n + 1] - row_ptr[n];
     for (int b = 0; b < len; b++) {
         const size_t q = q0 + 64 * (size_t)b;
-        const double *sc = sv + 3 * (size_t)sell_col[q];
+        const int cn = sell_col[q];
+        const double sc[3] = {sv[vidx3(cn, 0)], sv[vidx3(cn, 1)], sv[vidx3(cn, 2)]};
         for (int j = 0; j < 3; j++)
             for (int k = 0; k < 3; k++) offd[sell_vidx(q, 3 * j + k)] *= rf[j] * sc[k];
     }
@@ -80,7 +83,7 @@ __global__ void k_bj_setup(int N, const double *diag, double *dinv, float *dinv3
     double a[3][6];
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) {
-            a[r][c] = diag[9 * (size_t)n + 3 * r + c];
+            a[r][c] = diag[didx9(n, 3 * r + c)];
             a[r][3 + c] = (r == c) ? 1.0 : 0.0;
         }
     for (int k = 0; k < 3; k++) {
@@ -100,8 +103,8 @@ __global__ void k_bj_setup(int N, const double *diag, double *dinv, float *dinv3
     }
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) {
-            dinv[9 * (size_t)n + 3 * r + c] = a[r][3 + c];
-            if (dinv32) dinv32[9 * (size_t)n + 3 * r + c] = (float)a[r][3 + c];
+            dinv[didx9(n, 3 * r + c)] = a[r][3 + c];
+            if (dinv32) dinv32[didx9(n, 3 * r + c)] = (float)a[r][3 + c];
         }
 }
 
@@ -116,10 +119,14 @@ void launch_setup(Ctx *c) {
                        c->sell_ptr, c->scale);
     hipLaunchKernelGGL(k_scale_apply, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
                        c->sell_ptr, c->sell_col, c->scale, c->R, c->sol);
+    if (c->opts.preconditioner == 1) {  // profile LU of the node blocks (FP64)
+        launch_band_setup(c);
+        return;
+    }
     bool f32 = c->opts.precision == 1;
     hipLaunchKernelGGL(k_bj_setup, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->dinv, f32 ? c->dinv32 : nullptr);
     if (f32) {
-        size_t n9 = 9 * (size_t)c->N, m9 = 9 * c->nsell;
+        size_t n9 = 9 * (size_t)c->Np, m9 = 9 * c->nsell;
         hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for((int)n9, 256)), dim3(256), 0, c->stream, n9, c->diag, c->diag32);
         if (m9)
             hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for((int)m9, 256)), dim3(256), 0, c->stream, m9, c->offd,
@@ -199,43 +206,73 @@ __device__ __forceinline__ bool reduce_last(double (&v)[K], const int (&op)[K], 
 // Off-diagonal blocks come from the SELL-64 slice of this row's wavefront:
 // every lane runs the slice's K block slots (zero padding past the row's
 // length), and entry e of slot j is one coalesced 64-value read per wave.
-template <typename T, typename XF>
+template <typename T, int K, typename XF, bool BJ = true>
 __device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *offd, const int *sell_ptr,
                                                const int *sell_col, const T *dinv, XF xf, T (&y)[3]) {
-    const T *D = diag + 9 * (size_t)n;
-    T x0 = xf(3 * n), x1 = xf(3 * n + 1), x2 = xf(3 * n + 2);
-    T a0 = D[0] * x0 + D[1] * x1 + D[2] * x2;
-    T a1 = D[3] * x0 + D[4] * x1 + D[5] * x2;
-    T a2 = D[6] * x0 + D[7] * x1 + D[8] * x2;
+    const T *D = diag + didx9(n, 0);  // entry e at D[64 e]
+    T x0 = xf(vidx3(n, 0)), x1 = xf(vidx3(n, 1)), x2 = xf(vidx3(n, 2));
     T p0 = 0, p1 = 0, p2 = 0;
     const int s = n >> 6, lane = n & 63;
-    const int k0 = sell_ptr[s], k1 = sell_ptr[s + 1];
-    for (int b = k0; b < k1; b++) {
-        const size_t q = 64 * (size_t)b + lane;
-        const T *a = offd + 9 * 64 * (size_t)b + lane;
-        int cc = 3 * sell_col[q];
-        T y0 = xf(cc), y1 = xf(cc + 1), y2 = xf(cc + 2);
-        p0 += a[0] * y0 + a[64] * y1 + a[128] * y2;
-        p1 += a[192] * y0 + a[256] * y1 + a[320] * y2;
-        p2 += a[384] * y0 + a[448] * y1 + a[512] * y2;
+    if constexpr (K > 0) {
+        // uniform slices: all K slots' columns, gathers and blocks are
+        // independent loads, issued before any of them is consumed
+        const size_t q0 = 64 * (size_t)s * K + lane;
+        const T *A = offd + 9 * 64 * (size_t)s * K + lane;
+        int cn[K];
+#pragma unroll
+        for (int b = 0; b < K; b++) cn[b] = sell_col[q0 + 64 * b];
+        T g[K][3];
+#pragma unroll
+        for (int b = 0; b < K; b++) {
+            const size_t cc = vidx3(cn[b], 0);
+            g[b][0] = xf(cc);
+            g[b][1] = xf(cc + 64);
+            g[b][2] = xf(cc + 128);
+        }
+#pragma unroll
+        for (int b = 0; b < K; b++) {
+            const T *a = A + 576 * b;
+            p0 += a[0] * g[b][0] + a[64] * g[b][1] + a[128] * g[b][2];
+            p1 += a[192] * g[b][0] + a[256] * g[b][1] + a[320] * g[b][2];
+            p2 += a[384] * g[b][0] + a[448] * g[b][1] + a[512] * g[b][2];
+        }
+    } else {
+        const int k0 = sell_ptr[s], k1 = sell_ptr[s + 1];
+        for (int b = k0; b < k1; b++) {
+            const size_t q = 64 * (size_t)b + lane;
+            const T *a = offd + 9 * 64 * (size_t)b + lane;
+            const int cn = sell_col[q];
+            const size_t cc = vidx3(cn, 0);
+            T y0 = xf(cc), y1 = xf(cc + 64), y2 = xf(cc + 128);
+            p0 += a[0] * y0 + a[64] * y1 + a[128] * y2;
+            p1 += a[192] * y0 + a[256] * y1 + a[320] * y2;
+            p2 += a[384] * y0 + a[448] * y1 + a[512] * y2;
+        }
     }
+    T a0 = D[0] * x0 + D[64] * x1 + D[128] * x2;
+    T a1 = D[192] * x0 + D[256] * x1 + D[320] * x2;
+    T a2 = D[384] * x0 + D[448] * x1 + D[512] * x2;
     a0 += p0; a1 += p1; a2 += p2;
-    const T *M = dinv + 9 * (size_t)n;
-    y[0] = M[0] * a0 + M[1] * a1 + M[2] * a2;
-    y[1] = M[3] * a0 + M[4] * a1 + M[5] * a2;
-    y[2] = M[6] * a0 + M[7] * a1 + M[8] * a2;
+    if constexpr (!BJ) {
+        y[0] = a0; y[1] = a1; y[2] = a2;
+        return;
+    }
+    const T *M = dinv + didx9(n, 0);
+    y[0] = M[0] * a0 + M[64] * a1 + M[128] * a2;
+    y[1] = M[192] * a0 + M[256] * a1 + M[320] * a2;
+    y[2] = M[384] * a0 + M[448] * a1 + M[512] * a2;
 }
 
 template <typename T>
 struct VecX {
     const T *x;
-    __device__ T operator()(int i) const { return x[i]; }
+    __device__ T operator()(size_t i) const { return x[i]; }
 };
 template <typename T>
 struct VecS {  // s = r + (-alpha) Ap, the reference's copy + daxpy
     const T *r, *ap;
     T nalpha;
-    __device__ T operator()(int i) const { return r[i] + nalpha * ap[i]; }
+    __device__ T operator()(size_t i) const { return r[i] + nalpha * ap[i]; }
 };
 
 template <typename T>
@@ -251,15 +288,15 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<T> V, const double 
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < N) {
-        const T *M = V.dinv + 9 * (size_t)n;
-        T b0 = (T)rhs[3 * n], b1 = (T)rhs[3 * n + 1], b2 = (T)rhs[3 * n + 2];
+        const T *M = V.dinv + didx9(n, 0);
+        T b0 = (T)rhs[vidx3(n, 0)], b1 = (T)rhs[vidx3(n, 1)], b2 = (T)rhs[vidx3(n, 2)];
         T y[3];
-        y[0] = M[0] * b0 + M[1] * b1 + M[2] * b2;
-        y[1] = M[3] * b0 + M[4] * b1 + M[5] * b2;
-        y[2] = M[6] * b0 + M[7] * b1 + M[8] * b2;
+        y[0] = M[0] * b0 + M[64] * b1 + M[128] * b2;
+        y[1] = M[192] * b0 + M[256] * b1 + M[320] * b2;
+        y[2] = M[384] * b0 + M[448] * b1 + M[512] * b2;
         for (int j = 0; j < 3; j++) {
-            V.b[3 * n + j] = y[j];
-            V.u[3 * n + j] = y[j];
+            V.b[vidx3(n, j)] = y[j];
+            V.u[vidx3(n, j)] = y[j];
             v[0] = fmax(v[0], fabs((double)y[j]));
         }
     }
@@ -269,19 +306,20 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<T> V, const double 
 }
 
 // prologue part 2: r = b' - M^-1 A u, q = r, p = r, rho = (r,q), ||r||inf  (:138-188)
-template <typename T>
+template <typename T, int K>
 __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (n < N) {
         T y[3];
-        amult_prec_row<T>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.u}, y);
+        amult_prec_row<T, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.u}, y);
         for (int j = 0; j < 3; j++) {
-            T r = V.b[3 * n + j] + (T)(-1.0) * y[j];
-            V.r[3 * n + j] = r;
-            V.q[3 * n + j] = r;
-            V.p[3 * n + j] = r;
+            const size_t i = vidx3(n, j);
+            T r = V.b[i] + (T)(-1.0) * y[j];
+            V.r[i] = r;
+            V.q[i] = r;
+            V.p[i] = r;
             v[0] = fmax(v[0], fabs((double)r));
             v[1] += (double)r * (double)r;
         }
@@ -300,7 +338,7 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *se
     }
 }
 
-template <typename T>
+template <typename T, int K>
 __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
     if (sc->done) return;
@@ -308,10 +346,11 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *se
     double v[1] = {0.0};
     if (n < N) {
         T y[3];
-        amult_prec_row<T>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.p}, y);
+        amult_prec_row<T, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.p}, y);
         for (int j = 0; j < 3; j++) {
-            V.Ap[3 * n + j] = y[j];
-            v[0] += (double)V.q[3 * n + j] * (double)y[j];
+            const size_t i = vidx3(n, j);
+            V.Ap[i] = y[j];
+            v[0] += (double)V.q[i] * (double)y[j];
         }
     }
     const int op[1] = {0};
@@ -322,7 +361,7 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *se
     }
 }
 
-template <typename T>
+template <typename T, int K>
 __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
     if (sc->done) return;
@@ -332,14 +371,15 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *se
     if (n < N) {
         VecS<T> sf{V.r, V.Ap, nalpha};
         T y[3];
-        amult_prec_row<T>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, sf, y);
+        amult_prec_row<T, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, sf, y);
         for (int j = 0; j < 3; j++) {
-            T s = sf(3 * n + j);
-            V.s[3 * n + j] = s;
-            V.As[3 * n + j] = y[j];
+            const size_t i = vidx3(n, j);
+            T s = sf(i);
+            V.s[i] = s;
+            V.As[i] = y[j];
             v[0] += (double)y[j] * (double)s;
             v[1] += (double)y[j] * (double)y[j];
-            v[2] += (double)y[j] * (double)V.q[3 * n + j];
+            v[2] += (double)y[j] * (double)V.q[i];
         }
     }
     const int op[3] = {0, 0, 0};
@@ -362,7 +402,7 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<T> V, double *part,
     double v[1] = {0.0};
     if (n < N) {
         for (int j = 0; j < 3; j++) {
-            int i = 3 * n + j;
+            const size_t i = vidx3(n, j);
             T s = V.s[i], p = V.p[i];
             T u = V.u[i];
             u += ta * p;
@@ -383,6 +423,119 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<T> V, double *part,
         sc->rnorm = out[0];
         sc->it += 1;
         sc->done = !(sc->rnorm > sc->tol && sc->it < sc->max_it);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB with the profile-LU preconditioner (FP64).  Same recurrences as
+// K1..K3 above; the product M^-1 A x is split into a plain SpMV and the block
+// triangular solves (sw2_blk.hip), so each half-iteration is SpMV, solve,
+// then a reduction kernel.  K3 is shared.
+
+// y = A x with x = p (S = false) or x = s = r - alpha Ap (S = true, s stored)
+template <int K, bool S>
+__global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<double> V, const int *sell_ptr, const int *sell_col,
+                                                 const double *x, double *y, const KScal *sc, int check_done) {
+    if (check_done && sc->done) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double a[3];
+    if constexpr (S) {
+        VecS<double> sf{V.r, V.Ap, -sc->alpha};
+        amult_prec_row<double, K, VecS<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr, sf, a);
+        for (int j = 0; j < 3; j++) V.s[vidx3(n, j)] = sf(vidx3(n, j));
+    } else {
+        amult_prec_row<double, K, VecX<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr,
+                                                       VecX<double>{x}, a);
+    }
+    for (int j = 0; j < 3; j++) y[vidx3(n, j)] = a[j];
+}
+
+// b = rhs (the reference solves in place on its right-hand side)
+__global__ void k_pl_copy(size_t n, const double *src, double *dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// u = b' (= M^-1 b), ||b'||inf
+__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (n < N)
+        for (int j = 0; j < 3; j++) {
+            const size_t i = vidx3(n, j);
+            const double y = V.b[i];
+            V.u[i] = y;
+            v[0] = fmax(v[0], fabs(y));
+        }
+    const int op[1] = {1};
+    double out[1];
+    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) sc->bnorm = out[0];
+}
+
+// r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
+__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (n < N)
+        for (int j = 0; j < 3; j++) {
+            const size_t i = vidx3(n, j);
+            const double r = V.b[i] + (-1.0) * V.As[i];
+            V.r[i] = r;
+            V.q[i] = r;
+            V.p[i] = r;
+            v[0] = fmax(v[0], fabs(r));
+            v[1] += r * r;
+        }
+    const int op[2] = {1, 0};
+    double out[2];
+    if (reduce_last<2>(v, op, part, count, out) && threadIdx.x == 0) {
+        sc->rnorm = out[0];
+        sc->rho = out[1];
+        sc->rhop = 1.0;
+        sc->alpha = 1.0;
+        sc->omega = 1.0;
+        sc->tol = sc->atol + sc->bnorm * sc->rtol;
+        sc->it = 0;
+        sc->done = !(sc->rnorm > sc->tol && 0 < sc->max_it);
+    }
+}
+
+// gamma = (q, Ap), alpha = rho / gamma
+__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+    if (sc->done) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (n < N)
+        for (int j = 0; j < 3; j++) v[0] += V.q[vidx3(n, j)] * V.Ap[vidx3(n, j)];
+    const int op[1] = {0};
+    double out[1];
+    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) {
+        sc->gamma = out[0];
+        sc->alpha = sc->rho / sc->gamma;
+    }
+}
+
+// omega = (As, s) / (As, As), rho' = -omega (As, q)
+__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+    if (sc->done) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (n < N)
+        for (int j = 0; j < 3; j++) {
+            const size_t i = vidx3(n, j);
+            const double y = V.As[i];
+            v[0] += y * V.s[i];
+            v[1] += y * y;
+            v[2] += y * V.q[i];
+        }
+    const int op[3] = {0, 0, 0};
+    double out[3];
+    if (reduce_last<3>(v, op, part, count, out) && threadIdx.x == 0) {
+        double omega = out[0] / out[1];
+        sc->omega = omega;
+        sc->rhop = sc->rho;
+        sc->rho = -1.0 * omega * out[2];
     }
 }
 
@@ -408,38 +561,100 @@ KVecs<float> vecs<float>(Ctx *c) {
     return KVecs<float>{c->diag32, c->offd32, c->dinv32, c->fb, c->fr, c->fp, c->fAp, c->fs, c->fAs, c->fq, c->fu};
 }
 
-template <typename T>
-static void prologue(Ctx *c) {
+template <typename T, int K>
+static void prologue_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     KVecs<T> V = vecs<T>(c);
     hipLaunchKernelGGL(k_bcg_p1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, c->red.count + 0,
                        c->ksc);
-    hipLaunchKernelGGL(k_bcg_p2<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col, c->red.part,
-                       c->red.count + 1, c->ksc);
+    hipLaunchKernelGGL((k_bcg_p2<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+                       c->red.part, c->red.count + 1, c->ksc);
 }
 
-template <typename T>
-static void iters(Ctx *c, int k) {
+template <typename T, int K>
+static void iters_k(Ctx *c, int k) {
     int nb = nblocks_for(c->N, 256);
     KVecs<T> V = vecs<T>(c);
     for (int i = 0; i < k; i++) {
-        hipLaunchKernelGGL(k_bcg_k1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col, c->red.part,
-                           c->red.count + 2, c->ksc);
-        hipLaunchKernelGGL(k_bcg_k2<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col, c->red.part,
-                           c->red.count + 3, c->ksc);
+        hipLaunchKernelGGL((k_bcg_k1<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+                           c->red.part, c->red.count + 2, c->ksc);
+        hipLaunchKernelGGL((k_bcg_k2<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+                           c->red.part, c->red.count + 3, c->ksc);
         hipLaunchKernelGGL(k_bcg_k3<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 4,
                            c->ksc);
     }
 }
 
-void launch_k1_only(Ctx *c) {
+template <typename T, int K>
+static void k1_only_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
-    if (c->opts.precision == 1)
-        hipLaunchKernelGGL(k_bcg_k1<float>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<float>(c), c->sell_ptr,
-                           c->sell_col, c->red.part, c->red.count + 2, c->ksc);
-    else
-        hipLaunchKernelGGL(k_bcg_k1<double>, dim3(nb), dim3(256), 0, c->stream, c->N, vecs<double>(c), c->sell_ptr,
-                           c->sell_col, c->red.part, c->red.count + 2, c->ksc);
+    hipLaunchKernelGGL((k_bcg_k1<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, vecs<T>(c), c->sell_ptr,
+                       c->sell_col, c->red.part, c->red.count + 2, c->ksc);
+}
+
+// dispatch on the uniform slice width (0 = per-slice widths, looped)
+#define SW2_KSWITCH(FN, T, ...)                          \
+    switch (c->kuni) {                                   \
+        case 3: FN<T, 3>(__VA_ARGS__); break;            \
+        case 4: FN<T, 4>(__VA_ARGS__); break;            \
+        case 5: FN<T, 5>(__VA_ARGS__); break;            \
+        case 6: FN<T, 6>(__VA_ARGS__); break;            \
+        case 7: FN<T, 7>(__VA_ARGS__); break;            \
+        case 8: FN<T, 8>(__VA_ARGS__); break;            \
+        case 9: FN<T, 9>(__VA_ARGS__); break;            \
+        case 10: FN<T, 10>(__VA_ARGS__); break;          \
+        default: FN<T, 0>(__VA_ARGS__); break;           \
+    }
+
+template <typename T, int K>
+static void pl_prologue_k(Ctx *c) {
+    const int nb = nblocks_for(c->N, 256);
+    KVecs<double> V = vecs<double>(c);
+    const size_t n3 = 3 * (size_t)c->Np;
+    hipLaunchKernelGGL(k_pl_copy, dim3(nblocks_for((int)n3, 256)), dim3(256), 0, c->stream, n3, c->R, V.b);
+    launch_band_solve(c, V.b, nullptr);
+    hipLaunchKernelGGL(k_pl_p1, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 0, c->ksc);
+    hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+                       V.u, V.As, c->ksc, 0);
+    launch_band_solve(c, V.As, nullptr);
+    hipLaunchKernelGGL(k_pl_p2, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 1, c->ksc);
+}
+
+template <typename T, int K>
+static void pl_iters_k(Ctx *c, int k) {
+    const int nb = nblocks_for(c->N, 256);
+    KVecs<double> V = vecs<double>(c);
+    for (int i = 0; i < k; i++) {
+        hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
+                           c->sell_col, V.p, V.Ap, c->ksc, 1);
+        launch_band_solve(c, V.Ap, c->ksc);
+        hipLaunchKernelGGL(k_pl_gamma, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 2,
+                           c->ksc);
+        hipLaunchKernelGGL((k_pl_spmv<K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
+                           c->sell_col, nullptr, V.As, c->ksc, 1);
+        launch_band_solve(c, V.As, c->ksc);
+        hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 3,
+                           c->ksc);
+        hipLaunchKernelGGL(k_bcg_k3<double>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
+                           c->red.count + 4, c->ksc);
+    }
+}
+
+template <typename T>
+static void prologue(Ctx *c) {
+    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_prologue_k, T, c) }
+    else { SW2_KSWITCH(prologue_k, T, c) }
+}
+
+template <typename T>
+static void iters(Ctx *c, int k) {
+    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_iters_k, T, c, k) }
+    else { SW2_KSWITCH(iters_k, T, c, k) }
+}
+
+void launch_k1_only(Ctx *c) {
+    if (c->opts.precision == 1) { SW2_KSWITCH(k1_only_k, float, c) }
+    else { SW2_KSWITCH(k1_only_k, double, c) }
 }
 
 void launch_krylov_prologue(Ctx *c) {
@@ -453,7 +668,7 @@ void launch_krylov_iters(Ctx *c, int k) {
 }
 
 void launch_krylov_epilogue(Ctx *c) {
-    int n3 = 3 * c->N;
+    int n3 = 3 * c->Np;
     if (c->opts.precision == 1)
         hipLaunchKernelGGL(k_bcg_epi<float>, dim3(nblocks_for(n3, 256)), dim3(256), 0, c->stream, n3, c->fu, c->scale,
                            c->sol);

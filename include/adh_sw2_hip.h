@@ -84,8 +84,10 @@ typedef struct {
     double lin_rtol;     /* BiCGSTAB stop: ||r||inf <= lin_atol + lin_rtol*||M^-1 b||inf */
     double lin_atol;
     int precision;       /* 0: FP64 Krylov; 1: FP32 matrix + Krylov, FP64 residual and update */
-    int preconditioner;  /* 2: 3x3 block Jacobi */
+    int preconditioner;  /* 2: 3x3 block Jacobi; 1: profile LU of node blocks (OP PRE 1 with OP BLK
+                            nblock, src/solver/solv_blk_*.c), FP64 only */
     int check_every;     /* Krylov iterations between host convergence checks */
+    int nblock;          /* OP BLK: node blocks of the profile-LU preconditioner (default 1) */
 } adh_sw2_opts;
 
 typedef struct {

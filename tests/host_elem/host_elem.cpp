@@ -50,6 +50,14 @@ void he_body(void *p, int ie, double pert, int pn, int pv, int ps, int dry_rule,
     memcpy(out, r, sizeof(r));
 }
 
+// the fully wet variant of the element residual (valid when all depths > 0)
+void he_body_wet(void *p, int ie, double pert, int pn, int pv, int ps, double *out) {
+    d3 r[3];
+    const ElemData &D = ((HostElem *)p)->D;
+    body_resid<true>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, pert, pn, pv, ps, false, r);
+    memcpy(out, r, sizeof(r));
+}
+
 void he_boundary(void *p, int ie, double pert, int pn, int pv, int ps, int dry_zero, double *out) {
     d3 r[3];
     boundary_resid(((HostElem *)p)->D, ie, pert, pn, pv, ps, dry_zero != 0, r);

@@ -7,6 +7,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "host_elem", "host_elem.cpp")
+CSRC = os.path.join(os.path.dirname(HERE), "adhv8_kraken_amd", "csrc")
+DEPS = [SRC, os.path.join(CSRC, "sw2_elem.h"), os.path.join(CSRC, "cr_log.h")]
 LIB = os.path.join(HERE, "host_elem", "libhost_elem.so")
 PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int)
@@ -22,7 +24,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if (not os.path.exists(LIB)) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+        if (not os.path.exists(LIB)) or os.path.getmtime(LIB) < max(os.path.getmtime(d) for d in DEPS):
             build()
         L = C.CDLL(LIB)
         L.he_create.restype = C.c_void_p
@@ -30,6 +32,7 @@ def lib():
                                             PD, PD] + [C.c_double] * 7 + [C.c_int]
         L.he_destroy.argtypes = [C.c_void_p]
         L.he_body.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
+        L.he_body_wet.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, PD]
         L.he_boundary.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
         L.he_elem_jac.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, PD]
         L.he_cr_log.argtypes = [C.c_double]
@@ -79,6 +82,11 @@ class HostElem:
     def body(self, ie, pert=0.0, pn=0, pv=-1, ps=0, dry_rule=False):
         out = np.empty(9)
         self.L.he_body(self.h, ie, pert, pn, pv, ps, int(dry_rule), out.ctypes.data_as(PD))
+        return out
+
+    def body_wet(self, ie, pert=0.0, pn=0, pv=-1, ps=0):
+        out = np.empty(9)
+        self.L.he_body_wet(self.h, ie, pert, pn, pv, ps, out.ctypes.data_as(PD))
         return out
 
     def boundary(self, ie, pert=0.0, pn=0, pv=-1, ps=0, dry_zero=False):

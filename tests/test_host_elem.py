@@ -77,3 +77,40 @@ def test_bumpy_channel_varied_roughness_bit_exact():
     o = _stepped(p)
     bad, _ = _compare(o, p, range(0, p.nelems2d, 2))
     assert not bad, bad[:10]
+
+
+def _eps(v):
+    """num_diff_eps (fe_sw2_load perturbation size) restated for the split test."""
+    pert = 1.4901161193847656e-08
+    e = abs(v) * (1.0 + pert)
+    e -= abs(v)
+    return pert if e < pert else e
+
+
+def test_wet_variant_matches_general_code_on_wet_triangles():
+    """The element kernels run triangles whose depths stay positive under the
+    -eps perturbation through a variant compiled without the wet/dry clipping
+    code; on those triangles it must equal the general code bit for bit, for
+    the unperturbed and every perturbed residual."""
+    p = mesh.floodplain(20, 10, nsteps=1)
+    o = _stepped(p, newton_its=3)
+    st = o.get_state()
+    wd = o.wdflag()
+    dj, gr, dj1, nr1 = o.geometry()
+    H = HostElem(p, st, wd, dj, gr, dj1, nr1)
+    e = p.e2d.reshape(-1, 3)
+    h = st[0]
+    wet = [ie for ie in range(p.nelems2d) if all(h[n] + (-1.0) * _eps(h[n]) > 0.0 for n in e[ie])]
+    assert 0 < len(wet) < p.nelems2d
+    bad = []
+    for ie in wet:
+        if not np.array_equal(H.body(ie).view(np.int64), H.body_wet(ie).view(np.int64)):
+            bad.append(("body", ie))
+        for pv in (1, 2, 5):
+            for pn in range(3):
+                for ps in (1, -1):
+                    a = H.body(ie, 1e-7, pn, pv, ps)
+                    b = H.body_wet(ie, 1e-7, pn, pv, ps)
+                    if not np.array_equal(a.view(np.int64), b.view(np.int64)):
+                        bad.append(("pert", ie, pn, pv, ps))
+    assert not bad, bad[:10]
