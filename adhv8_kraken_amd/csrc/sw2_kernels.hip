@@ -418,11 +418,10 @@ void launch_probe(const double *x, int n, const crlog::tables *t, double *lg, do
 // ------------------------------------------------------------------- norms
 // get_residual_norms (fe_newton.c:79-150) with include_node_in_residual
 // (:177-187): max norms over nodes with min(h, h_old) > 0, l2 over all DOFs.
-// Block partials, then the last block to finish reduces them in block order.
+// Block partials, then a one-block finisher reduces them in block order.
 __global__ void __launch_bounds__(256) k_norms(int N, const double *R, const double *sol, const double *h,
-                                               const double *oh, double *part, unsigned *count, Norms *out) {
+                                               const double *oh, double *part) {
     __shared__ double smax[256], ssum[256], sinc[256];
-    __shared__ bool last;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0, s = 0.0, in = 0.0;
     if (n < N) {
@@ -448,15 +447,14 @@ __global__ void __launch_bounds__(256) k_norms(int N, const double *R, const dou
         part[3 * blockIdx.x] = smax[0];
         part[3 * blockIdx.x + 1] = ssum[0];
         part[3 * blockIdx.x + 2] = sinc[0];
-        __threadfence();
-        unsigned prev = atomicAdd(count, 1u);
-        last = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+// one block: the block partials of k_norms, reduced in block order
+__global__ void __launch_bounds__(256) k_norms_fin(int nparts, int N, const double *part, Norms *out) {
+    __shared__ double smax[256], ssum[256], sinc[256];
     double a = 0.0, b = 0.0, cc = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
         a = fmax(a, part[3 * k]);
         b += part[3 * k + 1];
         cc = fmax(cc, part[3 * k + 2]);
@@ -475,14 +473,14 @@ __global__ void __launch_bounds__(256) k_norms(int N, const double *R, const dou
         out->maxn = smax[0];
         out->l2 = sqrt(ssum[0] / (double)(N * 3));
         out->incn = sinc[0];
-        *count = 0;
     }
 }
 
 void launch_norms(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     hipLaunchKernelGGL(k_norms, dim3(nb), dim3(256), 0, c->stream, c->N, c->R, c->sol, c->head, c->old_head,
-                       c->red.part, c->red.count + 7, c->norms);
+                       c->red.part);
+    hipLaunchKernelGGL(k_norms_fin, dim3(1), dim3(256), 0, c->stream, nb, c->N, c->red.part, c->norms);
 }
 
 }  // namespace sw2

@@ -8,11 +8,11 @@
 //   K3  u += alpha p; u += omega s; r = s - omega As;
 //       p = beta (p - omega Ap) + r with beta = (rho' alpha)/(rho omega); ||r||inf
 //
-// Each kernel ends in a single-pass reduction (block partials, the last block
-// to finish reduces them in block order and updates the device scalars), so a
-// BiCGSTAB iteration is 3 launches with no host round trip; the host reads
-// the convergence flag every opts.check_every iterations and kernels launched
-// past convergence return at entry.  The same kernels run in FP64 or FP32
+// Each kernel writes block partials of its dot products, and a one-block
+// finisher kernel reduces them in block order and updates the device
+// scalars, so a BiCGSTAB iteration is 6 launches with no host round trip; the
+// host reads the convergence flag every opts.check_every iterations and
+// kernels launched past convergence return at entry.  The same kernels run in FP64 or FP32
 // (matrix, preconditioner and vectors), with dot products accumulated in FP64.
 //
 // HBM layout (wave-sliced, 64 nodes per slice): diagonal blocks at
@@ -135,13 +135,16 @@ void launch_setup(Ctx *c) {
 }
 
 // ------------------------------------------------------------ reductions
-// Block-level reduction of K values (op: 0 sum, 1 max) followed by the
-// last-block pass.  Returns true in the last block; `out` valid in thread 0.
+// Each reduction is split across a kernel boundary: every block of the
+// producing kernel writes its K block partials (wavefront shuffles, then the
+// block's waves in order), and a one-block finisher kernel reduces the
+// partials in block order and updates the device scalars.  The kernel
+// boundary publishes the partials; an in-kernel "last block" scheme would
+// need a device-scope fence per block, which on a multi-XCD part writes back
+// the XCD's L2 and costs more than the SpMV itself.
 template <int K>
-__device__ __forceinline__ bool reduce_last(double (&v)[K], const int (&op)[K], double *part, unsigned *count,
-                                            double (&out)[K]) {
+__device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K], double *part) {
     __shared__ double sh[K][256 / 64];
-    __shared__ bool last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -160,21 +163,22 @@ __device__ __forceinline__ bool reduce_last(double (&v)[K], const int (&op)[K], 
             for (int w = 1; w < nw; w++) x = op[k] ? fmax(x, sh[k][w]) : x + sh[k][w];
             part[K * blockIdx.x + k] = x;
         }
-        __threadfence();
-        unsigned prev = atomicAdd(count, 1u);
-        last = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// One-block reduction of nparts block partials: thread-strided sums in
+// fixed order, then a fixed tree, so the result is deterministic.
+template <int K>
+__device__ __forceinline__ void reduce_partials(int nparts, const double *part, const int (&op)[K], double (&out)[K]) {
+    __shared__ double sh[K][256 / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double acc[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) acc[k] = op[k] ? 0.0 : 0.0;
-    // thread-strided partial sums in fixed order, then a fixed tree: deterministic
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+    for (int k = 0; k < K; k++) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            double x = __builtin_nontemporal_load(part + K * b + k);
+            double x = part[K * b + k];
             acc[k] = op[k] ? fmax(acc[k], x) : acc[k] + x;
         }
 #pragma unroll
@@ -194,9 +198,48 @@ __device__ __forceinline__ bool reduce_last(double (&v)[K], const int (&op)[K], 
             for (int w = 1; w < nw; w++) x = op[k] ? fmax(x, sh[k][w]) : x + sh[k][w];
             out[k] = x;
         }
-        *count = 0;
     }
-    return true;
+}
+
+// finisher stages of the BiCGSTAB reductions (solv_bcgstab.c:117-290)
+enum { FIN_P1, FIN_P2, FIN_K1, FIN_K2, FIN_K3 };
+
+template <int STAGE>
+__global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KScal *sc) {
+    if ((STAGE == FIN_K1 || STAGE == FIN_K2 || STAGE == FIN_K3) && sc->done) return;
+    constexpr int K = STAGE == FIN_P2 ? 2 : (STAGE == FIN_K2 ? 3 : 1);
+    int op[K];
+    if constexpr (STAGE == FIN_P1 || STAGE == FIN_K3) op[0] = 1;
+    else if constexpr (STAGE == FIN_P2) { op[0] = 1; op[1] = 0; }
+    else if constexpr (STAGE == FIN_K1) op[0] = 0;
+    else { op[0] = 0; op[1] = 0; op[2] = 0; }
+    double out[K];
+    reduce_partials<K>(nparts, part, op, out);
+    if (threadIdx.x != 0) return;
+    if constexpr (STAGE == FIN_P1) {
+        sc->bnorm = out[0];
+    } else if constexpr (STAGE == FIN_P2) {
+        sc->rnorm = out[0];
+        sc->rho = out[1];
+        sc->rhop = 1.0;
+        sc->alpha = 1.0;
+        sc->omega = 1.0;
+        sc->tol = sc->atol + sc->bnorm * sc->rtol;
+        sc->it = 0;
+        sc->done = !(sc->rnorm > sc->tol && 0 < sc->max_it);
+    } else if constexpr (STAGE == FIN_K1) {
+        sc->gamma = out[0];
+        sc->alpha = sc->rho / sc->gamma;
+    } else if constexpr (STAGE == FIN_K2) {
+        double omega = out[0] / out[1];
+        sc->omega = omega;
+        sc->rhop = sc->rho;
+        sc->rho = -1.0 * omega * out[2];
+    } else {
+        sc->rnorm = out[0];
+        sc->it += 1;
+        sc->done = !(sc->rnorm > sc->tol && sc->it < sc->max_it);
+    }
 }
 
 // ------------------------------------------------------------ SpMV + M^-1
@@ -301,8 +344,7 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<T> V, const double 
         }
     }
     const int op[1] = {1};
-    double out[1];
-    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) sc->bnorm = out[0];
+    block_partials<1>(v, op, part);
 }
 
 // prologue part 2: r = b' - M^-1 A u, q = r, p = r, rho = (r,q), ||r||inf  (:138-188)
@@ -325,17 +367,7 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *se
         }
     }
     const int op[2] = {1, 0};
-    double out[2];
-    if (reduce_last<2>(v, op, part, count, out) && threadIdx.x == 0) {
-        sc->rnorm = out[0];
-        sc->rho = out[1];
-        sc->rhop = 1.0;
-        sc->alpha = 1.0;
-        sc->omega = 1.0;
-        sc->tol = sc->atol + sc->bnorm * sc->rtol;
-        sc->it = 0;
-        sc->done = !(sc->rnorm > sc->tol && 0 < sc->max_it);
-    }
+    block_partials<2>(v, op, part);
 }
 
 template <typename T, int K>
@@ -354,11 +386,7 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *se
         }
     }
     const int op[1] = {0};
-    double out[1];
-    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) {
-        sc->gamma = out[0];
-        sc->alpha = sc->rho / sc->gamma;
-    }
+    block_partials<1>(v, op, part);
 }
 
 template <typename T, int K>
@@ -383,13 +411,7 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *se
         }
     }
     const int op[3] = {0, 0, 0};
-    double out[3];
-    if (reduce_last<3>(v, op, part, count, out) && threadIdx.x == 0) {
-        double omega = out[0] / out[1];
-        sc->omega = omega;
-        sc->rhop = sc->rho;
-        sc->rho = -1.0 * omega * out[2];
-    }
+    block_partials<3>(v, op, part);
 }
 
 template <typename T>
@@ -418,12 +440,7 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<T> V, double *part,
         }
     }
     const int op[1] = {1};
-    double out[1];
-    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) {
-        sc->rnorm = out[0];
-        sc->it += 1;
-        sc->done = !(sc->rnorm > sc->tol && sc->it < sc->max_it);
-    }
+    block_partials<1>(v, op, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -469,8 +486,7 @@ __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<double> V, double *p
             v[0] = fmax(v[0], fabs(y));
         }
     const int op[1] = {1};
-    double out[1];
-    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) sc->bnorm = out[0];
+    block_partials<1>(v, op, part);
 }
 
 // r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
@@ -488,17 +504,7 @@ __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<double> V, double *p
             v[1] += r * r;
         }
     const int op[2] = {1, 0};
-    double out[2];
-    if (reduce_last<2>(v, op, part, count, out) && threadIdx.x == 0) {
-        sc->rnorm = out[0];
-        sc->rho = out[1];
-        sc->rhop = 1.0;
-        sc->alpha = 1.0;
-        sc->omega = 1.0;
-        sc->tol = sc->atol + sc->bnorm * sc->rtol;
-        sc->it = 0;
-        sc->done = !(sc->rnorm > sc->tol && 0 < sc->max_it);
-    }
+    block_partials<2>(v, op, part);
 }
 
 // gamma = (q, Ap), alpha = rho / gamma
@@ -509,11 +515,7 @@ __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<double> V, double
     if (n < N)
         for (int j = 0; j < 3; j++) v[0] += V.q[vidx3(n, j)] * V.Ap[vidx3(n, j)];
     const int op[1] = {0};
-    double out[1];
-    if (reduce_last<1>(v, op, part, count, out) && threadIdx.x == 0) {
-        sc->gamma = out[0];
-        sc->alpha = sc->rho / sc->gamma;
-    }
+    block_partials<1>(v, op, part);
 }
 
 // omega = (As, s) / (As, As), rho' = -omega (As, q)
@@ -530,13 +532,7 @@ __global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<double> V, double
             v[2] += y * V.q[i];
         }
     const int op[3] = {0, 0, 0};
-    double out[3];
-    if (reduce_last<3>(v, op, part, count, out) && threadIdx.x == 0) {
-        double omega = out[0] / out[1];
-        sc->omega = omega;
-        sc->rhop = sc->rho;
-        sc->rho = -1.0 * omega * out[2];
-    }
+    block_partials<3>(v, op, part);
 }
 
 // solv_return_old (src/solver/solv_return_old.c:23-30): sol = (u + u0) * scale
@@ -567,8 +563,10 @@ static void prologue_k(Ctx *c) {
     KVecs<T> V = vecs<T>(c);
     hipLaunchKernelGGL(k_bcg_p1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, c->red.count + 0,
                        c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
     hipLaunchKernelGGL((k_bcg_p2<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                        c->red.part, c->red.count + 1, c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
 }
 
 template <typename T, int K>
@@ -578,10 +576,13 @@ static void iters_k(Ctx *c, int k) {
     for (int i = 0; i < k; i++) {
         hipLaunchKernelGGL((k_bcg_k1<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                            c->red.part, c->red.count + 2, c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
         hipLaunchKernelGGL((k_bcg_k2<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                            c->red.part, c->red.count + 3, c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
         hipLaunchKernelGGL(k_bcg_k3<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 4,
                            c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
     }
 }
 
@@ -614,10 +615,12 @@ static void pl_prologue_k(Ctx *c) {
     hipLaunchKernelGGL(k_pl_copy, dim3(nblocks_for((int)n3, 256)), dim3(256), 0, c->stream, n3, c->R, V.b);
     launch_band_solve(c, V.b, nullptr);
     hipLaunchKernelGGL(k_pl_p1, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 0, c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
     hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                        V.u, V.As, c->ksc, 0);
     launch_band_solve(c, V.As, nullptr);
     hipLaunchKernelGGL(k_pl_p2, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 1, c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
 }
 
 template <typename T, int K>
@@ -630,13 +633,16 @@ static void pl_iters_k(Ctx *c, int k) {
         launch_band_solve(c, V.Ap, c->ksc);
         hipLaunchKernelGGL(k_pl_gamma, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 2,
                            c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
         hipLaunchKernelGGL((k_pl_spmv<K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
                            c->sell_col, nullptr, V.As, c->ksc, 1);
         launch_band_solve(c, V.As, c->ksc);
         hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 3,
                            c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
         hipLaunchKernelGGL(k_bcg_k3<double>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
                            c->red.count + 4, c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
     }
 }
 
