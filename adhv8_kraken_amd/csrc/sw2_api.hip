@@ -185,6 +185,12 @@ static int alloc_solver(adh_sw2_hip *h) {
     if ((rc = dalloc(h, &c.diag32, 9 * (size_t)c.Np)) || (rc = dalloc(h, &c.offd32, 9 * c.nsell)) ||
         (rc = dalloc(h, &c.dinv32, 9 * (size_t)c.Np)))
         return rc;
+    for (double *v : {c.kb, c.kr, c.kp, c.kAp, c.ks_, c.kAs, c.kq, c.ku}) HIPCHK(hipMemset(v, 0, sizeof(double) * n3));
+    for (float *v : {c.fb, c.fr, c.fp, c.fAp, c.fs, c.fAs, c.fq, c.fu}) HIPCHK(hipMemset(v, 0, sizeof(float) * n3));
+    // SELL padding blocks are read by the SpMV and must stay zero; set-up writes only real blocks
+    HIPCHK(hipMemset(c.offd32, 0, sizeof(float) * 9 * (c.nsell ? c.nsell : 1)));
+    HIPCHK(hipMemset(c.diag32, 0, sizeof(float) * 9 * (size_t)c.Np));
+    HIPCHK(hipMemset(c.dinv32, 0, sizeof(float) * 9 * (size_t)c.Np));
     return 0;
 }
 
@@ -395,10 +401,16 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     UP(offd_pos, c.h_offd_pos.data(), c.nnz);
     UP(slot_ptr, slot_ptr.data(), nslots + 1);
     UP(slot_src, slot_src.data(), slot_src.size());
+    {   // element block -> its position in the slot-ordered contributor list
+        c.h_blkpos.assign(slot_src.size(), 0);
+        for (size_t p = 0; p < slot_src.size(); p++) c.h_blkpos[slot_src[p]] = (int)p;
+        UP(blkpos, c.h_blkpos.data(), c.h_blkpos.size());
+    }
     UP(pre, (const double *)nullptr, PRE_STRIDE * (size_t)E);
     UP(elist_wet, (const int *)nullptr, E);
     UP(elist_gen, (const int *)nullptr, E);
     UP(ecount, (const int *)nullptr, 2);
+    UP(esplit, (const int *)nullptr, (size_t)nblocks_for(E, 256) + 1);
     c.pre_valid = 0;
     UP(ER, (const double *)nullptr, 9 * (size_t)E + 6 * (size_t)B);
     UP(EM, (const double *)nullptr, 81 * (size_t)E + 36 * (size_t)B);
@@ -626,8 +638,15 @@ extern "C" int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
-    if (em2d) HIPCHK(hipMemcpy(em2d, c.EM, sizeof(double) * 81 * c.E, hipMemcpyDeviceToHost));
-    if (em1d && c.B) HIPCHK(hipMemcpy(em1d, c.EM + 81 * (size_t)c.E, sizeof(double) * 36 * c.B, hipMemcpyDeviceToHost));
+    // the device keeps element blocks in slot order (k_assemble reads them contiguously)
+    std::vector<double> ems(9 * c.h_blkpos.size());
+    HIPCHK(hipMemcpy(ems.data(), c.EM, sizeof(double) * ems.size(), hipMemcpyDeviceToHost));
+    const size_t nb2 = 9 * (size_t)c.E;
+    for (size_t blk = 0; blk < c.h_blkpos.size(); blk++) {
+        double *dst = blk < nb2 ? (em2d ? em2d + 9 * blk : nullptr) : (em1d ? em1d + 9 * (blk - nb2) : nullptr);
+        if (!dst) continue;
+        for (int k = 0; k < 9; k++) dst[k] = ems[9 * (size_t)c.h_blkpos[blk] + k];
+    }
     return 0;
 }
 

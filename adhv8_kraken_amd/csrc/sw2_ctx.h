@@ -62,6 +62,8 @@ struct Ctx {
     // pattern + assembly
     int *row_ptr, *col;    // off-diagonal blocks, insertion order
     int *slot_ptr, *slot_src;  // N diagonal slots then nnz off-diagonal slots
+    int *blkpos;               // element block (9e+3i+j, then 9E+4b+2i+j) -> contributor position
+    std::vector<int> h_blkpos;
     std::vector<int> h_row_ptr, h_col;
     // Off-diagonal blocks in a wave-sliced ELL layout (SELL-64): block rows
     // 64s..64s+63 form slice s with K_s = sell_ptr[s+1]-sell_ptr[s] block
@@ -77,10 +79,11 @@ struct Ctx {
     std::vector<int> h_offd_pos;
     // work
     int *elist_wet, *elist_gen, *ecount;  // FD-Jacobian triangle lists [E], [E], counts [2]
+    int *esplit;                          // per-block counts / offsets of the split [E/256 + 1]
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state
     double *ER;            // element residuals [9E + 6B]
-    double *EM;            // element matrices [81E + 36B]
+    double *EM;            // element 3x3 blocks [81E + 36B], in slot-ordered contributor positions
     // Node vectors and 3x3 node blocks use the same wave-sliced layout as
     // the off-diagonal blocks: nodes 64s..64s+63 form slice s, and component
     // k of node n sits at vidx3(n, k) (entry e of a block at didx9(n, e)), so

@@ -98,7 +98,7 @@ void launch_resid(Ctx *c) {
 // positive under every perturbation and run without the wet/dry clipping code.
 template <bool WET>
 __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list, const int *count, int b0, int b1,
-                                                    int b2, double *EM) {
+                                                    int b2, const int *blkpos, double *EM) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int li = t / 18;
     if (li >= *count) return;  // both lanes of a pair leave together
@@ -125,16 +125,17 @@ __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list,
     const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
     const int lb[3] = {b0, b1, b2};
     jac_column_db<3>(D, nodes, lb, j, var, col);
-    double *o = EM + 81 * (size_t)ie;
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++) {
+        double *o = EM + 9 * (size_t)blkpos[9 * ie + 3 * i + j];
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++) o[(3 * i + j) * 9 + 3 * rr + var] = col[3 * i + rr];
+        for (int rr = 0; rr < 3; rr++) o[3 * rr + var] = col[3 * i + rr];
+    }
 }
-template __global__ void k_jac2d_list<true>(ElemData, const int *, const int *, int, int, int, double *);
-template __global__ void k_jac2d_list<false>(ElemData, const int *, const int *, int, int, int, double *);
+template __global__ void k_jac2d_list<true>(ElemData, const int *, const int *, int, int, int, const int *, double *);
+template __global__ void k_jac2d_list<false>(ElemData, const int *, const int *, int, int, int, const int *, double *);
 
-__global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, double *EM1) {
+__global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, const int *blkpos1, double *EM) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * B) return;
     int ie = t / 6, q = t % 6, j = q / 3, var = q % 3;
@@ -143,63 +144,95 @@ __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1
     const int nodes[2] = {D.e1d[2 * ie], D.e1d[2 * ie + 1]};
     const int lb[2] = {b0, b1};
     jac_column_db<2>(D, nodes, lb, j, var, col);
-    double *o = EM1 + 36 * (size_t)ie;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
+        double *o = EM + 9 * (size_t)blkpos1[4 * ie + 2 * i + j];
 #pragma unroll
-        for (int r = 0; r < 3; r++) o[(2 * i + j) * 9 + 3 * r + var] = col[3 * i + r];
+        for (int r = 0; r < 3; r++) o[3 * r + var] = col[3 * i + r];
+    }
 }
 
 // Splits the triangles into those whose three depths stay positive under the
 // -eps perturbation of the FD Jacobian (h - eps(h) > 0, the exact operation
-// the perturbed residual performs) and the rest.  One atomic per wavefront;
-// list order is irrelevant (each triangle writes only its own block).
-__global__ void __launch_bounds__(256) k_wet_split(int E, const int *e2d, const double *head, int *wet, int *gen,
-                                                   int *counts) {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    bool is_wet = false;
-    if (ie < E) {
-        is_wet = true;
-        for (int i = 0; i < 3; i++) {
-            const double h = head[e2d[3 * ie + i]];
-            if (!(h + (-1.0) * num_diff_eps(h) > 0.0)) is_wet = false;
-        }
+// the perturbed residual performs) and the rest, as two ordered lists: block
+// counts, a one-block scan, then the ordered scatter (no same-address
+// atomics, which serialise across the whole grid).
+__device__ __forceinline__ bool tri_stays_wet(int ie, const int *e2d, const double *head) {
+    bool w = true;
+    for (int i = 0; i < 3; i++) {
+        const double h = head[e2d[3 * ie + i]];
+        if (!(h + (-1.0) * num_diff_eps(h) > 0.0)) w = false;
     }
-    const unsigned long long bw = __ballot(ie < E && is_wet), bg = __ballot(ie < E && !is_wet);
-    const int lane = threadIdx.x & 63;
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int base_w = 0, base_g = 0;
-    if (lane == 0) {
-        if (bw) base_w = atomicAdd(counts, __popcll(bw));
-        if (bg) base_g = atomicAdd(counts + 1, __popcll(bg));
-    }
-    base_w = __shfl(base_w, 0, 64);
-    base_g = __shfl(base_g, 0, 64);
-    if (ie < E) {
-        if (is_wet) wet[base_w + __popcll(bw & below)] = ie;
-        else gen[base_g + __popcll(bg & below)] = ie;
-    }
+    return w;
 }
 
-__global__ void k_zero_int(int n, int *p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+__global__ void __launch_bounds__(256) k_split_count(int E, const int *e2d, const double *head, int *bcount) {
+    __shared__ int wc[4];
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool w = ie < E && tri_stays_wet(ie, e2d, head);
+    const unsigned long long bw = __ballot(w);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(bw);
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// exclusive scan of the block counts (one block); counts[0] = wet, counts[1] = rest
+__global__ void __launch_bounds__(256) k_split_scan(int nb, int E, int *bcount, int *counts) {
+    __shared__ int tot[256];
+    const int per = (nb + 255) / 256, b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; b++) s += bcount[b];
+    tot[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < 256; t++) { int v = tot[t]; tot[t] = run; run += v; }
+        counts[0] = run;
+        counts[1] = E - run;
+    }
+    __syncthreads();
+    int run = tot[threadIdx.x];
+    for (int b = b0; b < b1; b++) { int v = bcount[b]; bcount[b] = run; run += v; }
+}
+
+__global__ void __launch_bounds__(256) k_split_write(int E, const int *e2d, const double *head, const int *boff,
+                                                     int *wet, int *gen) {
+    __shared__ int wc[4];
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool w = ie < E && tri_stays_wet(ie, e2d, head);
+    const unsigned long long bw = __ballot(w);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wc[wid] = __popcll(bw);
+    __syncthreads();
+    int before = 0;  // wet triangles of this block in earlier waves
+    for (int k = 0; k < wid; k++) before += wc[k];
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int wet_rank = boff[blockIdx.x] + before + __popcll(bw & below);
+    const int idx_in_block = threadIdx.x;
+    const int gen_rank = (blockIdx.x * blockDim.x - boff[blockIdx.x]) + (idx_in_block - (before + __popcll(bw & below)));
+    if (ie < E) {
+        if (w) wet[wet_rank] = ie;
+        else gen[gen_rank] = ie;
+    }
 }
 
 static void launch_jac2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, 2, c->ecount);
-    hipLaunchKernelGGL(k_wet_split, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
-                       c->elist_wet, c->elist_gen, c->ecount);
+    const int nb = nblocks_for(c->E, 256);
+    hipLaunchKernelGGL(k_split_count, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->esplit);
+    hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(256), 0, c->stream, nb, c->E, c->esplit, c->ecount);
+    hipLaunchKernelGGL(k_split_write, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->esplit,
+                       c->elist_wet, c->elist_gen);
     (void)hipMemcpyAsync(c->h_flag, c->ecount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     (void)hipStreamSynchronize(c->stream);
     const int nwet = c->h_flag[0], ngen = c->h_flag[1];
     if (nwet > 0)
         hipLaunchKernelGGL(k_jac2d_list<true>, dim3(nblocks_for(18 * nwet, 128)), dim3(128), 0, c->stream, D,
-                           c->elist_wet, c->ecount, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->EM);
+                           c->elist_wet, c->ecount, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM);
     if (ngen > 0)
         hipLaunchKernelGGL(k_jac2d_list<false>, dim3(nblocks_for(18 * ngen, 128)), dim3(128), 0, c->stream, D,
-                           c->elist_gen, c->ecount + 1, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->EM);
+                           c->elist_gen, c->ecount + 1, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos,
+                           c->EM);
 }
 
 void launch_jacobian(Ctx *c) {
@@ -209,18 +242,20 @@ void launch_jacobian(Ctx *c) {
     launch_jac2d_split(c, D);
     if (c->B)
         hipLaunchKernelGGL(k_jac1d, dim3(nblocks_for(6 * c->B, 128)), dim3(128), 0, c->stream, D, c->B, c->loc_bc[0],
-                           c->loc_bc[1], c->EM + 81 * (size_t)c->E);
+                           c->loc_bc[1], c->blkpos + 9 * (size_t)c->E, c->EM);
 }
 
-// Ordered block assembly: thread per (slot, entry).  slot_src lists 9-double
-// element-block offsets in EM in ascending element order, 2D before 1D.
+// Ordered block assembly: thread per (slot, entry).  The element kernels store
+// each 3x3 block at its contributor position (slot order; within a slot the
+// contributors in ascending element order, 2D before 1D), so a slot's sum
+// reads one contiguous run.
 __global__ void k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_src, const int *offd_pos,
                            const double *EM, double *diag, double *offd) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nslots * 9) return;
     int s = (int)(t / 9), k = (int)(t % 9);
     double a = 0.0;
-    for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) a += EM[9 * (size_t)slot_src[p] + k];
+    for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) a += EM[9 * (size_t)p + k];
     if (s < N) diag[didx9(s, k)] = a;
     else offd[sell_vidx((size_t)offd_pos[s - N], k)] = a;
 }

@@ -54,36 +54,12 @@ __global__ void k_scale_rows(int N, const double *diag, const double *offd, cons
     for (int j = 0; j < 3; j++) sv[vidx3(n, j)] = s[j] < SOLV_TOL ? small_factor : 1.0 / sqrt(fabs(s[j]));
 }
 
-__global__ void k_scale_apply(int N, double *diag, double *offd, const int *row_ptr, const int *sell_ptr,
-                              const int *sell_col, const double *sv, double *R, double *sol) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    double rf[3] = {sv[vidx3(n, 0)], sv[vidx3(n, 1)], sv[vidx3(n, 2)]};
-    for (int j = 0; j < 3; j++) {
-        R[vidx3(n, j)] *= rf[j];
-        sol[vidx3(n, j)] /= rf[j];
-    }
-    for (int j = 0; j < 3; j++)
-        for (int k = 0; k < 3; k++) diag[didx9(n, 3 * j + k)] *= rf[j] * rf[k];
-    const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
-    const int len = row_ptr[n + 1] - row_ptr[n];
-    for (int b = 0; b < len; b++) {
-        const size_t q = q0 + 64 * (size_t)b;
-        const int cn = sell_col[q];
-        const double sc[3] = {sv[vidx3(cn, 0)], sv[vidx3(cn, 1)], sv[vidx3(cn, 2)]};
-        for (int j = 0; j < 3; j++)
-            for (int k = 0; k < 3; k++) offd[sell_vidx(q, 3 * j + k)] *= rf[j] * sc[k];
-    }
-}
-
 // 3x3 block inverse by Gauss-Jordan with partial pivoting (block-Jacobi set-up)
-__global__ void k_bj_setup(int N, const double *diag, double *dinv, float *dinv32) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+__device__ __forceinline__ void inv3(const double (&d)[9], double (&inv)[9]) {
     double a[3][6];
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) {
-            a[r][c] = diag[didx9(n, 3 * r + c)];
+            a[r][c] = d[3 * r + c];
             a[r][3 + c] = (r == c) ? 1.0 : 0.0;
         }
     for (int k = 0; k < 3; k++) {
@@ -102,10 +78,56 @@ __global__ void k_bj_setup(int N, const double *diag, double *dinv, float *dinv3
         }
     }
     for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) {
-            dinv[didx9(n, 3 * r + c)] = a[r][3 + c];
-            if (dinv32) dinv32[didx9(n, 3 * r + c)] = (float)a[r][3 + c];
+        for (int c = 0; c < 3; c++) inv[3 * r + c] = a[r][3 + c];
+}
+
+// Applies the scaling (A <- S A S, b <- S b, x <- x / S) and sets up the
+// block-Jacobi inverses in one pass over the matrix.  F32: the scaled
+// matrix and inverses are stored only as the FP32 copies the FP32 Krylov
+// reads (rounded from the same FP64 values); BJ = false (profile-LU
+// preconditioner): no inverses.
+template <bool F32, bool BJ>
+__global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double *offd, const int *row_ptr,
+                                                     const int *sell_ptr, const int *sell_col, const double *sv,
+                                                     double *R, double *sol, double *dinv, float *diag32,
+                                                     float *offd32, float *dinv32) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double rf[3] = {sv[vidx3(n, 0)], sv[vidx3(n, 1)], sv[vidx3(n, 2)]};
+    for (int j = 0; j < 3; j++) {
+        R[vidx3(n, j)] *= rf[j];
+        sol[vidx3(n, j)] /= rf[j];
+    }
+    double d[9];
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) {
+            const size_t i = didx9(n, 3 * j + k);
+            d[3 * j + k] = diag[i] * (rf[j] * rf[k]);
+            if constexpr (F32) diag32[i] = (float)d[3 * j + k];
+            else diag[i] = d[3 * j + k];
         }
+    if constexpr (BJ) {
+        double inv[9];
+        inv3(d, inv);
+        for (int e = 0; e < 9; e++) {
+            if constexpr (F32) dinv32[didx9(n, e)] = (float)inv[e];
+            else dinv[didx9(n, e)] = inv[e];
+        }
+    }
+    const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
+    const int len = row_ptr[n + 1] - row_ptr[n];
+    for (int b = 0; b < len; b++) {
+        const size_t q = q0 + 64 * (size_t)b;
+        const int cn = sell_col[q];
+        const double sc[3] = {sv[vidx3(cn, 0)], sv[vidx3(cn, 1)], sv[vidx3(cn, 2)]};
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                const size_t i = sell_vidx(q, 3 * j + k);
+                const double v = offd[i] * (rf[j] * sc[k]);
+                if constexpr (F32) offd32[i] = (float)v;
+                else offd[i] = v;
+            }
+    }
 }
 
 __global__ void k_to_f32(size_t n, const double *a, float *b) {
@@ -117,21 +139,21 @@ void launch_setup(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     hipLaunchKernelGGL(k_scale_rows, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
                        c->sell_ptr, c->scale);
-    hipLaunchKernelGGL(k_scale_apply, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
-                       c->sell_ptr, c->sell_col, c->scale, c->R, c->sol);
     if (c->opts.preconditioner == 1) {  // profile LU of the node blocks (FP64)
+        hipLaunchKernelGGL((k_scale_apply<false, false>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
+                           c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
+                           c->offd32, c->dinv32);
         launch_band_setup(c);
         return;
     }
-    bool f32 = c->opts.precision == 1;
-    hipLaunchKernelGGL(k_bj_setup, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->dinv, f32 ? c->dinv32 : nullptr);
-    if (f32) {
-        size_t n9 = 9 * (size_t)c->Np, m9 = 9 * c->nsell;
-        hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for((int)n9, 256)), dim3(256), 0, c->stream, n9, c->diag, c->diag32);
-        if (m9)
-            hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for((int)m9, 256)), dim3(256), 0, c->stream, m9, c->offd,
-                               c->offd32);
-    }
+    if (c->opts.precision == 1)
+        hipLaunchKernelGGL((k_scale_apply<true, true>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
+                           c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
+                           c->offd32, c->dinv32);
+    else
+        hipLaunchKernelGGL((k_scale_apply<false, true>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
+                           c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
+                           c->offd32, c->dinv32);
 }
 
 // ------------------------------------------------------------ reductions
