@@ -16,6 +16,8 @@
 
 using namespace sw2;
 
+static void drop_graphs(Ctx &c);
+
 static thread_local std::string g_err;
 
 static int fail(int code, const std::string &msg) {
@@ -48,6 +50,7 @@ extern "C" void adh_sw2_hip_default_opts(adh_sw2_opts *o) {
     o->preconditioner = 2;
     o->check_every = 8;
     o->nblock = 1;
+    o->graphs = 0;  // measured: no gain over eager launches on MI355X for single models
 }
 
 static int check_opts(const adh_sw2_opts *o) {
@@ -206,6 +209,7 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     Ctx &c = h->c;
     c.N = m->nnodes; c.E = m->nelems2d; c.B = m->nelems1d; c.M = m->nmat;
     c.Np = (c.N + 63) & ~63;
+    c.kgraph_prec = -1;
     if (opts) {
         int rc0 = check_opts(opts);
         if (rc0) { delete h; return rc0; }
@@ -460,6 +464,7 @@ extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     if (!h) return;
     Ctx &c = h->c;
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    drop_graphs(c);
     band_free(&c);
     for (void *p : h->allocs) (void)hipFree(p);
     if (c.h_ksc) (void)hipHostFree(c.h_ksc);
@@ -536,6 +541,7 @@ extern "C" int adh_sw2_hip_set_state(adh_sw2_hip *h, const double *head, const d
     if ((rc = set_pair(c, c.old_head, c.old_u, c.old_v, old_head, old_vel))) return rc;
     if ((rc = set_pair(c, c.older_head, c.older_u, c.older_v, older_head, older_vel))) return rc;
     c.pre_valid = 0;
+    c.split_valid = 0;
     return 0;
 }
 
@@ -681,6 +687,41 @@ extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) 
 }
 
 // BiCGSTAB on the scaled device system; returns the reference's YES/NO.
+static void drop_graphs(Ctx &c) {
+    for (int k = 0; k <= KGRAPH_MAX; k++)
+        if (c.kgraph[k]) { (void)hipGraphExecDestroy(c.kgraph[k]); c.kgraph[k] = nullptr; }
+}
+
+// k BiCGSTAB iterations: a captured graph of the 3k kernels and their
+// finishers, replayed (captured on first use for this k)
+static int run_iters(Ctx &c, int k) {
+    if (k > KGRAPH_MAX || !c.opts.graphs) {
+        launch_krylov_iters(&c, k);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    if (c.kgraph_prec != c.opts.precision || c.kgraph_pc != c.opts.preconditioner || c.kgraph_band != c.band) {
+        drop_graphs(c);
+        c.kgraph_prec = c.opts.precision;
+        c.kgraph_pc = c.opts.preconditioner;
+        c.kgraph_band = c.band;
+    }
+    if (!c.kgraph[k]) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        launch_krylov_iters(&c, k);
+        HIPCHK(hipStreamEndCapture(c.stream, &g));
+        hipError_t e = hipGraphInstantiate(&c.kgraph[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c.kgraph[k] = nullptr;
+            return fail(ADH_SW2_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipGraphLaunch(c.kgraph[k], c.stream));
+    return 0;
+}
+
 static int krylov(Ctx &c, int *lin_it, int *verdict) {
     KScal init{};
     init.atol = c.opts.lin_atol;
@@ -688,16 +729,22 @@ static int krylov(Ctx &c, int *lin_it, int *verdict) {
     init.max_it = c.opts.max_lin_it;
     HIPCHK(hipMemcpyAsync(c.ksc, &init, sizeof(KScal), hipMemcpyHostToDevice, c.stream));
     launch_krylov_prologue(&c);
-    for (;;) {
+    // first chunk: the previous solve's iteration count rounded up to a
+    // multiple of 4 (consecutive Newton systems converge alike), then chunks
+    // of 4; chunk lengths stay in a small set so few graphs are captured.
+    // The device flag makes iterations launched past convergence return at entry.
+    int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
+    for (int chunk = 0;; chunk++) {
         HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         if (c.h_ksc->done) break;
         int left = c.opts.max_lin_it - c.h_ksc->it;
-        int k = c.opts.check_every < left ? c.opts.check_every : left;
-        if (k < 1) k = 1;
-        launch_krylov_iters(&c, k);
-        HIPCHK(hipGetLastError());
+        int want = chunk == 0 ? first : 4;
+        int k = std::max(1, std::min(want, left));
+        int rc = run_iters(c, k);
+        if (rc) return rc;
     }
+    c.last_lin_it = c.h_ksc->it;
     launch_krylov_epilogue(&c);
     HIPCHK(hipGetLastError());
     if (lin_it) *lin_it = c.h_ksc->it;

@@ -12,6 +12,11 @@
 
 namespace sw2 {
 
+constexpr int KUNI_MAX = 10;                              // longest uniform slice of the unrolled SpMV
+constexpr int KGRAPH_MAX = 32;                            // longest captured Krylov chunk
+constexpr int BAND_MAX_DOF = 8190;                        // LDS of one block solve (64 KB)
+constexpr size_t BAND_MAX_BYTES = (size_t)8 << 30;       // band storage limit
+
 // Device-side scalars of one BiCGSTAB solve (solv_bcgstab.c:60-70).
 struct KScal {
     double rho, rhop, alpha, omega, beta, gamma;
@@ -80,6 +85,7 @@ struct Ctx {
     // work
     int *elist_wet, *elist_gen, *ecount;  // FD-Jacobian triangle lists [E], [E], counts [2]
     int *esplit;                          // per-block counts / offsets of the split [E/256 + 1]
+    int split_valid;                      // the lists match the current depths
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state
     double *ER;            // element residuals [9E + 6B]
@@ -105,6 +111,12 @@ struct Ctx {
     Norms *norms, *h_norms;
     int *h_flag;           // pinned scratch
     ElemData ed;
+    // captured BiCGSTAB iteration chunks (k iterations each), replayed with
+    // hipGraphLaunch; rebuilt when the kernels or their arguments change
+    hipGraphExec_t kgraph[KGRAPH_MAX + 1];
+    int kgraph_prec, kgraph_pc;
+    const double *kgraph_band;
+    int last_lin_it;       // iterations of the previous solve (sizes the first chunk)
     // profile-LU (OP BLK) preconditioner, built on demand for opts.nblock
     int band_nblock;       // block count of the current band structure (0 = none)
     int band_nb;           // blocks
@@ -115,9 +127,6 @@ struct Ctx {
     double *band;
 };
 
-constexpr int KUNI_MAX = 10;
-constexpr int BAND_MAX_DOF = 8190;                        // LDS of one block solve (64 KB)
-constexpr size_t BAND_MAX_BYTES = (size_t)8 << 30;       // band storage limit  // longest uniform slice the unrolled SpMV is built for
 
 // value index of entry e (0..8) of SELL slot q
 __host__ __device__ __forceinline__ size_t sell_vidx(size_t q, int e) {

@@ -11,24 +11,33 @@
 // which the reference accumulates (fe_sw2_resid.c:74-191,
 // fe_assemble_matrix.c:166-224) -- so sums are bit-identical to it without
 // atomics.
+#include <algorithm>
+
 #include "sw2_ctx.h"
 
 namespace sw2 {
 
 int nblocks_for(int n, int bs) { return n <= 0 ? 1 : (n + bs - 1) / bs; }
 
+static void launch_resid2d_split(Ctx *c, const ElemData &D);
+
 // ------------------------------------------------------------------ residual
-__global__ void __launch_bounds__(128) k_resid2d(ElemData D, int E, double *ER) {
-    int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= E) return;
-    d3 r[3];
-    body_resid(D, D.pre + PRE_STRIDE * (size_t)ie, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
-    double *o = ER + 9 * (size_t)ie;
+// The same over a triangle list (grid-stride over the device-side count):
+// WET lists hold triangles with positive depths, run without clipping code.
+template <bool WET>
+__global__ void __launch_bounds__(128) k_resid2d_list(ElemData D, const int *list, const int *count, double *ER) {
+    const int n = *count;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x) {
+        const int ie = list[li];
+        d3 r[3];
+        body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
+        double *o = ER + 9 * (size_t)ie;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        o[3 * i + 0] = r[i].x_eq;
-        o[3 * i + 1] = r[i].y_eq;
-        o[3 * i + 2] = r[i].c_eq;
+        for (int i = 0; i < 3; i++) {
+            o[3 * i + 0] = r[i].x_eq;
+            o[3 * i + 1] = r[i].y_eq;
+            o[3 * i + 2] = r[i].c_eq;
+        }
     }
 }
 
@@ -81,7 +90,7 @@ void launch_resid(Ctx *c) {
     if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = c->first_elem;
-    if (c->E) hipLaunchKernelGGL(k_resid2d, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->ER);
+    launch_resid2d_split(c, D);
     if (c->B) hipLaunchKernelGGL(k_resid1d, dim3(nblocks_for(c->B, 128)), dim3(128), 0, c->stream, D, c->B,
                                  c->ER + 9 * (size_t)c->E);
     hipLaunchKernelGGL(k_resid_gather, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->n2e_ptr,
@@ -99,9 +108,11 @@ void launch_resid(Ctx *c) {
 template <bool WET>
 __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list, const int *count, int b0, int b1,
                                                     int b2, const int *blkpos, double *EM) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = 18 * *count;
+    // grid-stride over (triangle, column, sign); the stride is a multiple of
+    // the wavefront, so both lanes of a pair run and leave together
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     const int li = t / 18;
-    if (li >= *count) return;  // both lanes of a pair leave together
     const int ie = list[li];
     const int q = (t >> 1) % 9, j = q / 3, var = q % 3;
     const int sgn = (t & 1) ? -1 : +1;
@@ -121,7 +132,7 @@ __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list,
         col[3 * i + 1] = (r[i].y_eq - my) / eps2;
         col[3 * i + 2] = (r[i].c_eq - mc) / eps2;
     }
-    if (t & 1) return;
+    if (t & 1) continue;
     const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
     const int lb[3] = {b0, b1, b2};
     jac_column_db<3>(D, nodes, lb, j, var, col);
@@ -130,6 +141,7 @@ __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list,
         double *o = EM + 9 * (size_t)blkpos[9 * ie + 3 * i + j];
 #pragma unroll
         for (int rr = 0; rr < 3; rr++) o[3 * rr + var] = col[3 * i + rr];
+    }
     }
 }
 template __global__ void k_jac2d_list<true>(ElemData, const int *, const int *, int, int, int, const int *, double *);
@@ -216,23 +228,39 @@ __global__ void __launch_bounds__(256) k_split_write(int E, const int *e2d, cons
     }
 }
 
-static void launch_jac2d_split(Ctx *c, const ElemData &D) {
+// ordered wet / general triangle lists for the current depths
+static void launch_split(Ctx *c) {
     if (!c->E) return;
     const int nb = nblocks_for(c->E, 256);
     hipLaunchKernelGGL(k_split_count, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->esplit);
     hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(256), 0, c->stream, nb, c->E, c->esplit, c->ecount);
     hipLaunchKernelGGL(k_split_write, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->esplit,
                        c->elist_wet, c->elist_gen);
-    (void)hipMemcpyAsync(c->h_flag, c->ecount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    (void)hipStreamSynchronize(c->stream);
-    const int nwet = c->h_flag[0], ngen = c->h_flag[1];
-    if (nwet > 0)
-        hipLaunchKernelGGL(k_jac2d_list<true>, dim3(nblocks_for(18 * nwet, 128)), dim3(128), 0, c->stream, D,
-                           c->elist_wet, c->ecount, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM);
-    if (ngen > 0)
-        hipLaunchKernelGGL(k_jac2d_list<false>, dim3(nblocks_for(18 * ngen, 128)), dim3(128), 0, c->stream, D,
-                           c->elist_gen, c->ecount + 1, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos,
-                           c->EM);
+    c->split_valid = 1;
+}
+
+// grid-stride launches sized to fill the chip at each kernel's occupancy
+// (the list lengths stay on the device)
+constexpr int JAC_WET_BLOCKS = 4096, JAC_GEN_BLOCKS = 1024, RES_WET_BLOCKS = 2048, RES_GEN_BLOCKS = 512;
+
+static void launch_jac2d_split(Ctx *c, const ElemData &D) {
+    if (!c->E) return;
+    if (!c->split_valid) launch_split(c);
+    hipLaunchKernelGGL(k_jac2d_list<true>, dim3(std::min(JAC_WET_BLOCKS, nblocks_for(18 * c->E, 128))), dim3(128), 0,
+                       c->stream, D, c->elist_wet, c->ecount, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos,
+                       c->EM);
+    hipLaunchKernelGGL(k_jac2d_list<false>, dim3(std::min(JAC_GEN_BLOCKS, nblocks_for(18 * c->E, 128))), dim3(128), 0,
+                       c->stream, D, c->elist_gen, c->ecount + 1, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos,
+                       c->EM);
+}
+
+static void launch_resid2d_split(Ctx *c, const ElemData &D) {
+    if (!c->E) return;
+    if (!c->split_valid) launch_split(c);
+    hipLaunchKernelGGL(k_resid2d_list<true>, dim3(std::min(RES_WET_BLOCKS, nblocks_for(c->E, 128))), dim3(128), 0,
+                       c->stream, D, c->elist_wet, c->ecount, c->ER);
+    hipLaunchKernelGGL(k_resid2d_list<false>, dim3(std::min(RES_GEN_BLOCKS, nblocks_for(c->E, 128))), dim3(128), 0,
+                       c->stream, D, c->elist_gen, c->ecount + 1, c->ER);
 }
 
 void launch_jacobian(Ctx *c) {
@@ -297,7 +325,7 @@ void launch_resid2d_only(Ctx *c) {
     if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = c->first_elem;
-    hipLaunchKernelGGL(k_resid2d, dim3(nblocks_for(c->E, 128)), dim3(128), 0, c->stream, D, c->E, c->ER);
+    launch_resid2d_split(c, D);
 }
 
 void launch_assemble_only(Ctx *c) {
@@ -366,6 +394,7 @@ void launch_old_terms(Ctx *c) {
 
 
 void launch_init(Ctx *c, int *bc_mask_dev) {
+    c->split_valid = 0;
     hipLaunchKernelGGL(k_init_nodes, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->old_head,
                        c->old_u, c->old_v, c->head, c->u, c->v, c->node_bc, c->node_bc_val, bc_mask_dev, c->wd_tmp);
     if (c->E) {
@@ -388,6 +417,7 @@ __global__ void k_shift(int N, double *h, double *u, double *v, double *oh, doub
 
 void launch_shift(Ctx *c) {
     c->pre_valid = 0;
+    c->split_valid = 0;
     hipLaunchKernelGGL(k_shift, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->head, c->u, c->v,
                        c->old_head, c->old_u, c->old_v, c->older_head, c->older_u, c->older_v);
 }
@@ -402,6 +432,7 @@ __global__ void k_inc(int N, const double *sol, double *h, double *u, double *v)
 }
 
 void launch_inc(Ctx *c, double) {
+    c->split_valid = 0;
     hipLaunchKernelGGL(k_inc, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->sol, c->head, c->u,
                        c->v);
 }

@@ -30,15 +30,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(p, precision, steps, warmup, rtol):
+def run_form(p, precision, steps, warmup, rtol, graphs=1):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts, YES
-    g = SW2Model(p, default_opts(precision=precision))
+    g = SW2Model(p, default_opts(precision=precision, graphs=graphs))
     # tolerance: 1e-8 of the initial residual of the first step (the reference's NTL is absolute)
     g.shift()
     g.init()
     r0 = float(np.abs(g.resid()).max())
     g.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
-    g.set_opts(default_opts(precision=precision, tol_nonlin=rtol * r0, max_nonlin_it=40))
+    g.set_opts(default_opts(precision=precision, tol_nonlin=rtol * r0, max_nonlin_it=40, graphs=graphs))
     for i in range(warmup):
         g.timestep()
     newton_its = 0
@@ -147,6 +147,7 @@ def main():
     ap.add_argument("--ncx", type=int, default=1000)
     ap.add_argument("--ncy", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graphs", type=int, default=0, help="1: replay captured HIP graphs of Krylov chunks")
     ap.add_argument("--kernels", action="store_true",
                     help="profiling mode: one time step per form, then the K1 kernel, the FD Jacobian "
                          "and the calibration copy launched back to back (for rocprofv3 counter passes)")
@@ -166,7 +167,7 @@ def main():
     roof = {}
     for prec in (0, 1):
         log(f"form precision={prec}")
-        g, r = run_form(p, prec, args.steps, args.warmup, 1e-8)
+        g, r = run_form(p, prec, args.steps, args.warmup, 1e-8, args.graphs)
         res[prec] = r
         roof[prec] = spmv_roofline(g, p, prec)
         roof[prec]["jac_us"] = round(g.time_kernel(1, reps=5) * 1e3, 1)

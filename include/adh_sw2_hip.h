@@ -88,6 +88,7 @@ typedef struct {
                             nblock, src/solver/solv_blk_*.c), FP64 only */
     int check_every;     /* Krylov iterations between host convergence checks */
     int nblock;          /* OP BLK: node blocks of the profile-LU preconditioner (default 1) */
+    int graphs;          /* 1: replay captured HIP graphs of Krylov iteration chunks; 0 (default): launch eagerly */
 } adh_sw2_opts;
 
 typedef struct {

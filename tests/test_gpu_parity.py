@@ -171,6 +171,35 @@ def test_linear_solve_profile_lu(nblock, stepped):
     assert rel < 1e-9
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_krylov_graph_replays_match_eager(precision, stepped):
+    """The Krylov iterations replay captured HIP graphs: several solves in a
+    row (each replaying graphs of different chunk lengths, with host
+    synchronisation and eager kernels between replays) give bit-identical
+    solutions and iteration counts to eager launches of the same kernels."""
+    name = "bumpy-60x30"
+    p = CASES[name]
+    say(f"graph replays: {name} precision={precision}")
+    res = {}
+    for graphs in (0, 1):
+        g = SW2Model(p, default_opts(precision=precision, graphs=graphs))
+        g.set_state(*stepped[name])
+        g.shift(); g.init(); g.resid(); g.load()
+        diag, offd, rhs = g.get_system()
+        out = []
+        for rep in range(4):
+            g.set_system(diag, offd, rhs)
+            g.linear_sys_setup()
+            vd, x, it = g.linear_sys_solve()
+            assert vd == YES
+            out.append((x, it))
+        g.close()
+        res[graphs] = out
+    for (xe, ie), (xg, ig) in zip(res[0], res[1]):
+        assert ie == ig
+        assert bits_equal(xe, xg)
+
+
 @pytest.mark.parametrize("name", ["channel-40x20", "bumpy-60x30"])
 def test_newton_history(name):
     """One time step of Newton (block-Jacobi BiCGSTAB on both sides): same
