@@ -13,7 +13,8 @@
 // scalars, so a BiCGSTAB iteration is 6 launches with no host round trip; the
 // host reads the convergence flag every opts.check_every iterations and
 // kernels launched past convergence return at entry.  The same kernels run in FP64 or FP32
-// (matrix, preconditioner and vectors), with dot products accumulated in FP64.
+// (precision 0), with FP32 matrix and preconditioner and FP64 vectors
+// (precision 1), or all in FP32 (precision 2); dot products accumulate in FP64.
 //
 // HBM layout (wave-sliced, 64 nodes per slice): diagonal blocks at
 // didx9(n, e), off-diagonal blocks in SELL-64 (sell_vidx), node vectors at
@@ -146,7 +147,7 @@ void launch_setup(Ctx *c) {
         launch_band_setup(c);
         return;
     }
-    if (c->opts.precision == 1)
+    if (c->opts.precision >= 1)
         hipLaunchKernelGGL((k_scale_apply<true, true>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
                            c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
                            c->offd32, c->dinv32);
@@ -271,10 +272,20 @@ __global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KSc
 // Off-diagonal blocks come from the SELL-64 slice of this row's wavefront:
 // every lane runs the slice's K block slots (zero padding past the row's
 // length), and entry e of slot j is one coalesced 64-value read per wave.
-template <typename T, int K, typename XF, bool BJ = true>
-__device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *offd, const int *sell_ptr,
-                                               const int *sell_col, const T *dinv, XF xf, T (&y)[3]) {
-    const T *D = diag + didx9(n, 0);  // entry e at D[64 e]
+// Precision tags of the Krylov solve: matrix/preconditioner storage type M
+// and vector/arithmetic type V (a float matrix entry times a double vector
+// entry is formed in double).
+struct PrecF64 { using M = double; using V = double; };   // precision 0
+struct PrecMix { using M = float; using V = double; };    // precision 1: FP32 matrix, FP64 vectors
+struct PrecF32 { using M = float; using V = float; };     // precision 2: all FP32
+
+template <typename P, int K, typename XF, bool BJ = true>
+__device__ __forceinline__ void amult_prec_row(int n, const typename P::M *diag, const typename P::M *offd,
+                                               const int *sell_ptr, const int *sell_col, const typename P::M *dinv,
+                                               XF xf, typename P::V (&y)[3]) {
+    using T = typename P::V;
+    using MT = typename P::M;
+    const MT *D = diag + didx9(n, 0);  // entry e at D[64 e]
     T x0 = xf(vidx3(n, 0)), x1 = xf(vidx3(n, 1)), x2 = xf(vidx3(n, 2));
     T p0 = 0, p1 = 0, p2 = 0;
     const int s = n >> 6, lane = n & 63;
@@ -282,7 +293,7 @@ __device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *of
         // uniform slices: all K slots' columns, gathers and blocks are
         // independent loads, issued before any of them is consumed
         const size_t q0 = 64 * (size_t)s * K + lane;
-        const T *A = offd + 9 * 64 * (size_t)s * K + lane;
+        const MT *A = offd + 9 * 64 * (size_t)s * K + lane;
         int cn[K];
 #pragma unroll
         for (int b = 0; b < K; b++) cn[b] = sell_col[q0 + 64 * b];
@@ -296,7 +307,7 @@ __device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *of
         }
 #pragma unroll
         for (int b = 0; b < K; b++) {
-            const T *a = A + 576 * b;
+            const MT *a = A + 576 * b;
             p0 += a[0] * g[b][0] + a[64] * g[b][1] + a[128] * g[b][2];
             p1 += a[192] * g[b][0] + a[256] * g[b][1] + a[320] * g[b][2];
             p2 += a[384] * g[b][0] + a[448] * g[b][1] + a[512] * g[b][2];
@@ -305,7 +316,7 @@ __device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *of
         const int k0 = sell_ptr[s], k1 = sell_ptr[s + 1];
         for (int b = k0; b < k1; b++) {
             const size_t q = 64 * (size_t)b + lane;
-            const T *a = offd + 9 * 64 * (size_t)b + lane;
+            const MT *a = offd + 9 * 64 * (size_t)b + lane;
             const int cn = sell_col[q];
             const size_t cc = vidx3(cn, 0);
             T y0 = xf(cc), y1 = xf(cc + 64), y2 = xf(cc + 128);
@@ -322,7 +333,7 @@ __device__ __forceinline__ void amult_prec_row(int n, const T *diag, const T *of
         y[0] = a0; y[1] = a1; y[2] = a2;
         return;
     }
-    const T *M = dinv + didx9(n, 0);
+    const MT *M = dinv + didx9(n, 0);
     y[0] = M[0] * a0 + M[64] * a1 + M[128] * a2;
     y[1] = M[192] * a0 + M[256] * a1 + M[320] * a2;
     y[2] = M[384] * a0 + M[448] * a1 + M[512] * a2;
@@ -340,20 +351,22 @@ struct VecS {  // s = r + (-alpha) Ap, the reference's copy + daxpy
     __device__ T operator()(size_t i) const { return r[i] + nalpha * ap[i]; }
 };
 
-template <typename T>
+template <typename P>
 struct KVecs {
-    const T *diag, *offd, *dinv;
-    T *b, *r, *p, *Ap, *s, *As, *q, *u;
+    const typename P::M *diag, *offd, *dinv;
+    typename P::V *b, *r, *p, *Ap, *s, *As, *q, *u;
 };
 
 // prologue part 1: b' = M^-1 b, u = b', ||b'||inf  (solv_bcgstab.c:117-137)
-template <typename T>
-__global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<T> V, const double *rhs, double *part, unsigned *count,
+template <typename P>
+__global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double *rhs, double *part, unsigned *count,
                                                 KScal *sc) {
+    using T = typename P::V;
+    using MT = typename P::M;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < N) {
-        const T *M = V.dinv + didx9(n, 0);
+        const MT *M = V.dinv + didx9(n, 0);
         T b0 = (T)rhs[vidx3(n, 0)], b1 = (T)rhs[vidx3(n, 1)], b2 = (T)rhs[vidx3(n, 2)];
         T y[3];
         y[0] = M[0] * b0 + M[64] * b1 + M[128] * b2;
@@ -370,14 +383,16 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<T> V, const double 
 }
 
 // prologue part 2: r = b' - M^-1 A u, q = r, p = r, rho = (r,q), ||r||inf  (:138-188)
-template <typename T, int K>
-__global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
+template <typename P, int K>
+__global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
+    using T = typename P::V;
+    using MT = typename P::M;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (n < N) {
         T y[3];
-        amult_prec_row<T, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.u}, y);
+        amult_prec_row<P, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.u}, y);
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             T r = V.b[i] + (T)(-1.0) * y[j];
@@ -392,15 +407,17 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<T> V, const int *se
     block_partials<2>(v, op, part);
 }
 
-template <typename T, int K>
-__global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
+template <typename P, int K>
+__global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
+    using T = typename P::V;
+    using MT = typename P::M;
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < N) {
         T y[3];
-        amult_prec_row<T, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.p}, y);
+        amult_prec_row<P, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.p}, y);
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             V.Ap[i] = y[j];
@@ -411,9 +428,11 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<T> V, const int *se
     block_partials<1>(v, op, part);
 }
 
-template <typename T, int K>
-__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *sell_ptr, const int *sell_col, double *part,
+template <typename P, int K>
+__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
                                                 unsigned *count, KScal *sc) {
+    using T = typename P::V;
+    using MT = typename P::M;
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     const T nalpha = (T)(-sc->alpha);
@@ -421,7 +440,7 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *se
     if (n < N) {
         VecS<T> sf{V.r, V.Ap, nalpha};
         T y[3];
-        amult_prec_row<T, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, sf, y);
+        amult_prec_row<P, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, sf, y);
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             T s = sf(i);
@@ -436,8 +455,10 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<T> V, const int *se
     block_partials<3>(v, op, part);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<T> V, double *part, unsigned *count, KScal *sc) {
+template <typename P>
+__global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part, unsigned *count, KScal *sc) {
+    using T = typename P::V;
+    using MT = typename P::M;
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     const double alpha = sc->alpha, omega = sc->omega;
@@ -473,7 +494,7 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<T> V, double *part,
 
 // y = A x with x = p (S = false) or x = s = r - alpha Ap (S = true, s stored)
 template <int K, bool S>
-__global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<double> V, const int *sell_ptr, const int *sell_col,
+__global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<PrecF64> V, const int *sell_ptr, const int *sell_col,
                                                  const double *x, double *y, const KScal *sc, int check_done) {
     if (check_done && sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -481,10 +502,10 @@ __global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<double> V, const i
     double a[3];
     if constexpr (S) {
         VecS<double> sf{V.r, V.Ap, -sc->alpha};
-        amult_prec_row<double, K, VecS<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr, sf, a);
+        amult_prec_row<PrecF64, K, VecS<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr, sf, a);
         for (int j = 0; j < 3; j++) V.s[vidx3(n, j)] = sf(vidx3(n, j));
     } else {
-        amult_prec_row<double, K, VecX<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr,
+        amult_prec_row<PrecF64, K, VecX<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr,
                                                        VecX<double>{x}, a);
     }
     for (int j = 0; j < 3; j++) y[vidx3(n, j)] = a[j];
@@ -497,7 +518,7 @@ __global__ void k_pl_copy(size_t n, const double *src, double *dst) {
 }
 
 // u = b' (= M^-1 b), ||b'||inf
-__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < N)
@@ -512,7 +533,7 @@ __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<double> V, double *p
 }
 
 // r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
-__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (n < N)
@@ -530,7 +551,7 @@ __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<double> V, double *p
 }
 
 // gamma = (q, Ap), alpha = rho / gamma
-__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
     if (sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -541,7 +562,7 @@ __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<double> V, double
 }
 
 // omega = (As, s) / (As, As), rho' = -omega (As, q)
-__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<double> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
     if (sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -568,50 +589,54 @@ __global__ void k_bcg_epi(int N3, const T *u, const double *scale, double *sol) 
     sol[i] = x;
 }
 
-template <typename T>
-static KVecs<T> vecs(Ctx *c);
+template <typename P>
+static KVecs<P> vecs(Ctx *c);
 template <>
-KVecs<double> vecs<double>(Ctx *c) {
-    return KVecs<double>{c->diag, c->offd, c->dinv, c->kb, c->kr, c->kp, c->kAp, c->ks_, c->kAs, c->kq, c->ku};
+KVecs<PrecF64> vecs<PrecF64>(Ctx *c) {
+    return KVecs<PrecF64>{c->diag, c->offd, c->dinv, c->kb, c->kr, c->kp, c->kAp, c->ks_, c->kAs, c->kq, c->ku};
 }
 template <>
-KVecs<float> vecs<float>(Ctx *c) {
-    return KVecs<float>{c->diag32, c->offd32, c->dinv32, c->fb, c->fr, c->fp, c->fAp, c->fs, c->fAs, c->fq, c->fu};
+KVecs<PrecMix> vecs<PrecMix>(Ctx *c) {
+    return KVecs<PrecMix>{c->diag32, c->offd32, c->dinv32, c->kb, c->kr, c->kp, c->kAp, c->ks_, c->kAs, c->kq, c->ku};
+}
+template <>
+KVecs<PrecF32> vecs<PrecF32>(Ctx *c) {
+    return KVecs<PrecF32>{c->diag32, c->offd32, c->dinv32, c->fb, c->fr, c->fp, c->fAp, c->fs, c->fAs, c->fq, c->fu};
 }
 
-template <typename T, int K>
+template <typename P, int K>
 static void prologue_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
-    KVecs<T> V = vecs<T>(c);
-    hipLaunchKernelGGL(k_bcg_p1<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, c->red.count + 0,
+    KVecs<P> V = vecs<P>(c);
+    hipLaunchKernelGGL(k_bcg_p1<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, c->red.count + 0,
                        c->ksc);
     hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
-    hipLaunchKernelGGL((k_bcg_p2<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+    hipLaunchKernelGGL((k_bcg_p2<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                        c->red.part, c->red.count + 1, c->ksc);
     hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
 }
 
-template <typename T, int K>
+template <typename P, int K>
 static void iters_k(Ctx *c, int k) {
     int nb = nblocks_for(c->N, 256);
-    KVecs<T> V = vecs<T>(c);
+    KVecs<P> V = vecs<P>(c);
     for (int i = 0; i < k; i++) {
-        hipLaunchKernelGGL((k_bcg_k1<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+        hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                            c->red.part, c->red.count + 2, c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
-        hipLaunchKernelGGL((k_bcg_k2<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
+        hipLaunchKernelGGL((k_bcg_k2<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                            c->red.part, c->red.count + 3, c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
-        hipLaunchKernelGGL(k_bcg_k3<T>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 4,
+        hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 4,
                            c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
     }
 }
 
-template <typename T, int K>
+template <typename P, int K>
 static void k1_only_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
-    hipLaunchKernelGGL((k_bcg_k1<T, K>), dim3(nb), dim3(256), 0, c->stream, c->N, vecs<T>(c), c->sell_ptr,
+    hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, vecs<P>(c), c->sell_ptr,
                        c->sell_col, c->red.part, c->red.count + 2, c->ksc);
 }
 
@@ -632,7 +657,7 @@ static void k1_only_k(Ctx *c) {
 template <typename T, int K>
 static void pl_prologue_k(Ctx *c) {
     const int nb = nblocks_for(c->N, 256);
-    KVecs<double> V = vecs<double>(c);
+    KVecs<PrecF64> V = vecs<PrecF64>(c);
     const size_t n3 = 3 * (size_t)c->Np;
     hipLaunchKernelGGL(k_pl_copy, dim3(nblocks_for((int)n3, 256)), dim3(256), 0, c->stream, n3, c->R, V.b);
     launch_band_solve(c, V.b, nullptr);
@@ -648,7 +673,7 @@ static void pl_prologue_k(Ctx *c) {
 template <typename T, int K>
 static void pl_iters_k(Ctx *c, int k) {
     const int nb = nblocks_for(c->N, 256);
-    KVecs<double> V = vecs<double>(c);
+    KVecs<PrecF64> V = vecs<PrecF64>(c);
     for (int i = 0; i < k; i++) {
         hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
                            c->sell_col, V.p, V.Ap, c->ksc, 1);
@@ -662,42 +687,45 @@ static void pl_iters_k(Ctx *c, int k) {
         hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 3,
                            c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
-        hipLaunchKernelGGL(k_bcg_k3<double>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
+        hipLaunchKernelGGL(k_bcg_k3<PrecF64>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
                            c->red.count + 4, c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
     }
 }
 
-template <typename T>
+template <typename P>
 static void prologue(Ctx *c) {
-    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_prologue_k, T, c) }
-    else { SW2_KSWITCH(prologue_k, T, c) }
+    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_prologue_k, P, c) }
+    else { SW2_KSWITCH(prologue_k, P, c) }
 }
 
-template <typename T>
+template <typename P>
 static void iters(Ctx *c, int k) {
-    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_iters_k, T, c, k) }
-    else { SW2_KSWITCH(iters_k, T, c, k) }
+    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_iters_k, P, c, k) }
+    else { SW2_KSWITCH(iters_k, P, c, k) }
 }
 
 void launch_k1_only(Ctx *c) {
-    if (c->opts.precision == 1) { SW2_KSWITCH(k1_only_k, float, c) }
-    else { SW2_KSWITCH(k1_only_k, double, c) }
+    if (c->opts.precision == 2) { SW2_KSWITCH(k1_only_k, PrecF32, c) }
+    else if (c->opts.precision == 1) { SW2_KSWITCH(k1_only_k, PrecMix, c) }
+    else { SW2_KSWITCH(k1_only_k, PrecF64, c) }
 }
 
 void launch_krylov_prologue(Ctx *c) {
-    if (c->opts.precision == 1) prologue<float>(c);
-    else prologue<double>(c);
+    if (c->opts.precision == 2) prologue<PrecF32>(c);
+    else if (c->opts.precision == 1) prologue<PrecMix>(c);
+    else prologue<PrecF64>(c);
 }
 
 void launch_krylov_iters(Ctx *c, int k) {
-    if (c->opts.precision == 1) iters<float>(c, k);
-    else iters<double>(c, k);
+    if (c->opts.precision == 2) iters<PrecF32>(c, k);
+    else if (c->opts.precision == 1) iters<PrecMix>(c, k);
+    else iters<PrecF64>(c, k);
 }
 
 void launch_krylov_epilogue(Ctx *c) {
     int n3 = 3 * c->Np;
-    if (c->opts.precision == 1)
+    if (c->opts.precision == 2)
         hipLaunchKernelGGL(k_bcg_epi<float>, dim3(nblocks_for(n3, 256)), dim3(256), 0, c->stream, n3, c->fu, c->scale,
                            c->sol);
     else

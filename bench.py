@@ -56,8 +56,9 @@ def run_form(p, precision, steps, warmup, rtol, graphs=1):
         log(f"  precision={precision} step {i}: newton {r['iterations']} lin {r['lin_its']} "
             f"final max|R| {r['hist_max'][-1]:.3e}")
     elapsed = time.perf_counter() - t0
+    h, vel, *_ = g.get_state()
     return g, dict(newton_its=newton_its, lin_its=lin_its, seconds=elapsed, r0=r0,
-                   newton_per_s=newton_its / elapsed, phases_ms=phases)
+                   newton_per_s=newton_its / elapsed, phases_ms=phases, head=h, vel=vel)
 
 
 def measured_traffic(precision):
@@ -65,7 +66,7 @@ def measured_traffic(precision):
     FETCH_SIZE + WRITE_SIZE passes, scaled by the copy-kernel calibration)."""
     import re
     path = os.path.join(ROOT, "profiles", "traffic_k_bcg_k1_channel1000x500.md")
-    key = "k_bcg_k1<double>" if precision == 0 else "k_bcg_k1<float>"
+    key = {0: "k_bcg_k1<double>", 1: "k_bcg_k1<mixed>", 2: "k_bcg_k1<float>"}[precision]
     try:
         txt = open(path).read()
     except OSError:
@@ -79,10 +80,11 @@ def spmv_roofline(g, p, precision):
     (K1) over its measured average duration."""
     N = p.nnodes
     nnz = len(g.pattern()[1])
-    vb = 8 if precision == 0 else 4
+    mb = 8 if precision == 0 else 4          # matrix / preconditioner entries
+    vb = 4 if precision == 2 else 8          # vector entries
     # matrix values (diag + off-diagonal blocks), block columns, row_ptr,
     # preconditioner blocks, x (p), q, Ap written
-    algo = 9 * vb * (N + nnz) + 4 * nnz + 4 * (N + 1) + 9 * vb * N + 3 * vb * N * 3
+    algo = 9 * mb * (N + nnz) + 4 * nnz + 4 * (N + 1) + 9 * mb * N + 3 * vb * N * 3
     ms = g.time_kernel(0, reps=50)
     achieved = algo / (ms * 1e-3) / 1e9
     traffic = measured_traffic(precision) if (p.nnodes, nnz) == (501501, 3003000) else None
@@ -126,7 +128,7 @@ def cpu_baseline(seconds_budget=20.0):
 def kernel_mode(p):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts
     out = {}
-    for prec in (1, 0):
+    for prec in (2, 1, 0):
         g = SW2Model(p, default_opts(precision=prec, max_nonlin_it=2))
         g.shift()
         g.timestep()
@@ -165,7 +167,7 @@ def main():
         return
     res = {}
     roof = {}
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         log(f"form precision={prec}")
         g, r = run_form(p, prec, args.steps, args.warmup, 1e-8, args.graphs)
         res[prec] = r
@@ -174,6 +176,12 @@ def main():
         g.close()
     cpu = None if args.no_cpu else cpu_baseline()
     r32, r64 = res[1], res[0]
+    # both forms ran the same steps from the same state: error of the FP32-refined
+    # state against the FP64 run, relative to the FP64 field's largest magnitude
+    def rel_err(r):
+        return dict(head=float(np.abs(r["head"] - r64["head"]).max() / np.abs(r64["head"]).max()),
+                    velocity=float(np.abs(r["vel"] - r64["vel"]).max() / np.abs(r64["vel"]).max()))
+    err, err32 = rel_err(r32), rel_err(res[2])
     out = {
         "metric": "SW2 Newton steps/sec @1M tris, FP64 vs FP32-refined, 1e-8 rel residual",
         "value": round(r32["newton_per_s"], 3),
@@ -192,7 +200,13 @@ def main():
                    "nodes": p.nnodes, "triangles": p.nelems2d},
         "fp64": {"value": round(r64["newton_per_s"], 3), "newton_its": r64["newton_its"],
                  "lin_its": r64["lin_its"], "phases_ms": r64["phases_ms"], "roofline": roof[0]},
-        "fp32_refined": {"newton_its": r32["newton_its"], "lin_its": r32["lin_its"], "phases_ms": r32["phases_ms"]},
+        "fp32_refined": {"newton_its": r32["newton_its"], "lin_its": r32["lin_its"], "phases_ms": r32["phases_ms"],
+                         "max_rel_error_vs_fp64": err,
+                         "form": "FP32 Jacobian storage, preconditioner and SpMV; FP64 Krylov vectors, residual, update"},
+        "fp32_all": {"value": round(res[2]["newton_per_s"], 3), "newton_its": res[2]["newton_its"],
+                     "lin_its": res[2]["lin_its"], "phases_ms": res[2]["phases_ms"],
+                     "max_rel_error_vs_fp64": err32, "roofline": roof[2],
+                     "form": "FP32 matrix and Krylov vectors; FP64 residual, update"},
         "roofline": roof[1],
         "cpu_baseline": cpu,
     }

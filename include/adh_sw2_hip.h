@@ -83,7 +83,8 @@ typedef struct {
     int max_cuts;        /* 5 line-search cuts */
     double lin_rtol;     /* BiCGSTAB stop: ||r||inf <= lin_atol + lin_rtol*||M^-1 b||inf */
     double lin_atol;
-    int precision;       /* 0: FP64 Krylov; 1: FP32 matrix + Krylov, FP64 residual and update */
+    int precision;       /* 0: FP64; 1: FP32 matrix + preconditioner, FP64 Krylov vectors; 2: FP32 matrix and
+                            Krylov; 1 and 2 keep the residual, update and Newton tests in FP64 */
     int preconditioner;  /* 2: 3x3 block Jacobi; 1: profile LU of node blocks (OP PRE 1 with OP BLK
                             nblock, src/solver/solv_blk_*.c), FP64 only */
     int check_every;     /* Krylov iterations between host convergence checks */

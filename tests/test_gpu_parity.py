@@ -122,7 +122,7 @@ def test_dry_node_velocity_rule():
 
 
 @pytest.mark.parametrize("name", ["channel-40x20", "bumpy-60x30"])
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_linear_solve(name, precision, stepped):
     """BiCGSTAB with 3x3 block-Jacobi on the same scaled system as the oracle.
     FP64: iteration count within 1 and solution within 1e-9 relative (dot
@@ -171,7 +171,7 @@ def test_linear_solve_profile_lu(nblock, stepped):
     assert rel < 1e-9
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_krylov_graph_replays_match_eager(precision, stepped):
     """The Krylov iterations replay captured HIP graphs: several solves in a
     row (each replaying graphs of different chunk lengths, with host
