@@ -135,6 +135,32 @@ static void edge_geometry(const adh_sw2_mesh *m, int ie, double *djac, double *n
     }
 }
 
+// Element geometry as given by the caller or computed from the coordinates.
+static void mesh_geometry(const adh_sw2_mesh *m, std::vector<double> &djac, std::vector<double> &grad,
+                          std::vector<double> &djac1d, std::vector<double> &nrml1d) {
+    const int E = m->nelems2d, B = m->nelems1d;
+    djac.assign(E, 0.0); grad.assign(6 * (size_t)E, 0.0); djac1d.assign(B, 0.0); nrml1d.assign(2 * (size_t)B, 0.0);
+    if (m->e2d_djac && m->e2d_grad) {
+        memcpy(djac.data(), m->e2d_djac, sizeof(double) * E);
+        memcpy(grad.data(), m->e2d_grad, sizeof(double) * 6 * E);
+    } else {
+        for (int i = 0; i < E; i++) {
+            const int *t = m->e2d + 3 * i;
+            tri_geometry(m->x[t[0]], m->x[t[1]], m->x[t[2]], m->y[t[0]], m->y[t[1]], m->y[t[2]], &djac[i],
+                         &grad[6 * (size_t)i]);
+        }
+    }
+    for (int i = 0; i < B; i++) {
+        if (m->e1d_djac && m->e1d_nrml) {
+            djac1d[i] = m->e1d_djac[i];
+            nrml1d[2 * i] = m->e1d_nrml[2 * i];
+            nrml1d[2 * i + 1] = m->e1d_nrml[2 * i + 1];
+        } else {
+            edge_geometry(m, i, &djac1d[i], &nrml1d[2 * (size_t)i]);
+        }
+    }
+}
+
 static int validate(const adh_sw2_mesh *m) {
     if (!m || m->nnodes <= 0 || m->nelems2d <= 0 || m->nmat <= 0 || m->nelems1d < 0)
         return fail(ADH_SW2_EINVAL, "mesh sizes must be positive");
@@ -197,7 +223,9 @@ static int alloc_solver(adh_sw2_hip *h) {
     return 0;
 }
 
-extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_hip **out) {
+// ninst > 0: the mesh is the union of an ensemble of ninst models (see
+// adh_sw2_hip_batch_create); the per-model scalars are sized for them.
+static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_hip **out, int ninst) {
     if (!out) return fail(ADH_SW2_EINVAL, "out is NULL");
     *out = nullptr;
     int rc = validate(m);
@@ -220,26 +248,8 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     const int N = c.N, E = c.E, B = c.B;
 
     // ---- geometry (host)
-    std::vector<double> djac(E), grad(6 * (size_t)E), djac1d(B), nrml1d(2 * (size_t)B);
-    if (m->e2d_djac && m->e2d_grad) {
-        memcpy(djac.data(), m->e2d_djac, sizeof(double) * E);
-        memcpy(grad.data(), m->e2d_grad, sizeof(double) * 6 * E);
-    } else {
-        for (int i = 0; i < E; i++) {
-            const int *t = m->e2d + 3 * i;
-            tri_geometry(m->x[t[0]], m->x[t[1]], m->x[t[2]], m->y[t[0]], m->y[t[1]], m->y[t[2]], &djac[i],
-                         &grad[6 * (size_t)i]);
-        }
-    }
-    for (int i = 0; i < B; i++) {
-        if (m->e1d_djac && m->e1d_nrml) {
-            djac1d[i] = m->e1d_djac[i];
-            nrml1d[2 * i] = m->e1d_nrml[2 * i];
-            nrml1d[2 * i + 1] = m->e1d_nrml[2 * i + 1];
-        } else {
-            edge_geometry(m, i, &djac1d[i], &nrml1d[2 * (size_t)i]);
-        }
-    }
+    std::vector<double> djac, grad, djac1d, nrml1d;
+    mesh_geometry(m, djac, grad, djac1d, nrml1d);
 
     // ---- adjacency (ascending element order)
     std::vector<int> n2e_ptr(N + 1, 0), n2b_ptr(N + 1, 0);
@@ -424,15 +434,16 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     UP(offd, (const double *)nullptr, 9 * c.nsell);
     UP(scale, (const double *)nullptr, 3 * (size_t)c.Np);
     UP(dinv, (const double *)nullptr, 9 * (size_t)c.Np);
-    UP(ksc, (const KScal *)nullptr, 1);
-    UP(norms, (const Norms *)nullptr, 1);
+    const size_t nmod = ninst > 0 ? (size_t)ninst : 1;
+    UP(ksc, (const KScal *)nullptr, nmod);
+    UP(norms, (const Norms *)nullptr, nmod);
     c.red.maxblocks = nblocks_for(N, 256) + 8;
     UP(red.part, (const double *)nullptr, 4 * (size_t)c.red.maxblocks);
     UP(red.count, (const unsigned *)nullptr, 8);
 #undef UP
     if ((rc = alloc_solver(h))) { adh_sw2_hip_destroy(h); return rc; }
-    if (hipHostMalloc((void **)&c.h_ksc, sizeof(KScal)) != hipSuccess ||
-        hipHostMalloc((void **)&c.h_norms, sizeof(Norms)) != hipSuccess ||
+    if (hipHostMalloc((void **)&c.h_ksc, sizeof(KScal) * nmod) != hipSuccess ||
+        hipHostMalloc((void **)&c.h_norms, sizeof(Norms) * nmod) != hipSuccess ||
         hipHostMalloc((void **)&c.h_flag, 64) != hipSuccess) {
         adh_sw2_hip_destroy(h);
         return fail(ADH_SW2_ENOMEM, "hipHostMalloc failed");
@@ -460,6 +471,10 @@ extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opt
     return ADH_SW2_OK;
 }
 
+extern "C" int adh_sw2_hip_create(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_hip **out) {
+    return create_impl(m, opts, out, 0);
+}
+
 extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     if (!h) return;
     Ctx &c = h->c;
@@ -485,6 +500,7 @@ extern "C" int adh_sw2_hip_set_opts(adh_sw2_hip *h, const adh_sw2_opts *o) {
 }
 
 extern "C" int adh_sw2_hip_set_dt(adh_sw2_hip *h, double dt) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h || !(dt > 0)) return fail(ADH_SW2_EINVAL, "dt must be positive");
     h->c.dt = dt;
     h->c.ed.dt = dt;
@@ -492,6 +508,7 @@ extern "C" int adh_sw2_hip_set_dt(adh_sw2_hip *h, double dt) {
 }
 
 extern "C" int adh_sw2_hip_geometry(adh_sw2_hip *h, double *djac, double *grad, double *djac1d, double *nrml1d) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     if (djac) HIPCHK(hipMemcpy(djac, c.djac, sizeof(double) * c.E, hipMemcpyDeviceToHost));
@@ -504,6 +521,7 @@ extern "C" int adh_sw2_hip_geometry(adh_sw2_hip *h, double *djac, double *grad, 
 extern "C" int adh_sw2_hip_pattern_nnz(adh_sw2_hip *h) { return h ? h->c.nnz : ADH_SW2_EINVAL; }
 
 extern "C" int adh_sw2_hip_pattern(adh_sw2_hip *h, int *row_ptr, int *col) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     if (row_ptr) memcpy(row_ptr, h->c.h_row_ptr.data(), sizeof(int) * (h->c.N + 1));
     if (col && h->c.nnz) memcpy(col, h->c.h_col.data(), sizeof(int) * h->c.nnz);
@@ -532,6 +550,7 @@ static int get_pair(Ctx &c, const double *dh, const double *du, const double *dv
 
 extern "C" int adh_sw2_hip_set_state(adh_sw2_hip *h, const double *head, const double *vel, const double *old_head,
                                      const double *old_vel, const double *older_head, const double *older_vel) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h || !head || !vel || !old_head || !old_vel || !older_head || !older_vel)
         return fail(ADH_SW2_EINVAL, "null argument");
     Ctx &c = h->c;
@@ -547,6 +566,7 @@ extern "C" int adh_sw2_hip_set_state(adh_sw2_hip *h, const double *head, const d
 
 extern "C" int adh_sw2_hip_get_state(adh_sw2_hip *h, double *head, double *vel, double *old_head, double *old_vel,
                                      double *older_head, double *older_vel) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -564,12 +584,14 @@ static int sync_check(Ctx &c) {
 }
 
 extern "C" int adh_sw2_hip_shift(adh_sw2_hip *h) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     launch_shift(&h->c);
     return sync_check(h->c);
 }
 
 extern "C" int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     int *dmask = nullptr;
@@ -601,6 +623,7 @@ static int nodes_from_host(Ctx &c, double *dev, const double *in, int per) {
 }
 
 extern "C" int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     launch_resid(&c);
@@ -628,6 +651,7 @@ static int offd_from_host(Ctx &c, const double *in) {
 }
 
 extern "C" int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdiag, double *residual) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     launch_jacobian(&c);
@@ -641,6 +665,7 @@ extern "C" int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdia
 }
 
 extern "C" int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *em1d) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -657,6 +682,7 @@ extern "C" int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *
 }
 
 extern "C" int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     int rc;
@@ -675,6 +701,7 @@ static int ensure_band(Ctx &c) {
 }
 
 extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     int rc0 = ensure_band(c);
@@ -762,6 +789,7 @@ static int krylov(Ctx &c, int *lin_it, int *verdict) {
 }
 
 extern "C" int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *lin_it, int *verdict) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     int vd = ADH_SW2_NO;
@@ -774,6 +802,7 @@ extern "C" int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *li
 }
 
 extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, const double *offdiag, const double *rhs) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h || !diagonal || !rhs || (!offdiag && h->c.nnz)) return fail(ADH_SW2_EINVAL, "null argument");
     Ctx &c = h->c;
     int rc;
@@ -785,6 +814,7 @@ extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, co
 }
 
 extern "C" int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *offdiag, double *rhs) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -826,6 +856,7 @@ extern "C" int adh_sw2_hip_probe_math(const double *x, int n, double *log_out, d
 }
 
 extern "C" int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h || !avg_ms || reps < 1 || which < 0 || which > 4) return fail(ADH_SW2_EINVAL, "bad arguments");
     Ctx &c = h->c;
     double *copy_dst = nullptr;
@@ -966,6 +997,7 @@ static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
 }
 
 extern "C" int adh_sw2_hip_newton(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h || !verdict) return fail(ADH_SW2_EINVAL, "null argument");
     if (rep) {
         rep->iterations = rep->lin_iterations = rep->nhist = 0;
@@ -975,6 +1007,7 @@ extern "C" int adh_sw2_hip_newton(adh_sw2_hip *h, adh_sw2_report *rep, int *verd
 }
 
 extern "C" int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int max_retries) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h || !verdict) return fail(ADH_SW2_EINVAL, "null argument");
     if (rep) {
         rep->iterations = rep->lin_iterations = rep->nhist = 0;
@@ -992,11 +1025,382 @@ extern "C" int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *ve
     return 0;
 }
 
-extern "C" int adh_sw2_hip_timestep_batch(adh_sw2_hip **hs, int n, adh_sw2_report *reps, int *verdicts) {
-    if (!hs || n < 0 || !verdicts) return fail(ADH_SW2_EINVAL, "null argument");
+// ===========================================================================
+// Ensembles: many independent SW2 models advanced together.  The models are
+// laid side by side in one mesh (each model's nodes start on a multiple of
+// 256; the gap nodes are isolated and stay zero), so every phase -- element
+// residuals and Jacobians, assembly, scaling, each BiCGSTAB kernel -- is one
+// launch over all models; reductions and Krylov scalars are per model
+// (256-node blocks never straddle two models), and the Newton/time-step
+// decisions are taken per model as the single-model path takes them.  A
+// model's arithmetic is the single-model arithmetic: same block partitions,
+// same reduction order, same element code.
+
+static int set_mask(Ctx &c, const std::vector<int> &m) {
+    HIPCHK(hipMemcpyAsync(c.inst_mask, m.data(), sizeof(int) * c.ninst, hipMemcpyHostToDevice, c.stream));
+    // the host vector may change before the copy runs
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+static int read_norms_all(Ctx &c, std::vector<Norms> &out) {
+    HIPCHK(hipMemcpyAsync(c.h_norms, c.norms, sizeof(Norms) * c.ninst, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    out.assign(c.h_norms, c.h_norms + c.ninst);
+    return 0;
+}
+
+// BiCGSTAB for the active models (inactive ones start converged)
+static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int> &lin_it, std::vector<int> &vd) {
+    const int n = c.ninst, cap = lin_it_cap(c);
+    std::vector<KScal> init(n);
     for (int i = 0; i < n; i++) {
-        int rc = adh_sw2_hip_timestep(hs[i], reps ? reps + i : nullptr, verdicts + i, 0);
+        init[i] = KScal{};
+        init[i].atol = c.opts.lin_atol;
+        init[i].rtol = c.opts.lin_rtol;
+        init[i].max_it = active[i] ? cap : 0;
+    }
+    HIPCHK(hipMemcpyAsync(c.ksc, init.data(), sizeof(KScal) * n, hipMemcpyHostToDevice, c.stream));
+    launch_krylov_prologue(&c);
+    int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
+    int maxit = 0;
+    for (int chunk = 0;; chunk++) {
+        HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal) * n, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        int left = 0;
+        bool all_done = true;
+        for (int i = 0; i < n; i++)
+            if (!c.h_ksc[i].done) {
+                all_done = false;
+                left = std::max(left, cap - c.h_ksc[i].it);
+            }
+        if (all_done) break;
+        int want = chunk == 0 ? first : 4;
+        int rc = run_iters(c, std::max(1, std::min(want, left)));
         if (rc) return rc;
+    }
+    launch_krylov_epilogue(&c);
+    HIPCHK(hipGetLastError());
+    lin_it.assign(n, 0);
+    vd.assign(n, ADH_SW2_YES);
+    for (int i = 0; i < n; i++) {
+        if (!active[i]) continue;
+        lin_it[i] = c.h_ksc[i].it;
+        vd[i] = c.h_ksc[i].it == cap ? ADH_SW2_NO : ADH_SW2_YES;
+        maxit = std::max(maxit, lin_it[i]);
+    }
+    c.last_lin_it = maxit;
+    return 0;
+}
+
+// fe_newton for every model with run[i] set (src/fe/fe_newton.c:214-912)
+static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *reps, std::vector<int> &verdict) {
+    const adh_sw2_opts &o = c.opts;
+    const int n = c.ninst;
+    int rc;
+    std::vector<Norms> nm;
+    if ((rc = set_mask(c, run))) return rc;
+    launch_init(&c, nullptr);   // masked: only the models of this attempt
+    launch_resid(&c);
+    launch_norms(&c);
+    if ((rc = read_norms_all(c, nm))) return rc;
+    std::vector<int> active(run), it(n, 0), solv(n, ADH_SW2_YES), lin_it, vd;
+    for (int i = 0; i < n; i++)
+        if (run[i] && reps) hist(reps + i, nm[i], 0);
+    auto any = [](const std::vector<int> &v) { for (int x : v) if (x) return true; return false; };
+    while (any(active)) {
+        launch_jacobian(&c);
+        launch_assemble(&c);
+        launch_zero(&c, c.sol, 3 * (size_t)c.Np);
+        launch_setup(&c);
+        if ((rc = krylov_batch(c, active, lin_it, vd))) return rc;
+        for (int i = 0; i < n; i++)
+            if (active[i]) solv[i] = (o.precision >= 1) ? ADH_SW2_YES : vd[i];
+        if ((rc = set_mask(c, active))) return rc;
+        launch_inc(&c, 1.0);
+        launch_scale_sol(&c, -1.0);
+        std::vector<double> old_l2(n);
+        for (int i = 0; i < n; i++) old_l2[i] = nm[i].l2;
+        launch_resid(&c);
+        launch_norms(&c);
+        {
+            // finished models keep their last norms (their correction
+            // vector no longer belongs to them)
+            std::vector<Norms> nm2;
+            if ((rc = read_norms_all(c, nm2))) return rc;
+            for (int i = 0; i < n; i++)
+                if (active[i]) nm[i] = nm2[i];
+        }
+        std::vector<int> cuts(n, 0);
+        for (int i = 0; i < n; i++)
+            if (active[i] && (isnan(nm[i].l2) || isinf(nm[i].l2))) solv[i] = ADH_SW2_NO;
+        for (;;) {   // line search, for the models that need it
+            std::vector<int> cut(n, 0);
+            for (int i = 0; i < n; i++)
+                cut[i] = active[i] && nm[i].maxn > o.tol_nonlin && nm[i].l2 > old_l2[i] && cuts[i] < o.max_cuts;
+            if (!any(cut)) break;
+            if ((rc = set_mask(c, cut))) return rc;
+            launch_scale_sol(&c, 0.5);
+            launch_inc(&c, 1.0);
+            launch_resid(&c);
+            launch_norms(&c);
+            std::vector<Norms> nm2;
+            if ((rc = read_norms_all(c, nm2))) return rc;
+            for (int i = 0; i < n; i++)
+                if (cut[i]) { nm[i] = nm2[i]; cuts[i]++; }
+        }
+        for (int i = 0; i < n; i++) {
+            if (!active[i]) continue;
+            it[i]++;
+            if (reps) {
+                hist(reps + i, nm[i], lin_it[i]);
+                reps[i].lin_iterations += lin_it[i];
+            }
+            const bool keep = (solv[i] == ADH_SW2_YES) && (it[i] < o.max_nonlin_it) && (nm[i].maxn < 1.0E+20) &&
+                              (nm[i].maxn > o.tol_nonlin && nm[i].incn > o.inc_nonlin);
+            if (!keep) active[i] = 0;
+        }
+    }
+    std::vector<int> failed(n, 0);
+    verdict.assign(n, ADH_SW2_YES);
+    for (int i = 0; i < n; i++) {
+        if (!run[i]) continue;
+        const bool f = (solv[i] != ADH_SW2_YES) || ((nm[i].maxn > o.tol_nonlin) && (nm[i].incn > o.inc_nonlin)) ||
+                       (nm[i].maxn >= 1.0E+20);
+        failed[i] = f;
+        verdict[i] = f ? ADH_SW2_NO : ADH_SW2_YES;
+        if (reps) {
+            reps[i].iterations += it[i];
+            reps[i].converged = !f;
+            reps[i].solv_fail = solv[i] != ADH_SW2_YES;
+        }
+    }
+    if (any(failed)) {
+        if ((rc = set_mask(c, failed))) return rc;
+        launch_init(&c, nullptr);  // fe_newton.c:883, for the failed models only
+    }
+    return sync_check(c);
+}
+
+extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const adh_sw2_opts *opts,
+                                        adh_sw2_hip **out) {
+    if (!meshes || n < 1 || !out) return fail(ADH_SW2_EINVAL, "null argument or empty ensemble");
+    if (opts && opts->preconditioner != 2)
+        return fail(ADH_SW2_EUNSUPPORTED, "ensembles use the 3x3 block-Jacobi preconditioner");
+    int rc;
+    for (int i = 0; i < n; i++)
+        if ((rc = validate(meshes + i))) return rc;
+    const adh_sw2_mesh &m0 = meshes[0];
+    for (int i = 1; i < n; i++) {
+        const adh_sw2_mesh &m = meshes[i];
+        if (m.gravity != m0.gravity || m.tau_temporal != m0.tau_temporal || m.tau_pg != m0.tau_pg ||
+            m.drying_lower_limit != m0.drying_lower_limit || m.drying_upper_limit != m0.drying_upper_limit ||
+            m.viscosity != m0.viscosity)
+            return fail(ADH_SW2_EINVAL, "ensemble models must share gravity, tau, drying limits and viscosity");
+    }
+    // union layout
+    std::vector<int> node0(n), E0(n), B0(n), M0(n);
+    long long nn = 0, ne = 0, nb = 0, nm = 0;
+    for (int i = 0; i < n; i++) {
+        node0[i] = (int)nn; E0[i] = (int)ne; B0[i] = (int)nb; M0[i] = (int)nm;
+        nn += meshes[i].nnodes;
+        if (i + 1 < n) nn = (nn + 255) / 256 * 256;
+        ne += meshes[i].nelems2d; nb += meshes[i].nelems1d; nm += meshes[i].nmat;
+    }
+    if (nn > 2000000000LL || ne > 200000000LL) return fail(ADH_SW2_EINVAL, "ensemble too large");
+    const int N = (int)nn, E = (int)ne, B = (int)nb, M = (int)nm;
+    std::vector<double> x(N, 0.0), y(N, 0.0), z(N, 0.0), bcval(3 * (size_t)N, 0.0);
+    std::vector<int> node_bc(N, 0), e2d(3 * (size_t)E), emat(E), esf(E), e1d(2 * (size_t)B), eadj(B), ebc(B);
+    std::vector<double> eks(E), esrc(E), eval(B);
+    std::vector<int> mturb(M), mmode(M);
+    std::vector<double> mcoef(M), mev(3 * (size_t)M), mcor(M);
+    std::vector<double> gdj(E), ggr(6 * (size_t)E), gdj1(B), gnr1(2 * (size_t)B);
+    for (int i = 0; i < n; i++) {
+        const adh_sw2_mesh &m = meshes[i];
+        std::vector<double> dj, gr, dj1, nr1;
+        mesh_geometry(&m, dj, gr, dj1, nr1);
+        std::copy(dj.begin(), dj.end(), gdj.begin() + E0[i]);
+        std::copy(gr.begin(), gr.end(), ggr.begin() + 6 * (size_t)E0[i]);
+        std::copy(dj1.begin(), dj1.end(), gdj1.begin() + B0[i]);
+        std::copy(nr1.begin(), nr1.end(), gnr1.begin() + 2 * (size_t)B0[i]);
+        for (int k = 0; k < m.nnodes; k++) {
+            x[node0[i] + k] = m.x[k]; y[node0[i] + k] = m.y[k]; z[node0[i] + k] = m.z[k];
+            if (m.node_bc) node_bc[node0[i] + k] = m.node_bc[k];
+            if (m.node_bc_val)
+                for (int j = 0; j < 3; j++) bcval[3 * (size_t)(node0[i] + k) + j] = m.node_bc_val[3 * (size_t)k + j];
+        }
+        for (int e = 0; e < m.nelems2d; e++) {
+            for (int j = 0; j < 3; j++) e2d[3 * (size_t)(E0[i] + e) + j] = m.e2d[3 * (size_t)e + j] + node0[i];
+            emat[E0[i] + e] = m.e2d_mat[e] + M0[i];
+            eks[E0[i] + e] = m.e2d_ks[e];
+            esf[E0[i] + e] = m.e2d_src_flag ? m.e2d_src_flag[e] : 0;
+            esrc[E0[i] + e] = m.e2d_src ? m.e2d_src[e] : 0.0;
+        }
+        for (int e = 0; e < m.nelems1d; e++) {
+            for (int j = 0; j < 2; j++) e1d[2 * (size_t)(B0[i] + e) + j] = m.e1d[2 * (size_t)e + j] + node0[i];
+            eadj[B0[i] + e] = m.e1d_adj ? m.e1d_adj[e] + E0[i] : E0[i];
+            ebc[B0[i] + e] = m.e1d_bc[e];
+            eval[B0[i] + e] = m.e1d_val[e];
+        }
+        for (int k = 0; k < m.nmat; k++) {
+            mturb[M0[i] + k] = m.mat_turb[k];
+            mmode[M0[i] + k] = m.mat_eev_mode ? m.mat_eev_mode[k] : 0;
+            mcoef[M0[i] + k] = m.mat_eev_coef ? m.mat_eev_coef[k] : 0.0;
+            for (int j = 0; j < 3; j++) mev[3 * (size_t)(M0[i] + k) + j] = m.mat_ev[3 * (size_t)k + j];
+            mcor[M0[i] + k] = m.mat_cor_speed[k];
+        }
+    }
+    adh_sw2_mesh u = {};
+    u.nnodes = N; u.nelems2d = E; u.nelems1d = B; u.nmat = M;
+    u.x = x.data(); u.y = y.data(); u.z = z.data(); u.node_bc = node_bc.data(); u.node_bc_val = bcval.data();
+    u.e2d = e2d.data(); u.e2d_mat = emat.data(); u.e2d_ks = eks.data(); u.e2d_src_flag = esf.data();
+    u.e2d_src = esrc.data();
+    u.e1d = B ? e1d.data() : nullptr; u.e1d_adj = B ? eadj.data() : nullptr; u.e1d_bc = B ? ebc.data() : nullptr;
+    u.e1d_val = B ? eval.data() : nullptr;
+    u.mat_turb = mturb.data(); u.mat_eev_mode = mmode.data(); u.mat_eev_coef = mcoef.data(); u.mat_ev = mev.data();
+    u.mat_cor_speed = mcor.data();
+    u.e2d_djac = gdj.data(); u.e2d_grad = ggr.data();
+    u.e1d_djac = B ? gdj1.data() : nullptr; u.e1d_nrml = B ? gnr1.data() : nullptr;
+    u.gravity = m0.gravity; u.dt = m0.dt; u.tau_temporal = m0.tau_temporal; u.tau_pg = m0.tau_pg;
+    u.drying_lower_limit = m0.drying_lower_limit; u.drying_upper_limit = m0.drying_upper_limit;
+    u.viscosity = m0.viscosity;
+    adh_sw2_hip *h = nullptr;
+    if ((rc = create_impl(&u, opts, &h, n))) return rc;
+    Ctx &c = h->c;
+    // model tables
+    // per 256-node block: its model and that model's end (gap nodes are never
+    // computed); per node: its model, -1 for gap and padding nodes
+    const int nblk = c.Np / 256 + 1;
+    std::vector<int> blk_inst(2 * (size_t)nblk, 0), inst_blk0(n + 1), inst_nodes(n), node_inst(c.Np, -1), elem_inst(E),
+        edge_inst(B > 0 ? B : 1, 0), loc_bc(3 * (size_t)n, 0), mask(n, 1);
+    std::vector<double> dts(n);
+    for (int i = 0; i < n; i++) {
+        const int a = node0[i], b = (i + 1 < n) ? node0[i + 1] : c.Np;
+        inst_blk0[i] = a / 256;
+        inst_nodes[i] = meshes[i].nnodes;
+        for (int k = a; k < a + meshes[i].nnodes; k++) node_inst[k] = i;
+        for (int k = a / 256; k < (b + 255) / 256 && k < nblk; k++) {
+            blk_inst[2 * k] = i;
+            blk_inst[2 * k + 1] = a + meshes[i].nnodes;
+        }
+        for (int e = 0; e < meshes[i].nelems2d; e++) elem_inst[E0[i] + e] = i;
+        for (int e = 0; e < meshes[i].nelems1d; e++) edge_inst[B0[i] + e] = i;
+        for (int k = 0; k < 3; k++)
+            loc_bc[3 * i + k] = (meshes[i].node_bc && k < meshes[i].nnodes) ? meshes[i].node_bc[k] : 0;
+        dts[i] = meshes[i].dt;
+    }
+    inst_blk0[n] = (c.N + 255) / 256;
+    auto up = [&](auto **dst, const auto *src, size_t cnt) -> int { return dupload(h, dst, src, cnt); };
+    if ((rc = up(&c.blk_inst, blk_inst.data(), blk_inst.size())) || (rc = up(&c.inst_blk0, inst_blk0.data(), n + 1)) ||
+        (rc = up(&c.inst_nodes, inst_nodes.data(), n)) || (rc = up(&c.node_inst, node_inst.data(), node_inst.size())) ||
+        (rc = up(&c.inst_mask, mask.data(), n)) || (rc = up(&c.elem_inst, elem_inst.data(), elem_inst.size())) ||
+        (rc = up(&c.edge_inst, edge_inst.data(), edge_inst.size())) || (rc = up(&c.dt_inst, dts.data(), n)) ||
+        (rc = up(&c.loc_bc_inst, loc_bc.data(), loc_bc.size()))) {
+        adh_sw2_hip_destroy(h);
+        return rc;
+    }
+    c.h_inst_node0 = node0;
+    c.h_inst_N = inst_nodes;
+    c.h_inst_dt = dts;
+    c.ninst = n;
+    c.ed.dt_inst = c.dt_inst;
+    c.ed.elem_inst = c.elem_inst;
+    c.ed.edge_inst = c.edge_inst;
+    c.ed.loc_bc_inst = c.loc_bc_inst;
+    HIPCHK(hipDeviceSynchronize());
+    *out = h;
+    return ADH_SW2_OK;
+}
+
+extern "C" int adh_sw2_hip_batch_size(adh_sw2_hip *h) { return h ? h->c.ninst : ADH_SW2_EINVAL; }
+
+static int model_range(adh_sw2_hip *h, int i, int *a, int *nn) {
+    if (!h || h->c.ninst < 1) return fail(ADH_SW2_EINVAL, "not an ensemble handle");
+    if (i < 0 || i >= h->c.ninst) return fail(ADH_SW2_EINVAL, "model index out of range");
+    *a = h->c.h_inst_node0[i];
+    *nn = h->c.h_inst_N[i];
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_batch_set_state(adh_sw2_hip *h, int i, const double *head, const double *vel,
+                                           const double *old_head, const double *old_vel, const double *older_head,
+                                           const double *older_vel) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    if (!head || !vel || !old_head || !old_vel || !older_head || !older_vel) return fail(ADH_SW2_EINVAL, "null argument");
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const double *src[6] = {head, vel, old_head, old_vel, older_head, older_vel};
+    double *hd[3] = {c.head, c.old_head, c.older_head}, *ud[3] = {c.u, c.old_u, c.older_u},
+           *vd[3] = {c.v, c.old_v, c.older_v};
+    std::vector<double> u(nn), v(nn);
+    for (int k = 0; k < 3; k++) {
+        HIPCHK(hipMemcpy(hd[k] + a, src[2 * k], sizeof(double) * nn, hipMemcpyHostToDevice));
+        for (int j = 0; j < nn; j++) { u[j] = src[2 * k + 1][2 * j]; v[j] = src[2 * k + 1][2 * j + 1]; }
+        HIPCHK(hipMemcpy(ud[k] + a, u.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(vd[k] + a, v.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+    }
+    c.pre_valid = 0;
+    c.split_valid = 0;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_batch_get_state(adh_sw2_hip *h, int i, double *head, double *vel) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (head) HIPCHK(hipMemcpy(head, c.head + a, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (vel) {
+        std::vector<double> u(nn), v(nn);
+        HIPCHK(hipMemcpy(u.data(), c.u + a, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v.data(), c.v + a, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        for (int j = 0; j < nn; j++) { vel[2 * j] = u[j]; vel[2 * j + 1] = v[j]; }
+    }
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_batch_set_dt(adh_sw2_hip *h, int i, double dt) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    if (!(dt > 0)) return fail(ADH_SW2_EINVAL, "dt must be positive");
+    Ctx &c = h->c;
+    c.h_inst_dt[i] = dt;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipMemcpy(c.dt_inst + i, &dt, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// One time step of every model (fe_main's attempts per model: shift, Newton,
+// dt * 0.25 after a failure, src/main/adh_func.c:535-549).  reps and
+// verdicts hold one entry per model.
+extern "C" int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, adh_sw2_report *reps, int *verdicts, int max_retries) {
+    if (!h || h->c.ninst < 1 || !verdicts) return fail(ADH_SW2_EINVAL, "null argument or not an ensemble");
+    Ctx &c = h->c;
+    const int n = c.ninst;
+    if (reps)
+        for (int i = 0; i < n; i++) {
+            reps[i].iterations = reps[i].lin_iterations = reps[i].nhist = 0;
+            reps[i].ms_load = reps[i].ms_setup = reps[i].ms_solve = reps[i].ms_resid = reps[i].ms_total = 0;
+        }
+    std::vector<int> run(n, 1), vd;
+    for (int i = 0; i < n; i++) verdicts[i] = ADH_SW2_NO;
+    for (int attempt = 0; attempt <= max_retries; attempt++) {
+        int rc;
+        if ((rc = set_mask(c, run))) return rc;
+        launch_shift(&c);   // masked: the models of this attempt
+        if ((rc = newton_batch(c, run, reps, vd))) return rc;
+        bool again = false;
+        for (int i = 0; i < n; i++) {
+            if (!run[i]) continue;
+            verdicts[i] = vd[i];
+            if (vd[i] == ADH_SW2_YES) { run[i] = 0; continue; }
+            double dt = c.h_inst_dt[i] * 0.25;   // tc_scale, DT_REDUCE_FACTOR (define.h:123)
+            if (dt <= SMALL) { run[i] = 0; continue; }
+            if ((rc = adh_sw2_hip_batch_set_dt(h, i, dt))) return rc;
+            again = true;
+        }
+        if (!again) break;
     }
     return 0;
 }

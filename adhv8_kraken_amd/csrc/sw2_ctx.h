@@ -105,12 +105,28 @@ struct Ctx {
     // Krylov vectors (FP64) and FP32 twins
     double *kb, *kr, *kp, *kAp, *ks_, *kAs, *kq, *ku;
     float *fb, *fr, *fp, *fAp, *fs, *fAs, *fq, *fu;
-    KScal *ksc;
-    KScal *h_ksc;          // pinned
+    KScal *ksc;            // [max(1, ninst)]
+    KScal *h_ksc;          // pinned, [max(1, ninst)]
     Red red;
-    Norms *norms, *h_norms;
+    Norms *norms, *h_norms;  // [max(1, ninst)]
     int *h_flag;           // pinned scratch
     ElemData ed;
+    // Ensemble of independent models sharing one mesh ("batch"): model i owns
+    // the node range [inst_node0[i], inst_node0[i] + inst_N[i]) starting on a
+    // multiple of 256, so every 256-node block belongs to one model; the gap
+    // nodes up to the next model are isolated (no elements, zero right-hand
+    // side) and stay zero.  ninst = 0 for a single model.
+    int ninst;
+    int *blk_inst;         // [2 * (Np/256 + 1)] per 256-node block: model, end of that model's nodes
+    int *inst_blk0;        // [ninst+1] first 256-node block of each model
+    int *inst_nodes;       // [ninst] nodes of each model (the l2 norm's count)
+    int *node_inst;        // [Np] model of each node (gap nodes: the model before them)
+    int *inst_mask;        // [ninst] 1 = the masked node kernels update this model
+    int *elem_inst, *edge_inst;
+    double *dt_inst;
+    int *loc_bc_inst;
+    std::vector<int> h_inst_node0, h_inst_N;
+    std::vector<double> h_inst_dt;
     // captured BiCGSTAB iteration chunks (k iterations each), replayed with
     // hipGraphLaunch; rebuilt when the kernels or their arguments change
     hipGraphExec_t kgraph[KGRAPH_MAX + 1];

@@ -83,7 +83,15 @@ struct ElemData {
     // constants
     double g, dt, alpha, tau_pg, dll, dul, visc;
     const crlog::tables *logt;
+    // ensembles of independent models in one mesh: per-model time step,
+    // looked up through the element's (edge's) model; null for one model
+    const double *dt_inst;
+    const int *elem_inst, *edge_inst;
+    const int *loc_bc_inst;    // [3 * models] node-string flags of each model's nodes 0..2
 };
+
+HDF double elem_dt(const ElemData &D, int ie) { return D.dt_inst ? D.dt_inst[D.elem_inst[ie]] : D.dt; }
+HDF double edge_dt(const ElemData &D, int ie) { return D.dt_inst ? D.dt_inst[D.edge_inst[ie]] : D.dt; }
 
 // ---------------------------------------------------------------- integrals
 // fe_elem2d_integrations.c:60-62
@@ -538,7 +546,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
         gs[i].x = D.grad[6 * ie + 2 * i];
         gs[i].y = D.grad[6 * ie + 2 * i + 1];
     }
-    const double alpha = D.alpha, dt = D.dt, djac = D.djac[ie], g = D.g, dll = D.dll, dul = D.dul;
+    const double alpha = D.alpha, dt = elem_dt(D, ie), djac = D.djac[ie], g = D.g, dll = D.dll, dul = D.dul;
     const int imat = D.mat[ie];
     const int wd_flag = D.wd_flag[ie];
     double t1;
@@ -836,7 +844,7 @@ HD void boundary_resid(const ElemData &D, int ie, double pert, int pnode, int pv
                        d3 *elem_rhs) {
     const int nd[2] = {D.e1d[2 * ie], D.e1d[2 * ie + 1]};
     const v2 nrml = {D.nrml1d[2 * ie], D.nrml1d[2 * ie + 1]};
-    const double dt = D.dt, djac = D.djac1d[ie], g = D.g;
+    const double dt = edge_dt(D, ie), djac = D.djac1d[ie], g = D.g;
     int PRESSURE_FLAG = 1;
     if (pvar == PERTURB_NONE) psign = 0;
     double h[2];

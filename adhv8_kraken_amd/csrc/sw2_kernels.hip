@@ -134,7 +134,8 @@ __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list,
     }
     if (t & 1) continue;
     const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
-    const int lb[3] = {b0, b1, b2};
+    const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.elem_inst[ie] : nullptr;
+    const int lb[3] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1, lbi ? lbi[2] : b2};
     jac_column_db<3>(D, nodes, lb, j, var, col);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -154,7 +155,8 @@ __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1
     double col[9];
     jac_column<2>(D, ie, j, var, col);
     const int nodes[2] = {D.e1d[2 * ie], D.e1d[2 * ie + 1]};
-    const int lb[2] = {b0, b1};
+    const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.edge_inst[ie] : nullptr;
+    const int lb[2] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1};
     jac_column_db<2>(D, nodes, lb, j, var, col);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
@@ -338,9 +340,10 @@ void launch_assemble_only(Ctx *c) {
 // fe_sw2_init (fe_sw2_init.c:33-79)
 __global__ void k_init_nodes(int N, const double *old_head, const double *old_u, const double *old_v, double *head,
                              double *u, double *v, const int *node_bc, const double *bcval, int *bc_mask,
-                             int *wd_tmp) {
+                             int *wd_tmp, const int *node_inst, const int *mask) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    if (mask && (node_inst[n] < 0 || !mask[node_inst[n]])) return;  // gap nodes (-1) never change
     double hh = old_head[n], uu = old_u[n], vv = old_v[n];
     int m0 = ADH_SW2_NO, m1 = ADH_SW2_NO, m2 = ADH_SW2_NO;
     int bc = node_bc[n];
@@ -360,9 +363,11 @@ __global__ void k_init_nodes(int N, const double *old_head, const double *old_u,
 }
 
 // fe_sw2_wdflag pass 1 (fe_sw2_wdflag.c:41-51): all writers store the same value
-__global__ void k_wd_mark(int E, const int *e2d, const double *head, int *wd_tmp) {
+__global__ void k_wd_mark(int E, const int *e2d, const double *head, int *wd_tmp, const int *elem_inst,
+                          const int *mask) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
+    if (mask && !mask[elem_inst[e]]) return;
     int a = e2d[3 * e], b = e2d[3 * e + 1], c = e2d[3 * e + 2];
     int wet = (head[a] > 0.0) + (head[b] > 0.0) + (head[c] > 0.0);
     if (wet != 3) { wd_tmp[a] = 1; wd_tmp[b] = 1; wd_tmp[c] = 1; }
@@ -370,9 +375,10 @@ __global__ void k_wd_mark(int E, const int *e2d, const double *head, int *wd_tmp
 
 // fe_sw2_wdflag pass 2 (:57-73)
 __global__ void k_wd_flag(int E, const int *e2d, const double *head, const double *old_head, const int *wd_tmp,
-                          int *wd_flag) {
+                          int *wd_flag, const int *elem_inst, const int *mask) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
+    if (mask && !mask[elem_inst[e]]) return;
     int a = e2d[3 * e], b = e2d[3 * e + 1], c = e2d[3 * e + 2];
     int f = (wd_tmp[a] != 0 || wd_tmp[b] != 0 || wd_tmp[c] != 0) ? 1 : 0;
     if (head[a] < 0 && head[b] < 0 && head[c] < 0 && old_head[a] < 0 && old_head[b] < 0 && old_head[c] < 0) f = 2;
@@ -395,22 +401,25 @@ void launch_old_terms(Ctx *c) {
 
 void launch_init(Ctx *c, int *bc_mask_dev) {
     c->split_valid = 0;
+    const int *mask = c->ninst ? c->inst_mask : nullptr;
     hipLaunchKernelGGL(k_init_nodes, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->old_head,
-                       c->old_u, c->old_v, c->head, c->u, c->v, c->node_bc, c->node_bc_val, bc_mask_dev, c->wd_tmp);
+                       c->old_u, c->old_v, c->head, c->u, c->v, c->node_bc, c->node_bc_val, bc_mask_dev, c->wd_tmp,
+                       c->node_inst, mask);
     if (c->E) {
         hipLaunchKernelGGL(k_wd_mark, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
-                           c->wd_tmp);
+                           c->wd_tmp, c->elem_inst, mask);
         hipLaunchKernelGGL(k_wd_flag, dim3(nblocks_for(c->E, 256)), dim3(256), 0, c->stream, c->E, c->e2d, c->head,
-                           c->old_head, c->wd_tmp, c->wd_flag);
+                           c->old_head, c->wd_tmp, c->wd_flag, c->elem_inst, mask);
     }
     launch_old_terms(c);
 }
 
 // fe_sw2 (fe_sw2.c:42-50)
 __global__ void k_shift(int N, double *h, double *u, double *v, double *oh, double *ou, double *ov, double *ooh,
-                        double *oou, double *oov) {
+                        double *oou, double *oov, const int *node_inst, const int *mask) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    if (mask && (node_inst[n] < 0 || !mask[node_inst[n]])) return;  // gap nodes (-1) never change
     ooh[n] = oh[n]; oou[n] = ou[n]; oov[n] = ov[n];
     oh[n] = h[n]; ou[n] = u[n]; ov[n] = v[n];
 }
@@ -419,13 +428,16 @@ void launch_shift(Ctx *c) {
     c->pre_valid = 0;
     c->split_valid = 0;
     hipLaunchKernelGGL(k_shift, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->head, c->u, c->v,
-                       c->old_head, c->old_u, c->old_v, c->older_head, c->older_u, c->older_v);
+                       c->old_head, c->old_u, c->old_v, c->older_head, c->older_u, c->older_v, c->node_inst,
+                       c->ninst ? c->inst_mask : nullptr);
 }
 
 // fe_sw2_inc (fe_sw2_inc.c:47-54): factor 1.0
-__global__ void k_inc(int N, const double *sol, double *h, double *u, double *v) {
+__global__ void k_inc(int N, const double *sol, double *h, double *u, double *v, const int *node_inst,
+                      const int *mask) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    if (mask && (node_inst[n] < 0 || !mask[node_inst[n]])) return;  // gap nodes (-1) never change
     u[n] += 1. * sol[vidx3(n, 0)];
     v[n] += 1. * sol[vidx3(n, 1)];
     h[n] += 1. * sol[vidx3(n, 2)];
@@ -434,17 +446,23 @@ __global__ void k_inc(int N, const double *sol, double *h, double *u, double *v)
 void launch_inc(Ctx *c, double) {
     c->split_valid = 0;
     hipLaunchKernelGGL(k_inc, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->sol, c->head, c->u,
-                       c->v);
+                       c->v, c->node_inst, c->ninst ? c->inst_mask : nullptr);
 }
 
-__global__ void k_scale_vec(size_t n, double *x, double f) {
+// x *= f over a sliced node vector; entry i belongs to node 64 (i / 192) + i mod 64
+__global__ void k_scale_vec(size_t n, double *x, double f, const int *node_inst, const int *mask) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] *= f;
+    if (i >= n) return;
+    if (mask) {
+        const int m = node_inst[64 * (i / 192) + (i & 63)];
+        if (m < 0 || !mask[m]) return;
+    }
+    x[i] *= f;
 }
 
 void launch_scale_sol(Ctx *c, double f) {
     hipLaunchKernelGGL(k_scale_vec, dim3(nblocks_for(3 * c->Np, 256)), dim3(256), 0, c->stream, (size_t)3 * c->Np,
-                       c->sol, f);
+                       c->sol, f, c->node_inst, c->ninst ? c->inst_mask : nullptr);
 }
 
 // plain streaming copy of n doubles (bandwidth calibration for the counters)
@@ -517,10 +535,16 @@ __global__ void __launch_bounds__(256) k_norms(int N, const double *R, const dou
 }
 
 // one block: the block partials of k_norms, reduced in block order
-__global__ void __launch_bounds__(256) k_norms_fin(int nparts, int N, const double *part, Norms *out) {
+// one block per model (blockIdx.x), over that model's 256-node blocks
+__global__ void __launch_bounds__(256) k_norms_fin(int nparts, int N, const double *part, Norms *out,
+                                                   const int *inst_blk0, const int *inst_nodes) {
     __shared__ double smax[256], ssum[256], sinc[256];
+    const int m = blockIdx.x;
+    const int k0 = inst_blk0 ? inst_blk0[m] : 0, k1 = inst_blk0 ? inst_blk0[m + 1] : nparts;
+    if (inst_nodes) N = inst_nodes[m];
+    out += m;
     double a = 0.0, b = 0.0, cc = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         a = fmax(a, part[3 * k]);
         b += part[3 * k + 1];
         cc = fmax(cc, part[3 * k + 2]);
@@ -546,7 +570,8 @@ void launch_norms(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     hipLaunchKernelGGL(k_norms, dim3(nb), dim3(256), 0, c->stream, c->N, c->R, c->sol, c->head, c->old_head,
                        c->red.part);
-    hipLaunchKernelGGL(k_norms_fin, dim3(1), dim3(256), 0, c->stream, nb, c->N, c->red.part, c->norms);
+    hipLaunchKernelGGL(k_norms_fin, dim3(c->ninst ? c->ninst : 1), dim3(256), 0, c->stream, nb, c->N, c->red.part,
+                       c->norms, c->ninst ? c->inst_blk0 : nullptr, c->ninst ? c->inst_nodes : nullptr);
 }
 
 }  // namespace sw2

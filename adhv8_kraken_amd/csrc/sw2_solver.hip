@@ -24,6 +24,12 @@
 
 namespace sw2 {
 
+// model tables of an ensemble (null for one model)
+#define BI(c) ((c)->ninst ? (const int *)(c)->blk_inst : (const int *)nullptr)
+#define IB0(c) ((c)->ninst ? (const int *)(c)->inst_blk0 : (const int *)nullptr)
+#define NFIN(c) ((c)->ninst ? (c)->ninst : 1)
+
+
 constexpr double SOLV_TOL = SMALL;  // src/include/define.h:177
 
 // ------------------------------------------------------------- Jacobi scaling
@@ -192,13 +198,14 @@ __device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K
 // One-block reduction of nparts block partials: thread-strided sums in
 // fixed order, then a fixed tree, so the result is deterministic.
 template <int K>
-__device__ __forceinline__ void reduce_partials(int nparts, const double *part, const int (&op)[K], double (&out)[K]) {
+__device__ __forceinline__ void reduce_partials(int k0, int k1, const double *part, const int (&op)[K],
+                                                double (&out)[K]) {
     __shared__ double sh[K][256 / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+    for (int b = k0 + threadIdx.x; b < k1; b += blockDim.x)
 #pragma unroll
         for (int k = 0; k < K; k++) {
             double x = part[K * b + k];
@@ -227,8 +234,12 @@ __device__ __forceinline__ void reduce_partials(int nparts, const double *part, 
 // finisher stages of the BiCGSTAB reductions (solv_bcgstab.c:117-290)
 enum { FIN_P1, FIN_P2, FIN_K1, FIN_K2, FIN_K3 };
 
+// one block per model (blockIdx.x), over that model's 256-node blocks
 template <int STAGE>
-__global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KScal *sc) {
+__global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KScal *sc, const int *inst_blk0) {
+    const int m = blockIdx.x;
+    const int k0 = inst_blk0 ? inst_blk0[m] : 0, k1 = inst_blk0 ? inst_blk0[m + 1] : nparts;
+    sc += m;
     if ((STAGE == FIN_K1 || STAGE == FIN_K2 || STAGE == FIN_K3) && sc->done) return;
     constexpr int K = STAGE == FIN_P2 ? 2 : (STAGE == FIN_K2 ? 3 : 1);
     int op[K];
@@ -237,7 +248,7 @@ __global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KSc
     else if constexpr (STAGE == FIN_K1) op[0] = 0;
     else { op[0] = 0; op[1] = 0; op[2] = 0; }
     double out[K];
-    reduce_partials<K>(nparts, part, op, out);
+    reduce_partials<K>(k0, k1, part, op, out);
     if (threadIdx.x != 0) return;
     if constexpr (STAGE == FIN_P1) {
         sc->bnorm = out[0];
@@ -359,13 +370,15 @@ struct KVecs {
 
 // prologue part 1: b' = M^-1 b, u = b', ||b'||inf  (solv_bcgstab.c:117-137)
 template <typename P>
-__global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double *rhs, double *part, unsigned *count,
+__global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double *rhs, double *part, const int *blk_inst,
                                                 KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
-    if (n < N) {
+    if (n < nend) {
         const MT *M = V.dinv + didx9(n, 0);
         T b0 = (T)rhs[vidx3(n, 0)], b1 = (T)rhs[vidx3(n, 1)], b2 = (T)rhs[vidx3(n, 2)];
         T y[3];
@@ -384,13 +397,14 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double 
 
 // prologue part 2: r = b' - M^-1 A u, q = r, p = r, rho = (r,q), ||r||inf  (:138-188)
 template <typename P, int K>
-__global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
-                                                unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part, const int *blk_inst, KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
-    if (n < N) {
+    if (n < nend) {
         T y[3];
         amult_prec_row<P, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.u}, y);
         for (int j = 0; j < 3; j++) {
@@ -408,14 +422,15 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *se
 }
 
 template <typename P, int K>
-__global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
-                                                unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part, const int *blk_inst, KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
-    if (n < N) {
+    if (n < nend) {
         T y[3];
         amult_prec_row<P, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, VecX<T>{V.p}, y);
         for (int j = 0; j < 3; j++) {
@@ -429,15 +444,16 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *se
 }
 
 template <typename P, int K>
-__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
-                                                unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part, const int *blk_inst, KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     const T nalpha = (T)(-sc->alpha);
     double v[3] = {0.0, 0.0, 0.0};
-    if (n < N) {
+    if (n < nend) {
         VecS<T> sf{V.r, V.Ap, nalpha};
         T y[3];
         amult_prec_row<P, K>(n, V.diag, V.offd, sell_ptr, sell_col, V.dinv, sf, y);
@@ -456,16 +472,18 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *se
 }
 
 template <typename P>
-__global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     const double alpha = sc->alpha, omega = sc->omega;
     const double beta = (sc->rho * alpha) / (sc->rhop * omega);
     const T ta = (T)alpha, to = (T)omega, tno = (T)(-omega), tb = (T)beta;
     double v[1] = {0.0};
-    if (n < N) {
+    if (n < nend) {
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             T s = V.s[i], p = V.p[i];
@@ -518,10 +536,12 @@ __global__ void k_pl_copy(size_t n, const double *src, double *dst) {
 }
 
 // u = b' (= M^-1 b), ||b'||inf
-__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
-    if (n < N)
+    if (n < nend)
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             const double y = V.b[i];
@@ -533,10 +553,12 @@ __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *
 }
 
 // r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
-__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
-    if (n < N)
+    if (n < nend)
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             const double r = V.b[i] + (-1.0) * V.As[i];
@@ -551,22 +573,26 @@ __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *
 }
 
 // gamma = (q, Ap), alpha = rho / gamma
-__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
-    if (n < N)
+    if (n < nend)
         for (int j = 0; j < 3; j++) v[0] += V.q[vidx3(n, j)] * V.Ap[vidx3(n, j)];
     const int op[1] = {0};
     block_partials<1>(v, op, part);
 }
 
 // omega = (As, s) / (As, As), rho' = -omega (As, q)
-__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, double *part, unsigned *count, KScal *sc) {
+__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
-    if (n < N)
+    if (n < nend)
         for (int j = 0; j < 3; j++) {
             const size_t i = vidx3(n, j);
             const double y = V.As[i];
@@ -608,12 +634,12 @@ template <typename P, int K>
 static void prologue_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     KVecs<P> V = vecs<P>(c);
-    hipLaunchKernelGGL(k_bcg_p1<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, c->red.count + 0,
+    hipLaunchKernelGGL(k_bcg_p1<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->R, c->red.part, BI(c),
                        c->ksc);
-    hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     hipLaunchKernelGGL((k_bcg_p2<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                       c->red.part, c->red.count + 1, c->ksc);
-    hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+                       c->red.part, BI(c), c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
 }
 
 template <typename P, int K>
@@ -622,14 +648,14 @@ static void iters_k(Ctx *c, int k) {
     KVecs<P> V = vecs<P>(c);
     for (int i = 0; i < k; i++) {
         hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                           c->red.part, c->red.count + 2, c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+                           c->red.part, BI(c), c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
         hipLaunchKernelGGL((k_bcg_k2<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                           c->red.part, c->red.count + 3, c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
-        hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 4,
+                           c->red.part, BI(c), c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
+        hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     }
 }
 
@@ -637,7 +663,7 @@ template <typename P, int K>
 static void k1_only_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, vecs<P>(c), c->sell_ptr,
-                       c->sell_col, c->red.part, c->red.count + 2, c->ksc);
+                       c->sell_col, c->red.part, BI(c), c->ksc);
 }
 
 // dispatch on the uniform slice width (0 = per-slice widths, looped)
@@ -661,13 +687,13 @@ static void pl_prologue_k(Ctx *c) {
     const size_t n3 = 3 * (size_t)c->Np;
     hipLaunchKernelGGL(k_pl_copy, dim3(nblocks_for((int)n3, 256)), dim3(256), 0, c->stream, n3, c->R, V.b);
     launch_band_solve(c, V.b, nullptr);
-    hipLaunchKernelGGL(k_pl_p1, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 0, c->ksc);
-    hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+    hipLaunchKernelGGL(k_pl_p1, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
                        V.u, V.As, c->ksc, 0);
     launch_band_solve(c, V.As, nullptr);
-    hipLaunchKernelGGL(k_pl_p2, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 1, c->ksc);
-    hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+    hipLaunchKernelGGL(k_pl_p2, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
+    hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
 }
 
 template <typename T, int K>
@@ -678,18 +704,18 @@ static void pl_iters_k(Ctx *c, int k) {
         hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
                            c->sell_col, V.p, V.Ap, c->ksc, 1);
         launch_band_solve(c, V.Ap, c->ksc);
-        hipLaunchKernelGGL(k_pl_gamma, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 2,
+        hipLaunchKernelGGL(k_pl_gamma, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
         hipLaunchKernelGGL((k_pl_spmv<K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
                            c->sell_col, nullptr, V.As, c->ksc, 1);
         launch_band_solve(c, V.As, c->ksc);
-        hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, c->red.count + 3,
+        hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
         hipLaunchKernelGGL(k_bcg_k3<PrecF64>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
-                           c->red.count + 4, c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(1), dim3(256), 0, c->stream, nb, c->red.part, c->ksc);
+                           BI(c), c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     }
 }
 

@@ -95,7 +95,12 @@ def lib():
         L.adh_sw2_hip_set_system.argtypes = [C.c_void_p, PD, PD, PD]
         L.adh_sw2_hip_newton.argtypes = [C.c_void_p, C.POINTER(Report), PI]
         L.adh_sw2_hip_timestep.argtypes = [C.c_void_p, C.POINTER(Report), PI, C.c_int]
-        L.adh_sw2_hip_timestep_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Report), PI]
+        L.adh_sw2_hip_batch_create.argtypes = [C.POINTER(Mesh), C.c_int, C.POINTER(Opts), C.POINTER(C.c_void_p)]
+        L.adh_sw2_hip_batch_size.argtypes = [C.c_void_p]
+        L.adh_sw2_hip_batch_set_state.argtypes = [C.c_void_p, C.c_int] + [PD] * 6
+        L.adh_sw2_hip_batch_get_state.argtypes = [C.c_void_p, C.c_int, PD, PD]
+        L.adh_sw2_hip_batch_set_dt.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.adh_sw2_hip_batch_timestep.argtypes = [C.c_void_p, C.POINTER(Report), PI, C.c_int]
         L.adh_sw2_hip_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, PD]
         L.adh_sw2_hip_get_system.argtypes = [C.c_void_p, PD, PD, PD]
         L.adh_sw2_hip_probe_math.argtypes = [PD, C.c_int, PD, PD, PD]
@@ -126,6 +131,38 @@ def _pi(a):
     return None if a is None else a.ctypes.data_as(PI)
 
 
+def _mesh_struct(p, keep):
+    """adh_sw2_mesh view of a problem; the arrays it points to go into keep."""
+    def k(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    m = Mesh()
+    m.nnodes, m.nelems2d, m.nelems1d, m.nmat = p.nnodes, p.nelems2d, p.nelems1d, p.nmat
+    m.x, m.y, m.z = _pd(k(p.x, np.float64)), _pd(k(p.y, np.float64)), _pd(k(p.z, np.float64))
+    m.node_bc = _pi(k(p.node_bc, np.int32))
+    m.node_bc_val = _pd(k(p.node_bc_val, np.float64))
+    m.e2d = _pi(k(p.e2d, np.int32))
+    m.e2d_mat = _pi(k(p.e2d_mat, np.int32))
+    m.e2d_ks = _pd(k(p.e2d_ks, np.float64))
+    m.e2d_src_flag = _pi(k(p.e2d_src_flag, np.int32))
+    m.e2d_src = _pd(k(p.e2d_src, np.float64))
+    if p.nelems1d:
+        m.e1d = _pi(k(p.e1d, np.int32))
+        m.e1d_adj = _pi(k(p.e1d_adj, np.int32))
+        m.e1d_bc = _pi(k(p.e1d_bc, np.int32))
+        m.e1d_val = _pd(k(p.e1d_val, np.float64))
+    m.mat_turb = _pi(k(p.mat_turb, np.int32))
+    m.mat_eev_mode = _pi(k(p.mat_eev_mode, np.int32))
+    m.mat_eev_coef = _pd(k(p.mat_eev_coef, np.float64))
+    m.mat_ev = _pd(k(p.mat_ev, np.float64))
+    m.mat_cor_speed = _pd(k(p.mat_cor_speed, np.float64))
+    m.gravity, m.dt, m.tau_temporal, m.tau_pg = p.g, p.dt, p.tau_temporal, p.tau_pg
+    m.drying_lower_limit, m.drying_upper_limit, m.viscosity = p.drying_lower, p.drying_upper, p.viscosity
+    return m
+
+
 class SW2Model:
     """One SW2 model resident on the GPU (see module docstring)."""
 
@@ -134,34 +171,7 @@ class SW2Model:
         p = problem
         self.p = p
         self._keep = []
-
-        def k(a, dt):
-            a = np.ascontiguousarray(a, dtype=dt)
-            self._keep.append(a)
-            return a
-
-        m = Mesh()
-        m.nnodes, m.nelems2d, m.nelems1d, m.nmat = p.nnodes, p.nelems2d, p.nelems1d, p.nmat
-        m.x, m.y, m.z = _pd(k(p.x, np.float64)), _pd(k(p.y, np.float64)), _pd(k(p.z, np.float64))
-        m.node_bc = _pi(k(p.node_bc, np.int32))
-        m.node_bc_val = _pd(k(p.node_bc_val, np.float64))
-        m.e2d = _pi(k(p.e2d, np.int32))
-        m.e2d_mat = _pi(k(p.e2d_mat, np.int32))
-        m.e2d_ks = _pd(k(p.e2d_ks, np.float64))
-        m.e2d_src_flag = _pi(k(p.e2d_src_flag, np.int32))
-        m.e2d_src = _pd(k(p.e2d_src, np.float64))
-        if p.nelems1d:
-            m.e1d = _pi(k(p.e1d, np.int32))
-            m.e1d_adj = _pi(k(p.e1d_adj, np.int32))
-            m.e1d_bc = _pi(k(p.e1d_bc, np.int32))
-            m.e1d_val = _pd(k(p.e1d_val, np.float64))
-        m.mat_turb = _pi(k(p.mat_turb, np.int32))
-        m.mat_eev_mode = _pi(k(p.mat_eev_mode, np.int32))
-        m.mat_eev_coef = _pd(k(p.mat_eev_coef, np.float64))
-        m.mat_ev = _pd(k(p.mat_ev, np.float64))
-        m.mat_cor_speed = _pd(k(p.mat_cor_speed, np.float64))
-        m.gravity, m.dt, m.tau_temporal, m.tau_pg = p.g, p.dt, p.tau_temporal, p.tau_pg
-        m.drying_lower_limit, m.drying_upper_limit, m.viscosity = p.drying_lower, p.drying_upper, p.viscosity
+        m = _mesh_struct(p, self._keep)
         self.opts = opts if opts is not None else default_opts()
         h = C.c_void_p()
         _check(L.adh_sw2_hip_create(C.byref(m), C.byref(self.opts), C.byref(h)))
@@ -300,6 +310,81 @@ class SW2Model:
         vd = C.c_int(0)
         _check(self.L.adh_sw2_hip_timestep(self.h, C.byref(r), C.byref(vd), max_retries))
         return self._rdict(r, vd.value)
+
+
+class SW2Ensemble:
+    """Independent SW2 models advanced together on one GPU: one device mesh
+    holds them side by side, so every phase is one launch over all models,
+    while Newton, Krylov and time-step decisions stay per model (each model
+    computes what it computes alone).  Block-Jacobi preconditioner only."""
+
+    def __init__(self, problems, opts: Opts | None = None):
+        L = lib()
+        self.ps = list(problems)
+        keep = []
+        arr = (Mesh * len(self.ps))()
+        for i, p in enumerate(self.ps):
+            arr[i] = _mesh_struct(p, keep)
+        self.opts = opts if opts is not None else default_opts()
+        h = C.c_void_p()
+        _check(L.adh_sw2_hip_batch_create(arr, len(self.ps), C.byref(self.opts), C.byref(h)))
+        self.h = h
+        self.L = L
+        for i, p in enumerate(self.ps):
+            self.set_state(i, p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+
+    def __len__(self):
+        return len(self.ps)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.adh_sw2_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_opts(self, opts: Opts):
+        self.opts = opts
+        _check(self.L.adh_sw2_hip_set_opts(self.h, C.byref(opts)))
+
+    def set_dt(self, i, dt):
+        _check(self.L.adh_sw2_hip_batch_set_dt(self.h, i, dt))
+
+    def set_state(self, i, head, vel, old_head, old_vel, older_head, older_vel):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (head, vel, old_head, old_vel, older_head, older_vel)]
+        _check(self.L.adh_sw2_hip_batch_set_state(self.h, i, *[_pd(x) for x in a]))
+
+    def get_state(self, i):
+        N = self.ps[i].nnodes
+        hd, vel = np.empty(N), np.empty(2 * N)
+        _check(self.L.adh_sw2_hip_batch_get_state(self.h, i, _pd(hd), _pd(vel)))
+        return hd, vel
+
+    def timestep(self, max_retries=0, hist_cap=64):
+        n = len(self.ps)
+        reps = (Report * n)()
+        hists = []
+        for i in range(n):
+            hh = [np.zeros(hist_cap), np.zeros(hist_cap), np.zeros(hist_cap), np.zeros(hist_cap, dtype=np.int32)]
+            hists.append(hh)
+            reps[i].hist_cap = hist_cap
+            reps[i].hist_max, reps[i].hist_l2, reps[i].hist_inc, reps[i].hist_lin_it = (
+                _pd(hh[0]), _pd(hh[1]), _pd(hh[2]), _pi(hh[3]))
+        vd = np.zeros(n, dtype=np.int32)
+        _check(self.L.adh_sw2_hip_batch_timestep(self.h, reps, _pi(vd), max_retries))
+        out = []
+        for i in range(n):
+            r = reps[i]
+            k = min(r.nhist, hist_cap)
+            out.append(dict(verdict=int(vd[i]), converged=r.converged, iterations=r.iterations,
+                            lin_its=r.lin_iterations, hist_max=hists[i][0][:k].copy(),
+                            hist_l2=hists[i][1][:k].copy(), hist_inc=hists[i][2][:k].copy(),
+                            hist_lin_it=hists[i][3][:k].copy()))
+        return out
 
 
 def cr_log(x: float) -> float:

@@ -163,9 +163,36 @@ int adh_sw2_hip_newton(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict);
  * fe_sw2_solve, retried with dt*0.25 on failure, src/main/adh_func.c:535-549). */
 int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int max_retries);
 
-/* Batched ensemble: n independent models advanced one time step each; the
- * element, assembly and Krylov phases of all models share launches. */
-int adh_sw2_hip_timestep_batch(adh_sw2_hip **hs, int n, adh_sw2_report *reps, int *verdicts);
+/* ---- Ensembles: n independent models advanced together -------------------
+ * No counterpart in the reference, whose fe_newton runs one model at a time
+ * (src/fe/fe_newton.c:214); this is the additive batched entry beside the
+ * single-model drop-in.  The models are laid side by side in one device mesh
+ * (model i's nodes start on a multiple of 256), so every element, assembly,
+ * scaling and BiCGSTAB launch covers all models, while reductions, Krylov
+ * scalars, Newton decisions, line searches and dt cuts stay per model.  A
+ * model in an ensemble computes exactly what it computes alone.
+ * The models must share gravity, tau_temporal, tau_pg, the drying limits and
+ * viscosity; meshes, materials, boundary conditions, states and dt may
+ * differ.  Ensembles use the 3x3 block-Jacobi preconditioner (opts NULL or
+ * preconditioner 2) in any precision form.  The handle works with
+ * set_opts and destroy; the single-model state, init and step calls are not
+ * meant for it. */
+int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const adh_sw2_opts *opts, adh_sw2_hip **out);
+/* number of models (ADH_SW2_EINVAL for a single-model handle) */
+int adh_sw2_hip_batch_size(adh_sw2_hip *h);
+/* state of model i, same layout as adh_sw2_hip_set_state over its own nodes */
+int adh_sw2_hip_batch_set_state(adh_sw2_hip *h, int i, const double *head, const double *vel,
+                                const double *old_head, const double *old_vel, const double *older_head,
+                                const double *older_vel);
+int adh_sw2_hip_batch_get_state(adh_sw2_hip *h, int i, double *head, double *vel);
+int adh_sw2_hip_batch_set_dt(adh_sw2_hip *h, int i, double dt);
+/* One time step of every model, each as adh_sw2_hip_timestep takes it:
+ * shift, fe_newton, and after a failure dt * 0.25 and another attempt, up
+ * to max_retries, for the failed models only (the others wait).  reps
+ * (optional) and verdicts hold n entries.  A model whose dt would fall
+ * below DBL_EPSILON stops with verdict NO (the single-model call returns
+ * an error there). */
+int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, adh_sw2_report *reps, int *verdicts, int max_retries);
 
 /* Reads the device linear system as it currently stands (after load it is
  * unscaled, after linear_sys_setup it is Jacobi-scaled); any pointer may be NULL. */

@@ -25,7 +25,7 @@ def test_header_declares_entry_points():
     names = _declared()
     for must in ["adh_sw2_hip_create", "adh_sw2_hip_resid", "adh_sw2_hip_load", "adh_sw2_hip_init",
                  "adh_sw2_hip_inc", "adh_sw2_hip_linear_sys_setup", "adh_sw2_hip_linear_sys_solve",
-                 "adh_sw2_hip_newton", "adh_sw2_hip_timestep_batch"]:
+                 "adh_sw2_hip_newton", "adh_sw2_hip_batch_create", "adh_sw2_hip_batch_timestep"]:
         assert must in names
 
 
@@ -76,3 +76,25 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(sw2, "_lib", None)
     with pytest.raises(sw2.SW2Error):
         sw2.lib()
+
+
+def test_ensemble_arguments_are_checked_before_the_device():
+    """adh_sw2_hip_batch_create rejects empty ensembles, the profile-LU
+    preconditioner and models that disagree on the shared constants before
+    any device call."""
+    from adhv8_kraken_amd import mesh
+    L = sw2.lib()
+    keep = []
+    a, b = mesh.channel_config1(), mesh.bumpy_channel(8, 4, seed=1)
+    arr = (sw2.Mesh * 2)()
+    arr[0] = sw2._mesh_struct(a, keep)
+    arr[1] = sw2._mesh_struct(b, keep)
+    h = C.c_void_p()
+    assert L.adh_sw2_hip_batch_create(arr, 0, None, C.byref(h)) == -1
+    o = sw2.default_opts(preconditioner=1)
+    assert L.adh_sw2_hip_batch_create(arr, 2, C.byref(o), C.byref(h)) == -5
+    arr[1].gravity = 9.81
+    assert L.adh_sw2_hip_batch_create(arr, 2, None, C.byref(h)) == -1
+    assert b"share gravity" in L.adh_sw2_hip_last_error()
+    assert L.adh_sw2_hip_batch_size(None) == -1
+    assert L.adh_sw2_hip_batch_timestep(None, None, None, 0) == -1

@@ -1,0 +1,101 @@
+"""Ensembles on the GPU: models advanced together through the
+adh_sw2_hip_batch_* entry points must compute exactly what each computes
+alone through the single-model entry points -- same element arithmetic,
+same 256-node reduction partitions and order, same Newton, line-search and
+dt-cut decisions per model.
+
+Bar: bit-exact states (np.array_equal, so +0.0 == -0.0), identical verdicts,
+Newton and Krylov iteration counts, and bit-identical residual histories.
+The channel family holds a model (dt = 100 s) whose first FP64 attempt fails
+and is retried at dt / 4 while the other models wait; the floodplain family
+exercises wetting and drying, line searches and a model that needs
+repeated cuts (dt = 30 s).
+"""
+import numpy as np
+import pytest
+
+from gpu_helpers import say
+from adhv8_kraken_amd import mesh
+from adhv8_kraken_amd.sw2 import SW2Ensemble, SW2Model, default_opts, YES
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 3
+RETRIES = 3
+
+
+def _family(name):
+    if name == "channel":
+        return [mesh.channel_config1(), mesh.bumpy_channel(30, 12, seed=9, dt=100.0),
+                mesh.bumpy_channel(20, 10, seed=4, dt=7.0), mesh.bumpy_channel(60, 30, seed=11, dt=5.0)], 5e-8
+    return [mesh.floodplain(20, 10), mesh.floodplain(16, 8, seed=7, dt=3.0), mesh.floodplain(12, 6, seed=3, dt=1.5),
+            mesh.floodplain(16, 8, seed=7, dt=30.0)], 1e-5
+
+
+def _attempts(r):
+    # every attempt records its initial residual plus one entry per Newton iteration
+    return len(r["hist_max"]) - r["iterations"]
+
+
+@pytest.mark.parametrize("family,precision", [("channel", 0), ("channel", 1), ("channel", 2),
+                                              ("floodplain", 0), ("floodplain", 1)])
+def test_ensemble_matches_single_models(family, precision):
+    ps, tol = _family(family)
+    o = default_opts(precision=precision, tol_nonlin=tol, max_nonlin_it=30)
+    say(f"ensemble {family} precision={precision}: {[p.nnodes for p in ps]} nodes")
+    single = []
+    for p in ps:
+        g = SW2Model(p, o)
+        reps = [g.timestep(max_retries=RETRIES, hist_cap=256) for _ in range(STEPS)]
+        h, v, *_ = g.get_state()
+        single.append((reps, h, v))
+        g.close()
+    ens = SW2Ensemble(ps, o)
+    assert len(ens) == len(ps)
+    ereps = [ens.timestep(max_retries=RETRIES, hist_cap=256) for _ in range(STEPS)]
+    retried = 0
+    for i in range(len(ps)):
+        reps, h, v = single[i]
+        for s in range(STEPS):
+            a, b = reps[s], ereps[s][i]
+            say(f"  model {i} step {s}: single vd={a['verdict']} its={a['iterations']} lin={a['lin_its']} "
+                f"attempts={_attempts(a)} | ensemble vd={b['verdict']} its={b['iterations']} lin={b['lin_its']} "
+                f"attempts={_attempts(b)}")
+            assert a["verdict"] == b["verdict"]
+            assert a["iterations"] == b["iterations"]
+            assert a["lin_its"] == b["lin_its"]
+            # a diverged Krylov solve leaves NaN in the l2 history (it fails the attempt)
+            for key in ("hist_max", "hist_l2", "hist_lin_it"):
+                assert np.array_equal(a[key], b[key], equal_nan=True), (key, a[key], b[key])
+            retried += _attempts(a) > 1
+        eh, ev = ens.get_state(i)
+        assert np.array_equal(h, eh), int(np.count_nonzero(h != eh))
+        assert np.array_equal(v, ev), int(np.count_nonzero(v != ev))
+    ens.close()
+    if precision == 0:
+        # FP64 fails a Newton attempt on an unconverged Krylov solve (the reference's rule); the
+        # refined forms take such a solve as a correction, so they may need no cut at all
+        assert retried >= 1, "no model needed a dt cut; the per-model retry path went untested"
+    assert sum(ereps[s][i]["verdict"] == YES for s in range(STEPS) for i in range(len(ps))) >= 3 * len(ps) // 2
+
+
+def test_ensemble_state_roundtrip_and_dt():
+    """Per-model state and dt calls address their own model's nodes only."""
+    ps, _ = _family("channel")
+    ens = SW2Ensemble(ps[:3])
+    rng = np.random.default_rng(3)
+    states = []
+    for i, p in enumerate(ps[:3]):
+        h = rng.uniform(0.5, 1.5, p.nnodes)
+        v = rng.uniform(-0.1, 0.1, 2 * p.nnodes)
+        ens.set_state(i, h, v, h, v, h, v)
+        states.append((h, v))
+    for i in range(3):
+        h, v = ens.get_state(i)
+        assert np.array_equal(h, states[i][0]) and np.array_equal(v, states[i][1])
+    ens.set_dt(1, 3.0)
+    with pytest.raises(Exception):
+        ens.set_dt(3, 1.0)
+    with pytest.raises(Exception):
+        ens.set_dt(0, 0.0)
+    ens.close()
