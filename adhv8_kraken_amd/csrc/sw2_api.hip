@@ -424,7 +424,7 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
     UP(elist_wet, (const int *)nullptr, E);
     UP(elist_gen, (const int *)nullptr, E);
     UP(ecount, (const int *)nullptr, 2);
-    UP(esplit, (const int *)nullptr, (size_t)nblocks_for(E, 256) + 1);
+    UP(esplit, (const int *)nullptr, 2 * (size_t)nblocks_for(E, 256) + 2);
     c.pre_valid = 0;
     UP(ER, (const double *)nullptr, 9 * (size_t)E + 6 * (size_t)B);
     UP(EM, (const double *)nullptr, 81 * (size_t)E + 36 * (size_t)B);
@@ -1037,6 +1037,7 @@ extern "C" int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *ve
 // same reduction order, same element code.
 
 static int set_mask(Ctx &c, const std::vector<int> &m) {
+    c.split_valid = 0;  // the element lists hold the masked models only
     HIPCHK(hipMemcpyAsync(c.inst_mask, m.data(), sizeof(int) * c.ninst, hipMemcpyHostToDevice, c.stream));
     // the host vector may change before the copy runs
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -1098,6 +1099,8 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
     const adh_sw2_opts &o = c.opts;
     const int n = c.ninst;
     int rc;
+    std::vector<double> tol(n);
+    for (int i = 0; i < n; i++) tol[i] = std::isnan(c.h_inst_tol[i]) ? o.tol_nonlin : c.h_inst_tol[i];
     std::vector<Norms> nm;
     if ((rc = set_mask(c, run))) return rc;
     launch_init(&c, nullptr);   // masked: only the models of this attempt
@@ -1109,6 +1112,7 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
         if (run[i] && reps) hist(reps + i, nm[i], 0);
     auto any = [](const std::vector<int> &v) { for (int x : v) if (x) return true; return false; };
     while (any(active)) {
+        if ((rc = set_mask(c, active))) return rc;
         launch_jacobian(&c);
         launch_assemble(&c);
         launch_zero(&c, c.sol, 3 * (size_t)c.Np);
@@ -1137,7 +1141,7 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
         for (;;) {   // line search, for the models that need it
             std::vector<int> cut(n, 0);
             for (int i = 0; i < n; i++)
-                cut[i] = active[i] && nm[i].maxn > o.tol_nonlin && nm[i].l2 > old_l2[i] && cuts[i] < o.max_cuts;
+                cut[i] = active[i] && nm[i].maxn > tol[i] && nm[i].l2 > old_l2[i] && cuts[i] < o.max_cuts;
             if (!any(cut)) break;
             if ((rc = set_mask(c, cut))) return rc;
             launch_scale_sol(&c, 0.5);
@@ -1157,7 +1161,7 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
                 reps[i].lin_iterations += lin_it[i];
             }
             const bool keep = (solv[i] == ADH_SW2_YES) && (it[i] < o.max_nonlin_it) && (nm[i].maxn < 1.0E+20) &&
-                              (nm[i].maxn > o.tol_nonlin && nm[i].incn > o.inc_nonlin);
+                              (nm[i].maxn > tol[i] && nm[i].incn > o.inc_nonlin);
             if (!keep) active[i] = 0;
         }
     }
@@ -1165,7 +1169,7 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
     verdict.assign(n, ADH_SW2_YES);
     for (int i = 0; i < n; i++) {
         if (!run[i]) continue;
-        const bool f = (solv[i] != ADH_SW2_YES) || ((nm[i].maxn > o.tol_nonlin) && (nm[i].incn > o.inc_nonlin)) ||
+        const bool f = (solv[i] != ADH_SW2_YES) || ((nm[i].maxn > tol[i]) && (nm[i].incn > o.inc_nonlin)) ||
                        (nm[i].maxn >= 1.0E+20);
         failed[i] = f;
         verdict[i] = f ? ADH_SW2_NO : ADH_SW2_YES;
@@ -1302,6 +1306,7 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
     c.h_inst_node0 = node0;
     c.h_inst_N = inst_nodes;
     c.h_inst_dt = dts;
+    c.h_inst_tol.assign(n, std::nan(""));
     c.ninst = n;
     c.ed.dt_inst = c.dt_inst;
     c.ed.elem_inst = c.elem_inst;
@@ -1371,10 +1376,42 @@ extern "C" int adh_sw2_hip_batch_set_dt(adh_sw2_hip *h, int i, double dt) {
     return 0;
 }
 
+extern "C" int adh_sw2_hip_batch_get_dt(adh_sw2_hip *h, int i, double *dt) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    if (!dt) return fail(ADH_SW2_EINVAL, "null argument");
+    *dt = h->c.h_inst_dt[i];
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_batch_set_tol(adh_sw2_hip *h, int i, double tol_nonlin) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    if (!(tol_nonlin > 0)) return fail(ADH_SW2_EINVAL, "tol_nonlin must be positive");
+    h->c.h_inst_tol[i] = tol_nonlin;
+    return 0;
+}
+
+// fe_newton's opening residual (init, residual, norms) for every model
+extern "C" int adh_sw2_hip_batch_residual_norm(adh_sw2_hip *h, double *maxn) {
+    if (!h || h->c.ninst < 1 || !maxn) return fail(ADH_SW2_EINVAL, "null argument or not an ensemble");
+    Ctx &c = h->c;
+    int rc;
+    std::vector<Norms> nm;
+    if ((rc = set_mask(c, std::vector<int>(c.ninst, 1)))) return rc;
+    launch_init(&c, nullptr);
+    launch_resid(&c);
+    launch_norms(&c);
+    if ((rc = read_norms_all(c, nm))) return rc;
+    for (int i = 0; i < c.ninst; i++) maxn[i] = nm[i].maxn;
+    return sync_check(c);
+}
+
 // One time step of every model (fe_main's attempts per model: shift, Newton,
 // dt * 0.25 after a failure, src/main/adh_func.c:535-549).  reps and
 // verdicts hold one entry per model.
-extern "C" int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, adh_sw2_report *reps, int *verdicts, int max_retries) {
+extern "C" int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, const int *which, adh_sw2_report *reps, int *verdicts,
+                                          int max_retries) {
     if (!h || h->c.ninst < 1 || !verdicts) return fail(ADH_SW2_EINVAL, "null argument or not an ensemble");
     Ctx &c = h->c;
     const int n = c.ninst;
@@ -1384,7 +1421,10 @@ extern "C" int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, adh_sw2_report *reps, 
             reps[i].ms_load = reps[i].ms_setup = reps[i].ms_solve = reps[i].ms_resid = reps[i].ms_total = 0;
         }
     std::vector<int> run(n, 1), vd;
-    for (int i = 0; i < n; i++) verdicts[i] = ADH_SW2_NO;
+    for (int i = 0; i < n; i++) {
+        if (which) run[i] = which[i] != 0;
+        verdicts[i] = run[i] ? ADH_SW2_NO : ADH_SW2_YES;
+    }
     for (int attempt = 0; attempt <= max_retries; attempt++) {
         int rc;
         if ((rc = set_mask(c, run))) return rc;

@@ -127,6 +127,7 @@ struct Ctx {
     int *loc_bc_inst;
     std::vector<int> h_inst_node0, h_inst_N;
     std::vector<double> h_inst_dt;
+    std::vector<double> h_inst_tol;   // per-model nonlinear tolerance (NaN: opts.tol_nonlin)
     // captured BiCGSTAB iteration chunks (k iterations each), replayed with
     // hipGraphLaunch; rebuilt when the kernels or their arguments change
     hipGraphExec_t kgraph[KGRAPH_MAX + 1];

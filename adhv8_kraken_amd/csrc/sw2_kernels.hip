@@ -180,64 +180,89 @@ __device__ __forceinline__ bool tri_stays_wet(int ie, const int *e2d, const doub
     return w;
 }
 
-__global__ void __launch_bounds__(256) k_split_count(int E, const int *e2d, const double *head, int *bcount) {
-    __shared__ int wc[4];
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool w = ie < E && tri_stays_wet(ie, e2d, head);
-    const unsigned long long bw = __ballot(w);
-    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(bw);
-    __syncthreads();
-    if (threadIdx.x == 0) bcount[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+// Triangle classes for the element kernels: "wet" (stays wet under the -eps
+// perturbation: the variant without clipping code), "general", or skipped
+// (an ensemble model that is not being iterated).  bcount holds the per-block
+// wet counts, then the per-block general counts.
+__device__ __forceinline__ void split_class(int ie, int E, const int *e2d, const double *head, const int *elem_inst,
+                                            const int *mask, bool &w, bool &g) {
+    const bool act = ie < E && (!mask || mask[elem_inst[ie]]);
+    w = act && tri_stays_wet(ie, e2d, head);
+    g = act && !w;
 }
 
-// exclusive scan of the block counts (one block); counts[0] = wet, counts[1] = rest
-__global__ void __launch_bounds__(256) k_split_scan(int nb, int E, int *bcount, int *counts) {
-    __shared__ int tot[256];
-    const int per = (nb + 255) / 256, b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
-    int s = 0;
-    for (int b = b0; b < b1; b++) s += bcount[b];
-    tot[threadIdx.x] = s;
+__global__ void __launch_bounds__(256) k_split_count(int E, const int *e2d, const double *head, const int *elem_inst,
+                                                     const int *mask, int *bcount) {
+    __shared__ int wc[2][4];
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    bool w, g;
+    split_class(ie, E, e2d, head, elem_inst, mask, w, g);
+    const unsigned long long bw = __ballot(w), bg = __ballot(g);
+    if ((threadIdx.x & 63) == 0) {
+        wc[0][threadIdx.x >> 6] = __popcll(bw);
+        wc[1][threadIdx.x >> 6] = __popcll(bg);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int run = 0;
-        for (int t = 0; t < 256; t++) { int v = tot[t]; tot[t] = run; run += v; }
-        counts[0] = run;
-        counts[1] = E - run;
+        bcount[blockIdx.x] = wc[0][0] + wc[0][1] + wc[0][2] + wc[0][3];
+        bcount[gridDim.x + blockIdx.x] = wc[1][0] + wc[1][1] + wc[1][2] + wc[1][3];
     }
-    __syncthreads();
-    int run = tot[threadIdx.x];
-    for (int b = b0; b < b1; b++) { int v = bcount[b]; bcount[b] = run; run += v; }
 }
 
-__global__ void __launch_bounds__(256) k_split_write(int E, const int *e2d, const double *head, const int *boff,
-                                                     int *wet, int *gen) {
-    __shared__ int wc[4];
+// exclusive scans of the two block-count arrays (one block); counts[0] = wet
+// total, counts[1] = general total
+__global__ void __launch_bounds__(256) k_split_scan(int nb, int *bcount, int *counts) {
+    __shared__ int tot[256];
+    const int per = (nb + 255) / 256, b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    for (int a = 0; a < 2; a++) {
+        int *bc = bcount + a * nb;
+        int s = 0;
+        for (int b = b0; b < b1; b++) s += bc[b];
+        tot[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int t = 0; t < 256; t++) { int v = tot[t]; tot[t] = run; run += v; }
+            counts[a] = run;
+        }
+        __syncthreads();
+        int run = tot[threadIdx.x];
+        for (int b = b0; b < b1; b++) { int v = bc[b]; bc[b] = run; run += v; }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_split_write(int E, const int *e2d, const double *head, const int *elem_inst,
+                                                     const int *mask, const int *boff, int *wet, int *gen) {
+    __shared__ int wc[2][4];
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool w = ie < E && tri_stays_wet(ie, e2d, head);
-    const unsigned long long bw = __ballot(w);
+    bool w, g;
+    split_class(ie, E, e2d, head, elem_inst, mask, w, g);
+    const unsigned long long bw = __ballot(w), bg = __ballot(g);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wc[wid] = __popcll(bw);
-    __syncthreads();
-    int before = 0;  // wet triangles of this block in earlier waves
-    for (int k = 0; k < wid; k++) before += wc[k];
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int wet_rank = boff[blockIdx.x] + before + __popcll(bw & below);
-    const int idx_in_block = threadIdx.x;
-    const int gen_rank = (blockIdx.x * blockDim.x - boff[blockIdx.x]) + (idx_in_block - (before + __popcll(bw & below)));
-    if (ie < E) {
-        if (w) wet[wet_rank] = ie;
-        else gen[gen_rank] = ie;
+    if (lane == 0) {
+        wc[0][wid] = __popcll(bw);
+        wc[1][wid] = __popcll(bg);
     }
+    __syncthreads();
+    int bef_w = 0, bef_g = 0;  // triangles of each class in earlier waves of this block
+    for (int k = 0; k < wid; k++) { bef_w += wc[0][k]; bef_g += wc[1][k]; }
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (w) wet[boff[blockIdx.x] + bef_w + __popcll(bw & below)] = ie;
+    if (g) gen[boff[gridDim.x + blockIdx.x] + bef_g + __popcll(bg & below)] = ie;
 }
 
-// ordered wet / general triangle lists for the current depths
+// ordered wet / general triangle lists for the current depths (and, for
+// ensembles, the models being iterated)
 static void launch_split(Ctx *c) {
     if (!c->E) return;
     const int nb = nblocks_for(c->E, 256);
-    hipLaunchKernelGGL(k_split_count, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->esplit);
-    hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(256), 0, c->stream, nb, c->E, c->esplit, c->ecount);
-    hipLaunchKernelGGL(k_split_write, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->esplit,
-                       c->elist_wet, c->elist_gen);
+    const int *mask = c->ninst ? c->inst_mask : nullptr;
+    hipLaunchKernelGGL(k_split_count, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->elem_inst, mask,
+                       c->esplit);
+    hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(256), 0, c->stream, nb, c->esplit, c->ecount);
+    hipLaunchKernelGGL(k_split_write, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->elem_inst, mask,
+                       c->esplit, c->elist_wet, c->elist_gen);
     c->split_valid = 1;
 }
 

@@ -248,6 +248,61 @@ def bumpy_channel(ncx: int, ncy: int, seed: int = 2, nbumps: int = 20, dt: float
     return p
 
 
+def log_uniform_sizes(n: int, lo: float, hi: float, seed: int) -> list:
+    """n triangle counts drawn log-uniform in [lo, hi]."""
+    rng = SplitMix64(seed)
+    return [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(n)]
+
+
+def _grid_for(tris: int):
+    # 2:1 channel of ncx x ncy cells, two triangles per cell
+    ncy = max(2, int(round(math.sqrt(tris / 4.0))))
+    return 2 * ncy, ncy
+
+
+def channel_instance(tris: int, seed: int, dt: float = 5.0, nbumps: int = 20) -> SW2Problem:
+    """One channel of the batched ensemble: about `tris` triangles (2:1 cells
+    grid, dx = dy = 10 m), bed slope 1e-4 plus seeded Gaussian bumps
+    (amplitude U(0,0.3) m, sigma U(5,30) m), one Manning n U(0.015,0.05) and
+    an inflow U(5,50) m^3/s for the instance, h0 = 1 m, 1 m tailwater."""
+    ncx, ncy = _grid_for(tris)
+    rng = SplitMix64(seed)
+    L, W = ncx * 10.0, ncy * 10.0
+    n = rng.uniform(0.015, 0.05)
+    Q = rng.uniform(5.0, 50.0)
+    bumps = [(rng.uniform(0, L), rng.uniform(0, W), rng.uniform(0.0, 0.3), rng.uniform(5.0, 30.0))
+             for _ in range(nbumps)]
+
+    def zfun(x, y):
+        z = -(1.0e-4 * x)
+        for (bx, by, a, sg) in bumps:
+            z = z + a * np.exp(-((x - bx) ** 2 + (y - by) ** 2) / (2.0 * sg * sg))
+        return z
+
+    p = flume(ncx + 1, ncy + 1, L, W, zfun=zfun, n_manning=n, inflow_q=Q / W, tail_depth=1.0, h0=1.0, dt=dt,
+              name=f"channel-{ncx}x{ncy}-s{seed}", nsteps=20)
+    p.meta.update(seed=seed, manning=n, inflow=Q)
+    return p
+
+
+def channel_ensemble(n: int = 256, lo: int = 5000, hi: int = 200000, seed: int = 4) -> list:
+    """Batched channel ensemble: n instances, triangle counts log-uniform in
+    [lo, hi], instance k seeded seed * 1000 + k."""
+    return [channel_instance(t, seed * 1000 + k) for k, t in enumerate(log_uniform_sizes(n, lo, hi, seed))]
+
+
+def floodplain_ensemble(n: int = 64, lo: int = 50000, hi: int = 500000, seed: int = 5, dt: float = 2.0) -> list:
+    """Batched wet/dry ensemble: n sloped floodplains (cross-slope 1e-3, 40 %
+    of nodes dry, rainfall 1e-5 m/s), triangle counts log-uniform in
+    [lo, hi], instance k seeded seed * 1000 + k (Manning n and inflow drawn
+    per instance)."""
+    out = []
+    for k, t in enumerate(log_uniform_sizes(n, lo, hi, seed)):
+        ncx, ncy = _grid_for(t)
+        out.append(floodplain(ncx, ncy, seed=seed * 1000 + k, dt=dt))
+    return out
+
+
 def config_problem(idx: int) -> SW2Problem:
     if idx == 0:
         return channel_config1()

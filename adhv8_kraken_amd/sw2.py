@@ -100,7 +100,10 @@ def lib():
         L.adh_sw2_hip_batch_set_state.argtypes = [C.c_void_p, C.c_int] + [PD] * 6
         L.adh_sw2_hip_batch_get_state.argtypes = [C.c_void_p, C.c_int, PD, PD]
         L.adh_sw2_hip_batch_set_dt.argtypes = [C.c_void_p, C.c_int, C.c_double]
-        L.adh_sw2_hip_batch_timestep.argtypes = [C.c_void_p, C.POINTER(Report), PI, C.c_int]
+        L.adh_sw2_hip_batch_timestep.argtypes = [C.c_void_p, PI, C.POINTER(Report), PI, C.c_int]
+        L.adh_sw2_hip_batch_get_dt.argtypes = [C.c_void_p, C.c_int, PD]
+        L.adh_sw2_hip_batch_set_tol.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.adh_sw2_hip_batch_residual_norm.argtypes = [C.c_void_p, PD]
         L.adh_sw2_hip_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, PD]
         L.adh_sw2_hip_get_system.argtypes = [C.c_void_p, PD, PD, PD]
         L.adh_sw2_hip_probe_math.argtypes = [PD, C.c_int, PD, PD, PD]
@@ -354,6 +357,20 @@ class SW2Ensemble:
     def set_dt(self, i, dt):
         _check(self.L.adh_sw2_hip_batch_set_dt(self.h, i, dt))
 
+    def get_dt(self, i):
+        dt = C.c_double(0.0)
+        _check(self.L.adh_sw2_hip_batch_get_dt(self.h, i, C.byref(dt)))
+        return dt.value
+
+    def set_tol(self, i, tol_nonlin):
+        _check(self.L.adh_sw2_hip_batch_set_tol(self.h, i, tol_nonlin))
+
+    def residual_norm(self):
+        """max-norm of each model's opening Newton residual (applies init)."""
+        out = np.empty(len(self.ps))
+        _check(self.L.adh_sw2_hip_batch_residual_norm(self.h, _pd(out)))
+        return out
+
     def set_state(self, i, head, vel, old_head, old_vel, older_head, older_vel):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (head, vel, old_head, old_vel, older_head, older_vel)]
         _check(self.L.adh_sw2_hip_batch_set_state(self.h, i, *[_pd(x) for x in a]))
@@ -364,8 +381,10 @@ class SW2Ensemble:
         _check(self.L.adh_sw2_hip_batch_get_state(self.h, i, _pd(hd), _pd(vel)))
         return hd, vel
 
-    def timestep(self, max_retries=0, hist_cap=64):
+    def timestep(self, max_retries=0, hist_cap=64, which=None):
+        """One time step of every model (or of those with which[i] true)."""
         n = len(self.ps)
+        sel = None if which is None else np.ascontiguousarray(np.asarray(which, dtype=np.int32))
         reps = (Report * n)()
         hists = []
         for i in range(n):
@@ -375,7 +394,7 @@ class SW2Ensemble:
             reps[i].hist_max, reps[i].hist_l2, reps[i].hist_inc, reps[i].hist_lin_it = (
                 _pd(hh[0]), _pd(hh[1]), _pd(hh[2]), _pi(hh[3]))
         vd = np.zeros(n, dtype=np.int32)
-        _check(self.L.adh_sw2_hip_batch_timestep(self.h, reps, _pi(vd), max_retries))
+        _check(self.L.adh_sw2_hip_batch_timestep(self.h, _pi(sel), reps, _pi(vd), max_retries))
         out = []
         for i in range(n):
             r = reps[i]

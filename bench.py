@@ -125,6 +125,127 @@ def cpu_baseline(seconds_budget=20.0):
                        f"{its} Newton its in {dt:.2f}s; rate scaled by 40k/1M triangles")
 
 
+def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2):
+    """Every instance of the ensemble advanced `steps` time steps together
+    (one launch per phase for all instances); each instance's nonlinear
+    tolerance is rtol times its own opening residual."""
+    from adhv8_kraken_amd.sw2 import SW2Ensemble, default_opts, YES
+    t0 = time.perf_counter()
+    ens = SW2Ensemble(ps, default_opts(precision=precision, max_nonlin_it=40))
+    t_build = time.perf_counter() - t0
+    r0 = ens.residual_norm()
+    for i, p in enumerate(ps):
+        ens.set_state(i, p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+        ens.set_tol(i, rtol * r0[i])
+    for _ in range(warmup):
+        ens.timestep(max_retries=max_retries)
+    its = np.zeros(len(ps), dtype=np.int64)
+    lin = np.zeros(len(ps), dtype=np.int64)
+    fails = 0
+    dts = [[] for _ in ps]
+    t0 = time.perf_counter()
+    for s in range(steps):
+        rs = ens.timestep(max_retries=max_retries)
+        for i, r in enumerate(rs):
+            its[i] += r["iterations"]
+            lin[i] += r["lin_its"]
+            fails += r["verdict"] != YES
+            dts[i].append((r["verdict"], ens.get_dt(i)))
+        log(f"  precision={precision} step {s}: newton its {int(sum(r['iterations'] for r in rs))} "
+            f"(max {max(r['iterations'] for r in rs)}), failed {sum(r['verdict'] != YES for r in rs)}")
+    elapsed = time.perf_counter() - t0
+    tris = np.array([p.nelems2d for p in ps], dtype=np.float64)
+    states = [ens.get_state(i) for i in range(len(ps))]
+    ens.close()
+    return dict(seconds=elapsed, build_s=t_build, newton_its=int(its.sum()), lin_its=int(lin.sum()), fails=fails,
+                inst_newton_per_s=float(its.sum() / elapsed), tri_newton_per_s=float((its * tris).sum() / elapsed),
+                states=states, dts=dts)
+
+
+def run_sequential(ps, precision, steps, rtol, max_retries=2):
+    """The same instances one after another through the single-model path."""
+    from adhv8_kraken_amd.sw2 import SW2Model, default_opts
+    tri_its = 0.0
+    total = 0.0
+    for p in ps:
+        g = SW2Model(p, default_opts(precision=precision))
+        g.shift()
+        g.init()
+        r0 = float(np.abs(g.resid()).max())
+        g.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+        g.set_opts(default_opts(precision=precision, tol_nonlin=rtol * r0, max_nonlin_it=40))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = g.timestep(max_retries=max_retries)
+            tri_its += r["iterations"] * p.nelems2d
+        total += time.perf_counter() - t0
+        g.close()
+    return dict(seconds=total, tri_newton_per_s=tri_its / total)
+
+
+def ensemble_mode(args):
+    from adhv8_kraken_amd import mesh
+    if args.workload == "channel-ensemble":
+        ps = mesh.channel_ensemble()
+        steps = args.steps if args.steps_set else 20
+        forms = (0, 1)
+        desc = "256 channels, 5k-200k triangles log-uniform (seed 4), Manning n U(0.015,0.05), inflow U(5,50) m^3/s"
+    else:
+        ps = mesh.floodplain_ensemble()
+        steps = args.steps if args.steps_set else 10   # a bounded sample of the 50-step run
+        forms = (1, 0)
+        desc = ("64 wet/dry floodplains, 50k-500k triangles log-uniform (seed 5), cross-slope 1e-3, 40% dry, "
+                "rain 1e-5 m/s, dt 2 s (cut by 0.25 per failed attempt, up to 2 retries)")
+    ntri = sum(p.nelems2d for p in ps)
+    log(f"{args.workload}: {len(ps)} instances, {ntri} triangles, {sum(p.nnodes for p in ps)} nodes")
+    res = {}
+    for prec in forms:
+        log(f"form precision={prec}")
+        res[prec] = run_ensemble(ps, prec, steps, args.warmup, 1e-8)
+        log(f"  {res[prec]['tri_newton_per_s'] / 1e6:.1f} M triangle-Newton/s")
+    err = None
+    if 0 in res and 1 in res:
+        # instances whose two runs took the same time steps (a failed step
+        # cuts dt, after which the difference is time discretisation, not precision)
+        eh = ev = 0.0
+        same = 0
+        for i, ((h64, v64), (h32, v32)) in enumerate(zip(res[0]["states"], res[1]["states"])):
+            if res[0]["dts"][i] != res[1]["dts"][i]:
+                continue
+            same += 1
+            eh = max(eh, float(np.abs(h32 - h64).max() / max(np.abs(h64).max(), 1e-300)))
+            ev = max(ev, float(np.abs(v32 - v64).max() / max(np.abs(v64).max(), 1e-300)))
+        err = dict(head=eh, velocity=ev, instances_compared=same)
+    seq = None
+    if args.compare > 0:
+        sub = ps[: args.compare]
+        csteps = 3
+        e = run_ensemble(sub, forms[0], csteps, 0, 1e-8)
+        q = run_sequential(sub, forms[0], csteps, 1e-8)
+        seq = dict(instances=len(sub), steps=csteps, ensemble_tri_newton_per_s=round(e["tri_newton_per_s"]),
+                   sequential_tri_newton_per_s=round(q["tri_newton_per_s"]),
+                   speedup=round(e["tri_newton_per_s"] / q["tri_newton_per_s"], 2))
+    main = res[forms[0]]
+    out = {
+        "metric": "SW2 ensemble Newton throughput (sum of triangles x Newton iterations per second)",
+        "value": round(main["tri_newton_per_s"]),
+        "unit": "triangle-Newton steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if forms[0] == 0 else "f32+f64", "data": "synthetic",
+        "config": {"workload": desc, "instances": len(ps), "triangles": ntri, "time_steps": steps,
+                   "newton_tol": "1e-8 of each instance's opening residual"},
+        "forms": {str(k): dict(tri_newton_per_s=round(v["tri_newton_per_s"]),
+                               instance_newton_per_s=round(v["inst_newton_per_s"], 1),
+                               equiv_1M_tri_newton_per_s=round(v["tri_newton_per_s"] / 1e6, 2),
+                               newton_its=v["newton_its"], lin_its=v["lin_its"], failed_steps=v["fails"],
+                               seconds=round(v["seconds"], 2), build_s=round(v["build_s"], 1))
+                  for k, v in res.items()},
+        "fp32_refined_max_rel_error_vs_fp64": err,
+        "vs_sequential_single_model": seq,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def kernel_mode(p):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts
     out = {}
@@ -144,20 +265,30 @@ def kernel_mode(p):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4, help="timed time steps (each several Newton steps)")
+    ap.add_argument("--steps", type=int, default=None, help="timed time steps (each several Newton steps)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ncx", type=int, default=1000)
     ap.add_argument("--ncy", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=0, help="1: replay captured HIP graphs of Krylov chunks")
+    ap.add_argument("--workload", default="channel", choices=["channel", "channel-ensemble", "floodplain-ensemble"],
+                    help="channel: the 1M-triangle headline; the ensembles print their own JSON line")
+    ap.add_argument("--compare", type=int, default=0,
+                    help="ensembles: also run the first K instances one by one through the single-model path")
     ap.add_argument("--kernels", action="store_true",
                     help="profiling mode: one time step per form, then the K1 kernel, the FD Jacobian "
                          "and the calibration copy launched back to back (for rocprofv3 counter passes)")
     args = ap.parse_args()
+    args.steps_set = args.steps is not None
+    if args.steps is None:
+        args.steps = 4
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described in "
               f"DESIGN.md but not run here", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "channel":
+        ensemble_mode(args)
+        return
     from adhv8_kraken_amd import mesh
     log(f"building the {args.ncx}x{args.ncy}-cell channel")
     p = mesh.bumpy_channel(args.ncx, args.ncy, seed=2)

@@ -186,13 +186,24 @@ int adh_sw2_hip_batch_set_state(adh_sw2_hip *h, int i, const double *head, const
                                 const double *older_vel);
 int adh_sw2_hip_batch_get_state(adh_sw2_hip *h, int i, double *head, double *vel);
 int adh_sw2_hip_batch_set_dt(adh_sw2_hip *h, int i, double dt);
-/* One time step of every model, each as adh_sw2_hip_timestep takes it:
- * shift, fe_newton, and after a failure dt * 0.25 and another attempt, up
- * to max_retries, for the failed models only (the others wait).  reps
- * (optional) and verdicts hold n entries.  A model whose dt would fall
- * below DBL_EPSILON stops with verdict NO (the single-model call returns
- * an error there). */
-int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, adh_sw2_report *reps, int *verdicts, int max_retries);
+/* current dt of model i (after any cuts the last time step made) */
+int adh_sw2_hip_batch_get_dt(adh_sw2_hip *h, int i, double *dt);
+/* nonlinear tolerance of model i (the reference's NTL is one absolute value
+ * per model); models without one use opts.tol_nonlin */
+int adh_sw2_hip_batch_set_tol(adh_sw2_hip *h, int i, double tol_nonlin);
+/* max-norm of every model's opening Newton residual (init, residual, norms
+ * as fe_newton starts); maxn holds n entries.  Applies init to every model,
+ * as adh_sw2_hip_init does. */
+int adh_sw2_hip_batch_residual_norm(adh_sw2_hip *h, double *maxn);
+/* One time step of every selected model (which[i] != 0; which NULL = all),
+ * each as adh_sw2_hip_timestep takes it: shift, fe_newton, and after a
+ * failure dt * 0.25 and another attempt, up to max_retries, for the failed
+ * models only (the others wait).  Unselected models are left untouched and
+ * report YES.  reps (optional) and verdicts hold n entries.  A model whose
+ * dt would fall below DBL_EPSILON stops with verdict NO (the single-model
+ * call returns an error there). */
+int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, const int *which, adh_sw2_report *reps, int *verdicts,
+                               int max_retries);
 
 /* Reads the device linear system as it currently stands (after load it is
  * unscaled, after linear_sys_setup it is Jacobi-scaled); any pointer may be NULL. */

@@ -97,4 +97,4 @@ def test_ensemble_arguments_are_checked_before_the_device():
     assert L.adh_sw2_hip_batch_create(arr, 2, None, C.byref(h)) == -1
     assert b"share gravity" in L.adh_sw2_hip_last_error()
     assert L.adh_sw2_hip_batch_size(None) == -1
-    assert L.adh_sw2_hip_batch_timestep(None, None, None, 0) == -1
+    assert L.adh_sw2_hip_batch_timestep(None, None, None, None, 0) == -1

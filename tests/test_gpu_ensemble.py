@@ -99,3 +99,30 @@ def test_ensemble_state_roundtrip_and_dt():
     with pytest.raises(Exception):
         ens.set_dt(0, 0.0)
     ens.close()
+
+
+def test_ensemble_subset_steps_and_dt_readback():
+    """Models left out of a time step stay untouched; the dt a failed attempt
+    cut is what get_dt reports (channel model 1 fails at dt 100 s in FP64)."""
+    ps, tol = _family("channel")
+    ps = ps[:3]
+    o = default_opts(tol_nonlin=tol, max_nonlin_it=30)
+    counts = [3, 1, 3]
+    single = []
+    for p, k in zip(ps, counts):
+        g = SW2Model(p, o)
+        for _ in range(k):
+            g.timestep(max_retries=RETRIES)
+        h, v, *_ = g.get_state()
+        single.append((h, v))
+        g.close()
+    ens = SW2Ensemble(ps, o)
+    ens.timestep(max_retries=RETRIES, which=[1, 1, 1])
+    assert ens.get_dt(1) == 25.0 and ens.get_dt(0) == ps[0].dt
+    for _ in range(2):
+        rs = ens.timestep(max_retries=RETRIES, which=[1, 0, 1])
+        assert rs[1]["verdict"] == YES and rs[1]["iterations"] == 0
+    for i in range(3):
+        h, v = ens.get_state(i)
+        assert np.array_equal(h, single[i][0]) and np.array_equal(v, single[i][1]), i
+    ens.close()
