@@ -1444,3 +1444,81 @@ extern "C" int adh_sw2_hip_batch_timestep(adh_sw2_hip *h, const int *which, adh_
     }
     return 0;
 }
+
+// ===========================================================================
+// Launch-shape autotuning of the element list kernels
+
+extern "C" int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out) {
+    if (!h || !out) return fail(ADH_SW2_EINVAL, "null argument");
+    for (int k = 0; k < 4; k++) {
+        out->block[k] = h->c.tune.bs[k];
+        out->grid[k] = h->c.tune.grid[k];
+        out->ms[k] = 0.0;
+    }
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_set_tuning(adh_sw2_hip *h, const adh_sw2_tuning *t) {
+    if (!h || !t) return fail(ADH_SW2_EINVAL, "null argument");
+    for (int k = 0; k < 4; k++) {
+        if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
+            return fail(ADH_SW2_EINVAL, "list kernel block size must be 64, 128 or 256");
+        if (t->grid[k] < 1 || t->grid[k] > (1 << 20)) return fail(ADH_SW2_EINVAL, "list kernel grid out of range");
+    }
+    for (int k = 0; k < 4; k++) {
+        h->c.tune.bs[k] = t->block[k];
+        h->c.tune.grid[k] = t->grid[k];
+    }
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *out) {
+    if (!h || reps < 1) return fail(ADH_SW2_EINVAL, "null handle or reps < 1");
+    Ctx &c = h->c;
+    int rc;
+    if (c.ninst && (rc = set_mask(c, std::vector<int>(c.ninst, 1)))) return rc;
+    if (!c.E) {
+        if (out) adh_sw2_hip_get_tuning(h, out);
+        return 0;
+    }
+    ensure_lists(&c);
+    int cnt[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, c.ecount, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    static const int BS[3] = {64, 128, 256};
+    static const int GRID[8] = {256, 512, 1024, 2048, 4096, 8192, 16384, 32768};
+    adh_sw2_tuning best;
+    adh_sw2_hip_get_tuning(h, &best);
+    for (int phase = 0; phase < 4; phase++) {
+        const int items = (phase < 2 ? 18 : 1) * cnt[phase & 1];
+        if (items == 0) continue;
+        double tbest = 1e30;
+        for (int b : BS) {
+            const int full = nblocks_for(items, b);
+            for (int gi = 0; gi < 8; gi++) {
+                const int g = std::min(GRID[gi], full);
+                if (gi > 0 && std::min(GRID[gi - 1], full) == g) continue;   // same shape as the previous one
+                launch_list_phase(&c, phase, b, g);
+                (void)hipEventRecord(c.ev[0], c.stream);
+                for (int r = 0; r < reps; r++) launch_list_phase(&c, phase, b, g);
+                (void)hipEventRecord(c.ev[1], c.stream);
+                HIPCHK(hipEventSynchronize(c.ev[1]));
+                HIPCHK(hipGetLastError());
+                const double ms = (double)elapsed(c, 0, 1) / reps;
+                if (ms < tbest) {
+                    tbest = ms;
+                    best.block[phase] = b;
+                    best.grid[phase] = g;
+                    best.ms[phase] = ms;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 4; k++) {
+        c.tune.bs[k] = best.block[k];
+        c.tune.grid[k] = best.grid[k];
+    }
+    if (out) *out = best;
+    return sync_check(c);
+}
+

@@ -86,6 +86,12 @@ struct Ctx {
     int *elist_wet, *elist_gen, *ecount;  // FD-Jacobian triangle lists [E], [E], counts [2]
     int *esplit;                          // per-block counts / offsets of the split [E/256 + 1]
     int split_valid;                      // the lists match the current depths
+    // launch shape of the four list kernels (Jacobian wet/general, residual
+    // wet/general): block size and grid-stride grid size
+    struct Tuning {
+        int bs[4] = {128, 128, 128, 128};
+        int grid[4] = {4096, 1024, 2048, 512};
+    } tune;
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state
     double *ER;            // element residuals [9E + 6B]
@@ -181,6 +187,8 @@ void launch_k1_only(Ctx *c);
 void launch_jac2d_only(Ctx *c);
 void launch_resid2d_only(Ctx *c);
 void launch_assemble_only(Ctx *c);
+void ensure_lists(Ctx *c);
+void launch_list_phase(Ctx *c, int phase, int bs, int grid);
 std::string band_structure(Ctx *c, int nblock);
 void band_free(Ctx *c);
 void launch_band_setup(Ctx *c);

@@ -24,8 +24,8 @@ static void launch_resid2d_split(Ctx *c, const ElemData &D);
 // ------------------------------------------------------------------ residual
 // The same over a triangle list (grid-stride over the device-side count):
 // WET lists hold triangles with positive depths, run without clipping code.
-template <bool WET>
-__global__ void __launch_bounds__(128) k_resid2d_list(ElemData D, const int *list, const int *count, double *ER) {
+template <bool WET, int BS>
+__global__ void __launch_bounds__(BS) k_resid2d_list(ElemData D, const int *list, const int *count, double *ER) {
     const int n = *count;
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x) {
         const int ie = list[li];
@@ -105,9 +105,9 @@ void launch_resid(Ctx *c) {
 // each lane carries one residual evaluation; the pair forms the central
 // difference with a lane exchange.  WET lists hold triangles whose depths stay
 // positive under every perturbation and run without the wet/dry clipping code.
-template <bool WET>
-__global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list, const int *count, int b0, int b1,
-                                                    int b2, const int *blkpos, double *EM) {
+template <bool WET, int BS>
+__global__ void __launch_bounds__(BS) k_jac2d_list(ElemData D, const int *list, const int *count, int b0, int b1,
+                                                   int b2, const int *blkpos, double *EM) {
     const int nt = 18 * *count;
     // grid-stride over (triangle, column, sign); the stride is a multiple of
     // the wavefront, so both lanes of a pair run and leave together
@@ -145,8 +145,6 @@ __global__ void __launch_bounds__(128) k_jac2d_list(ElemData D, const int *list,
     }
     }
 }
-template __global__ void k_jac2d_list<true>(ElemData, const int *, const int *, int, int, int, const int *, double *);
-template __global__ void k_jac2d_list<false>(ElemData, const int *, const int *, int, int, int, const int *, double *);
 
 __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, const int *blkpos1, double *EM) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,28 +264,65 @@ static void launch_split(Ctx *c) {
     c->split_valid = 1;
 }
 
-// grid-stride launches sized to fill the chip at each kernel's occupancy
-// (the list lengths stay on the device)
-constexpr int JAC_WET_BLOCKS = 4096, JAC_GEN_BLOCKS = 1024, RES_WET_BLOCKS = 2048, RES_GEN_BLOCKS = 512;
+// Grid-stride list launches.  Block size and grid size change only the
+// schedule (every triangle's arithmetic and output slot are fixed), so they
+// are free parameters: defaults below, or what adh_sw2_hip_autotune measured.
+template <bool WET, int BS>
+static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
+    hipLaunchKernelGGL((k_jac2d_list<WET, BS>), dim3(std::min(grid, nblocks_for(18 * c->E, BS))), dim3(BS), 0,
+                       c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM);
+}
+template <bool WET, int BS>
+static void res_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
+    hipLaunchKernelGGL((k_resid2d_list<WET, BS>), dim3(std::min(grid, nblocks_for(c->E, BS))), dim3(BS), 0, c->stream,
+                       D, list, count, c->ER);
+}
+
+// phase: 0 Jacobian wet list, 1 Jacobian general list, 2 residual wet list, 3 residual general list
+void launch_list(Ctx *c, const ElemData &D, int phase, int bs, int grid) {
+    const bool wet = phase == 0 || phase == 2;
+    const int *list = wet ? c->elist_wet : c->elist_gen;
+    const int *count = wet ? c->ecount : c->ecount + 1;
+#define SW2_LIST_CASE(B)                                                                      \
+    case B:                                                                                    \
+        if (phase == 0) jac_list_bs<true, B>(c, D, list, count, grid);                         \
+        else if (phase == 1) jac_list_bs<false, B>(c, D, list, count, grid);                   \
+        else if (phase == 2) res_list_bs<true, B>(c, D, list, count, grid);                    \
+        else res_list_bs<false, B>(c, D, list, count, grid);                                   \
+        break;
+    switch (bs) {
+        SW2_LIST_CASE(64)
+        SW2_LIST_CASE(256)
+        default:
+        SW2_LIST_CASE(128)
+    }
+#undef SW2_LIST_CASE
+}
+
+void ensure_lists(Ctx *c) {
+    if (!c->pre_valid) launch_old_terms(c);
+    if (!c->split_valid) launch_split(c);
+}
+
+// one list kernel with the element data its phase uses (autotuning)
+void launch_list_phase(Ctx *c, int phase, int bs, int grid) {
+    ElemData D = c->ed;
+    D.first_elem = phase >= 2 ? c->first_elem : nullptr;
+    launch_list(c, D, phase, bs, grid);
+}
 
 static void launch_jac2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
     if (!c->split_valid) launch_split(c);
-    hipLaunchKernelGGL(k_jac2d_list<true>, dim3(std::min(JAC_WET_BLOCKS, nblocks_for(18 * c->E, 128))), dim3(128), 0,
-                       c->stream, D, c->elist_wet, c->ecount, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos,
-                       c->EM);
-    hipLaunchKernelGGL(k_jac2d_list<false>, dim3(std::min(JAC_GEN_BLOCKS, nblocks_for(18 * c->E, 128))), dim3(128), 0,
-                       c->stream, D, c->elist_gen, c->ecount + 1, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos,
-                       c->EM);
+    launch_list(c, D, 0, c->tune.bs[0], c->tune.grid[0]);
+    launch_list(c, D, 1, c->tune.bs[1], c->tune.grid[1]);
 }
 
 static void launch_resid2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
     if (!c->split_valid) launch_split(c);
-    hipLaunchKernelGGL(k_resid2d_list<true>, dim3(std::min(RES_WET_BLOCKS, nblocks_for(c->E, 128))), dim3(128), 0,
-                       c->stream, D, c->elist_wet, c->ecount, c->ER);
-    hipLaunchKernelGGL(k_resid2d_list<false>, dim3(std::min(RES_GEN_BLOCKS, nblocks_for(c->E, 128))), dim3(128), 0,
-                       c->stream, D, c->elist_gen, c->ecount + 1, c->ER);
+    launch_list(c, D, 2, c->tune.bs[2], c->tune.grid[2]);
+    launch_list(c, D, 3, c->tune.bs[3], c->tune.grid[3]);
 }
 
 void launch_jacobian(Ctx *c) {

@@ -59,6 +59,16 @@ class Report(C.Structure):
                 ("ms_total", C.c_double)]
 
 
+class Tuning(C.Structure):
+    """adh_sw2_tuning: launch shapes of the element list kernels
+    (0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general)."""
+    _fields_ = [("block", C.c_int * 4), ("grid", C.c_int * 4), ("ms", C.c_double * 4)]
+
+    def as_dict(self):
+        names = ("jac_wet", "jac_gen", "res_wet", "res_gen")
+        return {n: dict(block=self.block[k], grid=self.grid[k], ms=round(self.ms[k], 4)) for k, n in enumerate(names)}
+
+
 class SW2Error(RuntimeError):
     pass
 
@@ -103,6 +113,9 @@ def lib():
         L.adh_sw2_hip_batch_timestep.argtypes = [C.c_void_p, PI, C.POINTER(Report), PI, C.c_int]
         L.adh_sw2_hip_batch_get_dt.argtypes = [C.c_void_p, C.c_int, PD]
         L.adh_sw2_hip_batch_set_tol.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.adh_sw2_hip_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.adh_sw2_hip_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.adh_sw2_hip_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(Tuning)]
         L.adh_sw2_hip_batch_residual_norm.argtypes = [C.c_void_p, PD]
         L.adh_sw2_hip_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, PD]
         L.adh_sw2_hip_get_system.argtypes = [C.c_void_p, PD, PD, PD]
@@ -166,7 +179,26 @@ def _mesh_struct(p, keep):
     return m
 
 
-class SW2Model:
+class _Tunable:
+    """Launch-shape autotuning shared by single models and ensembles."""
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times the candidate launch shapes of the element list kernels on
+        the current state, keeps the fastest, returns them."""
+        t = Tuning()
+        _check(self.L.adh_sw2_hip_autotune(self.h, reps, C.byref(t)))
+        return t.as_dict()
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(self.L.adh_sw2_hip_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: Tuning):
+        _check(self.L.adh_sw2_hip_set_tuning(self.h, C.byref(t)))
+
+
+class SW2Model(_Tunable):
     """One SW2 model resident on the GPU (see module docstring)."""
 
     def __init__(self, problem, opts: Opts | None = None):
@@ -315,7 +347,7 @@ class SW2Model:
         return self._rdict(r, vd.value)
 
 
-class SW2Ensemble:
+class SW2Ensemble(_Tunable):
     """Independent SW2 models advanced together on one GPU: one device mesh
     holds them side by side, so every phase is one launch over all models,
     while Newton, Krylov and time-step decisions stay per model (each model

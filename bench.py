@@ -30,7 +30,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(p, precision, steps, warmup, rtol, graphs=1):
+def run_form(p, precision, steps, warmup, rtol, graphs=1, autotune=True):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts, YES
     g = SW2Model(p, default_opts(precision=precision, graphs=graphs))
     # tolerance: 1e-8 of the initial residual of the first step (the reference's NTL is absolute)
@@ -41,6 +41,7 @@ def run_form(p, precision, steps, warmup, rtol, graphs=1):
     g.set_opts(default_opts(precision=precision, tol_nonlin=rtol * r0, max_nonlin_it=40, graphs=graphs))
     for i in range(warmup):
         g.timestep()
+    tuning = g.autotune(reps=3) if autotune else None
     newton_its = 0
     lin_its = 0
     phases = dict(load=0.0, setup=0.0, solve=0.0, resid=0.0)
@@ -58,7 +59,7 @@ def run_form(p, precision, steps, warmup, rtol, graphs=1):
     elapsed = time.perf_counter() - t0
     h, vel, *_ = g.get_state()
     return g, dict(newton_its=newton_its, lin_its=lin_its, seconds=elapsed, r0=r0,
-                   newton_per_s=newton_its / elapsed, phases_ms=phases, head=h, vel=vel)
+                   newton_per_s=newton_its / elapsed, phases_ms=phases, head=h, vel=vel, tuning=tuning)
 
 
 def measured_traffic(precision):
@@ -139,6 +140,7 @@ def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2):
         ens.set_tol(i, rtol * r0[i])
     for _ in range(warmup):
         ens.timestep(max_retries=max_retries)
+    tuning = ens.autotune(reps=2)
     its = np.zeros(len(ps), dtype=np.int64)
     lin = np.zeros(len(ps), dtype=np.int64)
     fails = 0
@@ -159,7 +161,7 @@ def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2):
     ens.close()
     return dict(seconds=elapsed, build_s=t_build, newton_its=int(its.sum()), lin_its=int(lin.sum()), fails=fails,
                 inst_newton_per_s=float(its.sum() / elapsed), tri_newton_per_s=float((its * tris).sum() / elapsed),
-                states=states, dts=dts)
+                states=states, dts=dts, tuning=tuning)
 
 
 def run_sequential(ps, precision, steps, rtol, max_retries=2):
@@ -238,7 +240,8 @@ def ensemble_mode(args):
                                instance_newton_per_s=round(v["inst_newton_per_s"], 1),
                                equiv_1M_tri_newton_per_s=round(v["tri_newton_per_s"] / 1e6, 2),
                                newton_its=v["newton_its"], lin_its=v["lin_its"], failed_steps=v["fails"],
-                               seconds=round(v["seconds"], 2), build_s=round(v["build_s"], 1))
+                               seconds=round(v["seconds"], 2), build_s=round(v["build_s"], 1),
+                               list_kernel_shapes=v["tuning"])
                   for k, v in res.items()},
         "fp32_refined_max_rel_error_vs_fp64": err,
         "vs_sequential_single_model": seq,
@@ -332,6 +335,7 @@ def main():
         "fp64": {"value": round(r64["newton_per_s"], 3), "newton_its": r64["newton_its"],
                  "lin_its": r64["lin_its"], "phases_ms": r64["phases_ms"], "roofline": roof[0]},
         "fp32_refined": {"newton_its": r32["newton_its"], "lin_its": r32["lin_its"], "phases_ms": r32["phases_ms"],
+                         "list_kernel_shapes": r32["tuning"],
                          "max_rel_error_vs_fp64": err,
                          "form": "FP32 Jacobian storage, preconditioner and SpMV; FP64 Krylov vectors, residual, update"},
         "fp32_all": {"value": round(res[2]["newton_per_s"], 3), "newton_its": res[2]["newton_its"],

@@ -219,6 +219,25 @@ int adh_sw2_hip_get_system(adh_sw2_hip *h, double *diagonal, double *offdiag, do
  * Krylov work vectors; no counterpart in the reference (measurement hook). */
 int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms);
 
+/* Launch shapes of the four element list kernels (FD Jacobian over the wet
+ * and the general triangle lists, residual over the same lists): block size
+ * (64, 128 or 256 threads) and grid size of the grid-stride launch.  They
+ * change the schedule only, never a result.  The *_ms fields are outputs of
+ * adh_sw2_hip_autotune (average launch time of the chosen shape) and are
+ * ignored by adh_sw2_hip_set_tuning.  No counterpart in the reference. */
+typedef struct {
+    int block[4];      /* 0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general */
+    int grid[4];
+    double ms[4];
+} adh_sw2_tuning;
+int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out);
+int adh_sw2_hip_set_tuning(adh_sw2_hip *h, const adh_sw2_tuning *t);
+/* Times every candidate shape of each list kernel on the handle's current
+ * state and triangle lists (HIP events on the library's stream, one warm-up
+ * and `reps` timed launches per shape), keeps the fastest and returns it.
+ * For an ensemble the lists cover every model.  Leaves the state unchanged. */
+int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *out);
+
 /* Correctly rounded log used by the friction law, evaluated on the host. */
 double adh_sw2_hip_cr_log(double x);
 /* Evaluates the device's log (cr_log) and sqrt on n inputs -- a check that

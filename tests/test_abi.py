@@ -98,3 +98,11 @@ def test_ensemble_arguments_are_checked_before_the_device():
     assert b"share gravity" in L.adh_sw2_hip_last_error()
     assert L.adh_sw2_hip_batch_size(None) == -1
     assert L.adh_sw2_hip_batch_timestep(None, None, None, None, 0) == -1
+
+
+def test_tuning_arguments_are_checked():
+    L = sw2.lib()
+    t = sw2.Tuning()
+    assert L.adh_sw2_hip_set_tuning(None, C.byref(t)) == -1
+    assert L.adh_sw2_hip_get_tuning(None, C.byref(t)) == -1
+    assert L.adh_sw2_hip_autotune(None, 3, None) == -1

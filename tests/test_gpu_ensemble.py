@@ -126,3 +126,25 @@ def test_ensemble_subset_steps_and_dt_readback():
         h, v = ens.get_state(i)
         assert np.array_equal(h, single[i][0]) and np.array_equal(v, single[i][1]), i
     ens.close()
+
+
+def test_ensemble_graph_replays_and_autotuned_shapes_match_eager():
+    """Krylov chunks replayed from captured HIP graphs, and autotuned element
+    launch shapes, give the ensemble the eager, default-shape results bit for bit."""
+    ps, tol = _family("channel")
+    out = []
+    for graphs, tune in ((0, False), (1, True)):
+        ens = SW2Ensemble(ps, default_opts(precision=1, tol_nonlin=tol, max_nonlin_it=30, graphs=graphs))
+        if tune:
+            say(f"  ensemble autotuned: {ens.autotune(reps=2)}")
+        reps = [ens.timestep(max_retries=RETRIES) for _ in range(2)]
+        out.append((reps, [ens.get_state(i) for i in range(len(ps))]))
+        ens.close()
+    (ra, sa), (rb, sb) = out
+    for s in range(2):
+        for i in range(len(ps)):
+            assert ra[s][i]["iterations"] == rb[s][i]["iterations"]
+            assert ra[s][i]["lin_its"] == rb[s][i]["lin_its"]
+            assert np.array_equal(ra[s][i]["hist_max"], rb[s][i]["hist_max"], equal_nan=True)
+    for (h1, v1), (h2, v2) in zip(sa, sb):
+        assert np.array_equal(h1, h2) and np.array_equal(v1, v2)

@@ -246,3 +246,35 @@ def test_newton_fp32_refined():
     assert r32["hist_max"][-1] <= 1e-8 * r0
     s64, s32 = g64.get_state(), g32.get_state()
     assert np.abs(s64[0] - s32[0]).max() <= 1e-7 * np.abs(s64[0]).max()
+
+
+@pytest.mark.parametrize("name", ["floodplain-20x10", "bumpy-60x30"])
+def test_autotuned_launch_shapes_change_nothing(name):
+    """The autotuner picks block and grid sizes of the element list kernels;
+    a Newton solve with the picked shapes, and with deliberately odd ones,
+    equals the default-shape solve bit for bit."""
+    from adhv8_kraken_amd.sw2 import Tuning
+    p = CASES[name]
+    tol = 1e-5 if name.startswith("flood") else 5e-8
+    runs = []
+    for mode in ("default", "auto", "odd"):
+        g = SW2Model(p, default_opts(tol_nonlin=tol, max_nonlin_it=30))
+        if mode == "auto":
+            chosen = g.autotune(reps=2)
+            say(f"  {name} autotuned: {chosen}")
+            for v in chosen.values():
+                assert v["block"] in (64, 128, 256) and v["grid"] >= 1
+        elif mode == "odd":
+            t = Tuning()
+            for k in range(4):
+                t.block[k] = (64, 256, 64, 256)[k]
+                t.grid[k] = (3, 1, 7, 2)[k]
+            g.set_tuning(t)
+        r = g.timestep(max_retries=2)
+        h, v, *_ = g.get_state()
+        runs.append((r, h, v))
+        g.close()
+    for r, h, v in runs[1:]:
+        assert r["iterations"] == runs[0][0]["iterations"]
+        assert np.array_equal(r["hist_max"], runs[0][0]["hist_max"])
+        assert bits_equal(h, runs[0][1]) and bits_equal(v, runs[0][2])
