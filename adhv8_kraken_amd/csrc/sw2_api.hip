@@ -164,6 +164,7 @@ static void mesh_geometry(const adh_sw2_mesh *m, std::vector<double> &djac, std:
 static int validate(const adh_sw2_mesh *m) {
     if (!m || m->nnodes <= 0 || m->nelems2d <= 0 || m->nmat <= 0 || m->nelems1d < 0)
         return fail(ADH_SW2_EINVAL, "mesh sizes must be positive");
+    if (m->nelems2d > MAX_TRIANGLES) return fail(ADH_SW2_EINVAL, "more than 115,000,000 triangles per device");
     if (!m->x || !m->y || !m->z || !m->e2d || !m->e2d_mat || !m->e2d_ks || !m->mat_turb || !m->mat_ev ||
         !m->mat_cor_speed)
         return fail(ADH_SW2_EINVAL, "missing required mesh array");
@@ -1211,7 +1212,7 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
         if (i + 1 < n) nn = (nn + 255) / 256 * 256;
         ne += meshes[i].nelems2d; nb += meshes[i].nelems1d; nm += meshes[i].nmat;
     }
-    if (nn > 2000000000LL || ne > 200000000LL) return fail(ADH_SW2_EINVAL, "ensemble too large");
+    if (nn > 2000000000LL || ne > MAX_TRIANGLES) return fail(ADH_SW2_EINVAL, "ensemble too large");
     const int N = (int)nn, E = (int)ne, B = (int)nb, M = (int)nm;
     std::vector<double> x(N, 0.0), y(N, 0.0), z(N, 0.0), bcval(3 * (size_t)N, 0.0);
     std::vector<int> node_bc(N, 0), e2d(3 * (size_t)E), emat(E), esf(E), e1d(2 * (size_t)B), eadj(B), ebc(B);
@@ -1453,6 +1454,7 @@ extern "C" int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out) {
     for (int k = 0; k < 4; k++) {
         out->block[k] = h->c.tune.bs[k];
         out->grid[k] = h->c.tune.grid[k];
+        out->waves[k] = h->c.tune.occ[k];
         out->ms[k] = 0.0;
     }
     return 0;
@@ -1464,10 +1466,13 @@ extern "C" int adh_sw2_hip_set_tuning(adh_sw2_hip *h, const adh_sw2_tuning *t) {
         if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
             return fail(ADH_SW2_EINVAL, "list kernel block size must be 64, 128 or 256");
         if (t->grid[k] < 1 || t->grid[k] > (1 << 20)) return fail(ADH_SW2_EINVAL, "list kernel grid out of range");
+        if (t->waves[k] != 0 && !(k == 0 && (t->waves[k] == 3 || t->waves[k] == 4)))
+            return fail(ADH_SW2_EINVAL, "waves must be 0, or 3 or 4 for the Jacobian wet list");
     }
     for (int k = 0; k < 4; k++) {
         h->c.tune.bs[k] = t->block[k];
         h->c.tune.grid[k] = t->grid[k];
+        h->c.tune.occ[k] = t->waves[k];
     }
     return 0;
 }
@@ -1493,23 +1498,27 @@ extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *ou
         const int items = (phase < 2 ? 18 : 1) * cnt[phase & 1];
         if (items == 0) continue;
         double tbest = 1e30;
-        for (int b : BS) {
-            const int full = nblocks_for(items, b);
-            for (int gi = 0; gi < 8; gi++) {
-                const int g = std::min(GRID[gi], full);
-                if (gi > 0 && std::min(GRID[gi - 1], full) == g) continue;   // same shape as the previous one
-                launch_list_phase(&c, phase, b, g);
-                (void)hipEventRecord(c.ev[0], c.stream);
-                for (int r = 0; r < reps; r++) launch_list_phase(&c, phase, b, g);
-                (void)hipEventRecord(c.ev[1], c.stream);
-                HIPCHK(hipEventSynchronize(c.ev[1]));
-                HIPCHK(hipGetLastError());
-                const double ms = (double)elapsed(c, 0, 1) / reps;
-                if (ms < tbest) {
-                    tbest = ms;
-                    best.block[phase] = b;
-                    best.grid[phase] = g;
-                    best.ms[phase] = ms;
+        for (int occ : {0, 3, 4}) {
+            if (occ && phase != 0) continue;
+            for (int b : BS) {
+                const int full = nblocks_for(items, b);
+                for (int gi = 0; gi < 8; gi++) {
+                    const int g = std::min(GRID[gi], full);
+                    if (gi > 0 && std::min(GRID[gi - 1], full) == g) continue;   // same shape as the previous one
+                    launch_list_phase(&c, phase, b, g, occ);
+                    (void)hipEventRecord(c.ev[0], c.stream);
+                    for (int r = 0; r < reps; r++) launch_list_phase(&c, phase, b, g, occ);
+                    (void)hipEventRecord(c.ev[1], c.stream);
+                    HIPCHK(hipEventSynchronize(c.ev[1]));
+                    HIPCHK(hipGetLastError());
+                    const double ms = (double)elapsed(c, 0, 1) / reps;
+                    if (ms < tbest) {
+                        tbest = ms;
+                        best.block[phase] = b;
+                        best.grid[phase] = g;
+                        best.waves[phase] = occ;
+                        best.ms[phase] = ms;
+                    }
                 }
             }
         }
@@ -1517,6 +1526,7 @@ extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *ou
     for (int k = 0; k < 4; k++) {
         c.tune.bs[k] = best.block[k];
         c.tune.grid[k] = best.grid[k];
+        c.tune.occ[k] = best.waves[k];
     }
     if (out) *out = best;
     return sync_check(c);

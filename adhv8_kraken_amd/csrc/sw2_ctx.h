@@ -12,6 +12,8 @@
 
 namespace sw2 {
 
+// 18 * triangles (the FD Jacobian's lane count) plus two grid strides must fit an int
+constexpr long long MAX_TRIANGLES = 115000000LL;
 constexpr int KUNI_MAX = 10;                              // longest uniform slice of the unrolled SpMV
 constexpr int KGRAPH_MAX = 32;                            // longest captured Krylov chunk
 constexpr int BAND_MAX_DOF = 8190;                        // LDS of one block solve (64 KB)
@@ -91,6 +93,7 @@ struct Ctx {
     struct Tuning {
         int bs[4] = {128, 128, 128, 128};
         int grid[4] = {4096, 1024, 2048, 512};
+        int occ[4] = {0, 0, 0, 0};   // minimum waves per SIMD asked of the compiler (0: its choice)
     } tune;
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state
@@ -188,7 +191,7 @@ void launch_jac2d_only(Ctx *c);
 void launch_resid2d_only(Ctx *c);
 void launch_assemble_only(Ctx *c);
 void ensure_lists(Ctx *c);
-void launch_list_phase(Ctx *c, int phase, int bs, int grid);
+void launch_list_phase(Ctx *c, int phase, int bs, int grid, int occ);
 std::string band_structure(Ctx *c, int nblock);
 void band_free(Ctx *c);
 void launch_band_setup(Ctx *c);

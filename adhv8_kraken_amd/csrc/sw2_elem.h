@@ -538,9 +538,11 @@ HD void elem_old_terms(const ElemData &D, int ie, double *pre) {
 // state (:248) feeds only debug output and is not formed.
 template <bool WET = false>
 HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, int pnode, int pvar, int psign,
-                   bool dry_rule, d3 *elem_rhs) {
+                   bool dry_rule, d3 *elem_rhs, const int *ndp = nullptr) {
     int PRESSURE_FLAG = 1;
-    const int nd[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
+    // ndp: the triangle's node ids when the caller already holds them
+    const int nd[3] = {ndp ? ndp[0] : D.e2d[3 * ie], ndp ? ndp[1] : D.e2d[3 * ie + 1],
+                       ndp ? ndp[2] : D.e2d[3 * ie + 2]};
     v2 gs[3];
     for (int i = 0; i < 3; i++) {
         gs[i].x = D.grad[6 * ie + 2 * i];

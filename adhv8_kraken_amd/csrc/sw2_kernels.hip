@@ -105,23 +105,41 @@ void launch_resid(Ctx *c) {
 // each lane carries one residual evaluation; the pair forms the central
 // difference with a lane exchange.  WET lists hold triangles whose depths stay
 // positive under every perturbation and run without the wet/dry clipping code.
-template <bool WET, int BS>
-__global__ void __launch_bounds__(BS) k_jac2d_list(ElemData D, const int *list, const int *count, int b0, int b1,
-                                                   int b2, const int *blkpos, double *EM) {
+// OCC > 0 asks the compiler for at least OCC waves per SIMD (fewer registers,
+// possibly spills); 0 leaves the register budget to the compiler.
+template <bool WET, int BS, int OCC>
+__global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D, const int *list, const int *count,
+                                                                      int b0, int b1, int b2, const int *blkpos,
+                                                                      double *EM) {
     const int nt = 18 * *count;
-    // grid-stride over (triangle, column, sign); the stride is a multiple of
-    // the wavefront, so both lanes of a pair run and leave together
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-    const int li = t / 18;
-    const int ie = list[li];
+    const int stride = gridDim.x * blockDim.x;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // Grid-stride over (triangle, column, sign); the stride is a multiple of
+    // the wavefront, so both lanes of a pair run and leave together.
+    // Software pipeline: the triangle id two iterations ahead and the node ids
+    // one iteration ahead are loaded while the current triangle is computed,
+    // so an iteration waits on one memory round trip (the node data) instead
+    // of three dependent ones (list -> triangle -> nodes).
+    int ie_a = t < nt ? list[t / 18] : 0;
+    int ie_b = t + stride < nt ? list[(t + stride) / 18] : 0;
+    int nd_a[3] = {0, 0, 0};
+    if (t < nt)
+        for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
+    for (; t < nt; t += stride) {
+    const int ie = ie_a;
+    const int nodes[3] = {nd_a[0], nd_a[1], nd_a[2]};
+    ie_a = ie_b;
+    ie_b = t + 2 * stride < nt ? list[(t + 2 * stride) / 18] : 0;
+    if (t + stride < nt)
+        for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
     const int q = (t >> 1) % 9, j = q / 3, var = q % 3;
     const int sgn = (t & 1) ? -1 : +1;
-    const int n = D.e2d[3 * ie + j];
+    const int n = nodes[j];
     const double val = var == 0 ? D.u[n] : (var == 1 ? D.v[n] : D.head[n]);
     const double eps = num_diff_eps(val), eps2 = 2.0 * eps;
     const int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
     d3 r[3];
-    body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, eps, j, pv, sgn, false, r);
+    body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, eps, j, pv, sgn, false, r, nodes);
     double col[9];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -133,7 +151,6 @@ __global__ void __launch_bounds__(BS) k_jac2d_list(ElemData D, const int *list, 
         col[3 * i + 2] = (r[i].c_eq - mc) / eps2;
     }
     if (t & 1) continue;
-    const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
     const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.elem_inst[ie] : nullptr;
     const int lb[3] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1, lbi ? lbi[2] : b2};
     jac_column_db<3>(D, nodes, lb, j, var, col);
@@ -267,10 +284,19 @@ static void launch_split(Ctx *c) {
 // Grid-stride list launches.  Block size and grid size change only the
 // schedule (every triangle's arithmetic and output slot are fixed), so they
 // are free parameters: defaults below, or what adh_sw2_hip_autotune measured.
-template <bool WET, int BS>
-static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
-    hipLaunchKernelGGL((k_jac2d_list<WET, BS>), dim3(std::min(grid, nblocks_for(18 * c->E, BS))), dim3(BS), 0,
+template <bool WET, int BS, int OCC>
+static void jac_list_occ(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
+    hipLaunchKernelGGL((k_jac2d_list<WET, BS, OCC>), dim3(std::min(grid, nblocks_for(18 * c->E, BS))), dim3(BS), 0,
                        c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM);
+}
+// the wet list carries almost all triangles: it also gets register-budget variants
+template <bool WET, int BS>
+static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid, int occ) {
+    if constexpr (WET) {
+        if (occ == 3) return jac_list_occ<WET, BS, 3>(c, D, list, count, grid);
+        if (occ == 4) return jac_list_occ<WET, BS, 4>(c, D, list, count, grid);
+    }
+    jac_list_occ<WET, BS, 0>(c, D, list, count, grid);
 }
 template <bool WET, int BS>
 static void res_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
@@ -279,14 +305,14 @@ static void res_list_bs(Ctx *c, const ElemData &D, const int *list, const int *c
 }
 
 // phase: 0 Jacobian wet list, 1 Jacobian general list, 2 residual wet list, 3 residual general list
-void launch_list(Ctx *c, const ElemData &D, int phase, int bs, int grid) {
+void launch_list(Ctx *c, const ElemData &D, int phase, int bs, int grid, int occ) {
     const bool wet = phase == 0 || phase == 2;
     const int *list = wet ? c->elist_wet : c->elist_gen;
     const int *count = wet ? c->ecount : c->ecount + 1;
 #define SW2_LIST_CASE(B)                                                                      \
     case B:                                                                                    \
-        if (phase == 0) jac_list_bs<true, B>(c, D, list, count, grid);                         \
-        else if (phase == 1) jac_list_bs<false, B>(c, D, list, count, grid);                   \
+        if (phase == 0) jac_list_bs<true, B>(c, D, list, count, grid, occ);                    \
+        else if (phase == 1) jac_list_bs<false, B>(c, D, list, count, grid, occ);              \
         else if (phase == 2) res_list_bs<true, B>(c, D, list, count, grid);                    \
         else res_list_bs<false, B>(c, D, list, count, grid);                                   \
         break;
@@ -305,24 +331,24 @@ void ensure_lists(Ctx *c) {
 }
 
 // one list kernel with the element data its phase uses (autotuning)
-void launch_list_phase(Ctx *c, int phase, int bs, int grid) {
+void launch_list_phase(Ctx *c, int phase, int bs, int grid, int occ) {
     ElemData D = c->ed;
     D.first_elem = phase >= 2 ? c->first_elem : nullptr;
-    launch_list(c, D, phase, bs, grid);
+    launch_list(c, D, phase, bs, grid, occ);
 }
 
 static void launch_jac2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
     if (!c->split_valid) launch_split(c);
-    launch_list(c, D, 0, c->tune.bs[0], c->tune.grid[0]);
-    launch_list(c, D, 1, c->tune.bs[1], c->tune.grid[1]);
+    launch_list(c, D, 0, c->tune.bs[0], c->tune.grid[0], c->tune.occ[0]);
+    launch_list(c, D, 1, c->tune.bs[1], c->tune.grid[1], c->tune.occ[1]);
 }
 
 static void launch_resid2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
     if (!c->split_valid) launch_split(c);
-    launch_list(c, D, 2, c->tune.bs[2], c->tune.grid[2]);
-    launch_list(c, D, 3, c->tune.bs[3], c->tune.grid[3]);
+    launch_list(c, D, 2, c->tune.bs[2], c->tune.grid[2], c->tune.occ[2]);
+    launch_list(c, D, 3, c->tune.bs[3], c->tune.grid[3], c->tune.occ[3]);
 }
 
 void launch_jacobian(Ctx *c) {

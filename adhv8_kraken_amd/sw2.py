@@ -62,11 +62,12 @@ class Report(C.Structure):
 class Tuning(C.Structure):
     """adh_sw2_tuning: launch shapes of the element list kernels
     (0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general)."""
-    _fields_ = [("block", C.c_int * 4), ("grid", C.c_int * 4), ("ms", C.c_double * 4)]
+    _fields_ = [("block", C.c_int * 4), ("grid", C.c_int * 4), ("waves", C.c_int * 4), ("ms", C.c_double * 4)]
 
     def as_dict(self):
         names = ("jac_wet", "jac_gen", "res_wet", "res_gen")
-        return {n: dict(block=self.block[k], grid=self.grid[k], ms=round(self.ms[k], 4)) for k, n in enumerate(names)}
+        return {n: dict(block=self.block[k], grid=self.grid[k], waves=self.waves[k], ms=round(self.ms[k], 4))
+                for k, n in enumerate(names)}
 
 
 class SW2Error(RuntimeError):

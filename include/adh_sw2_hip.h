@@ -228,6 +228,8 @@ int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms)
 typedef struct {
     int block[4];      /* 0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general */
     int grid[4];
+    int waves[4];      /* register budget: 0 = the compiler's, 3 or 4 = at least that many waves
+                          per SIMD (Jacobian wet list only; 0 elsewhere) */
     double ms[4];
 } adh_sw2_tuning;
 int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out);
