@@ -189,6 +189,14 @@ static int validate(const adh_sw2_mesh *m) {
         if (t == TURB_EEV && (!m->mat_eev_mode || m->mat_eev_mode[i] < 1 || m->mat_eev_mode[i] > 3))
             return fail(ADH_SW2_EINVAL, "EEV mode must be 1, 2 or 3");
     }
+    if (m->wind) {
+        for (int i = 0; i < m->nmat; i++) {
+            if (m->mat_wind_flag && m->mat_wind_flag[i] != 0)
+                return fail(ADH_SW2_EUNSUPPORTED, "wind stress law not supported (MP WND STR 0 only)");
+            if (m->mat_windatt && (m->mat_windatt[i] < 0.0 || m->mat_windatt[i] > 1.0))
+                return fail(ADH_SW2_EINVAL, "wind attenuation must lie in [0, 1]");
+        }
+    }
     if (m->node_bc) {
         for (int i = 0; i < m->nnodes; i++) {
             int b = m->node_bc[i];
@@ -400,6 +408,13 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
     UP(mat_eev_coef, eevc.data(), c.M);
     UP(mat_ev, m->mat_ev, 3 * (size_t)c.M);
     UP(mat_cor, m->mat_cor_speed, c.M);
+    if (m->wind) {
+        std::vector<double> watt(c.M, 1.0);
+        if (m->mat_windatt)
+            for (int k = 0; k < c.M; k++) watt[k] = m->mat_windatt[k];
+        UP(mat_windatt, watt.data(), c.M);
+        UP(wind, (const double *)nullptr, 2 * (size_t)c.N);
+    }
     {
         crlog::tables T;
         crlog::make_tables(&T);
@@ -467,6 +482,9 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
     D.g = m->gravity; D.dt = m->dt; D.alpha = m->tau_temporal; D.tau_pg = m->tau_pg;
     D.dll = m->drying_lower_limit; D.dul = m->drying_upper_limit; D.visc = m->viscosity;
     D.logt = c.logt;
+    D.wind = m->wind ? c.wind : nullptr;
+    D.mat_windatt = c.mat_windatt;
+    D.density = m->density;
     HIPCHK(hipDeviceSynchronize());
     *out = h;
     return ADH_SW2_OK;
@@ -505,6 +523,16 @@ extern "C" int adh_sw2_hip_set_dt(adh_sw2_hip *h, double dt) {
     if (!h || !(dt > 0)) return fail(ADH_SW2_EINVAL, "dt must be positive");
     h->c.dt = dt;
     h->c.ed.dt = dt;
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_set_wind(adh_sw2_hip *h, const double *stress_xy) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
+    if (!h || !stress_xy) return fail(ADH_SW2_EINVAL, "null argument");
+    if (!h->c.ed.wind) return fail(ADH_SW2_EINVAL, "the model was created without winds");
+    Ctx &c = h->c;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipMemcpy(c.wind, stress_xy, sizeof(double) * 2 * (size_t)c.N, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1200,8 +1228,9 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
         const adh_sw2_mesh &m = meshes[i];
         if (m.gravity != m0.gravity || m.tau_temporal != m0.tau_temporal || m.tau_pg != m0.tau_pg ||
             m.drying_lower_limit != m0.drying_lower_limit || m.drying_upper_limit != m0.drying_upper_limit ||
-            m.viscosity != m0.viscosity)
-            return fail(ADH_SW2_EINVAL, "ensemble models must share gravity, tau, drying limits and viscosity");
+            m.viscosity != m0.viscosity || m.density != m0.density)
+            return fail(ADH_SW2_EINVAL,
+                        "ensemble models must share gravity, tau, drying limits, viscosity and density");
     }
     // union layout
     std::vector<int> node0(n), E0(n), B0(n), M0(n);
@@ -1220,6 +1249,9 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
     std::vector<int> mturb(M), mmode(M);
     std::vector<double> mcoef(M), mev(3 * (size_t)M), mcor(M);
     std::vector<double> gdj(E), ggr(6 * (size_t)E), gdj1(B), gnr1(2 * (size_t)B);
+    std::vector<double> mwatt(M, 1.0);
+    std::vector<int> mwflag(M, 0);
+    int any_wind = 0;
     for (int i = 0; i < n; i++) {
         const adh_sw2_mesh &m = meshes[i];
         std::vector<double> dj, gr, dj1, nr1;
@@ -1253,7 +1285,10 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
             mcoef[M0[i] + k] = m.mat_eev_coef ? m.mat_eev_coef[k] : 0.0;
             for (int j = 0; j < 3; j++) mev[3 * (size_t)(M0[i] + k) + j] = m.mat_ev[3 * (size_t)k + j];
             mcor[M0[i] + k] = m.mat_cor_speed[k];
+            if (m.wind && m.mat_windatt) mwatt[M0[i] + k] = m.mat_windatt[k];
+            if (m.wind && m.mat_wind_flag) mwflag[M0[i] + k] = m.mat_wind_flag[k];
         }
+        any_wind |= m.wind != 0;
     }
     adh_sw2_mesh u = {};
     u.nnodes = N; u.nelems2d = E; u.nelems1d = B; u.nmat = M;
@@ -1264,6 +1299,10 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
     u.e1d_val = B ? eval.data() : nullptr;
     u.mat_turb = mturb.data(); u.mat_eev_mode = mmode.data(); u.mat_eev_coef = mcoef.data(); u.mat_ev = mev.data();
     u.mat_cor_speed = mcor.data();
+    u.wind = any_wind;
+    u.mat_windatt = mwatt.data();
+    u.mat_wind_flag = mwflag.data();
+    u.density = m0.density;
     u.e2d_djac = gdj.data(); u.e2d_grad = ggr.data();
     u.e1d_djac = B ? gdj1.data() : nullptr; u.e1d_nrml = B ? gnr1.data() : nullptr;
     u.gravity = m0.gravity; u.dt = m0.dt; u.tau_temporal = m0.tau_temporal; u.tau_pg = m0.tau_pg;
@@ -1382,6 +1421,17 @@ extern "C" int adh_sw2_hip_batch_get_dt(adh_sw2_hip *h, int i, double *dt) {
     if ((rc = model_range(h, i, &a, &nn))) return rc;
     if (!dt) return fail(ADH_SW2_EINVAL, "null argument");
     *dt = h->c.h_inst_dt[i];
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_batch_set_wind(adh_sw2_hip *h, int i, const double *stress_xy) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    if (!stress_xy) return fail(ADH_SW2_EINVAL, "null argument");
+    Ctx &c = h->c;
+    if (!c.ed.wind) return fail(ADH_SW2_EINVAL, "the ensemble was created without winds");
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipMemcpy(c.wind + 2 * (size_t)a, stress_xy, sizeof(double) * 2 * (size_t)nn, hipMemcpyHostToDevice));
     return 0;
 }
 

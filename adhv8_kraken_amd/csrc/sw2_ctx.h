@@ -62,6 +62,7 @@ struct Ctx {
     double *djac1d, *nrml1d, *val1d;
     int *mat_turb, *mat_eev_mode;
     double *mat_eev_coef, *mat_ev, *mat_cor;
+    double *wind, *mat_windatt;   // node wind values [2N] and attenuation [M] (models with winds)
     crlog::tables *logt;
     // adjacency
     int *n2e_ptr, *n2e;    // node -> (element*3 + local), ascending element

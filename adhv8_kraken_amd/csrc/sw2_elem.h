@@ -88,6 +88,10 @@ struct ElemData {
     const double *dt_inst;
     const int *elem_inst, *edge_inst;
     const int *loc_bc_inst;    // [3 * models] node-string flags of each model's nodes 0..2
+    // winds: node wind values (x, y per node; null = no wind), attenuation per material, density
+    const double *wind;
+    const double *mat_windatt;
+    double density;
 };
 
 HDF double elem_dt(const ElemData &D, int ie) { return D.dt_inst ? D.dt_inst[D.elem_inst[ie]] : D.dt; }
@@ -745,9 +749,39 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
             elem_rhs[i].y_eq += ryq[i];
         }
     }
-    // wind/wave stress (:674-719) is zero without wind or wave libraries; its
-    // update elem_rhs -= dt * 0.0 is an exact no-op.
-    const v2 total_stress = {0., 0.};
+    // wind and wave stress (:674-719) on fully wet triangles; without winds
+    // (there are never waves here) the update elem_rhs -= dt * 0.0 is an
+    // exact no-op and is skipped
+    v2 total_stress = {0., 0.};
+    if (wd_flag == 0 && D.wind) {
+        // swind_elem2d_get_local_stress (src/structs/smeteor.c:241-308), stress
+        // law 0: the node value is the shear; windatt * windatt is the
+        // reference's pow(windatt, 2.0) for the attenuations decks use (1.0)
+        v2 wind_stress = {0., 0.};
+        for (int i = 0; i < 3; i++) {
+            double shear_x = 0.0, shear_y = 0.0;
+            if (D.density > 0.0) {
+                shear_x = D.wind[2 * nd[i]];
+                shear_y = D.wind[2 * nd[i] + 1];
+            }
+            wind_stress.x += shear_x;
+            wind_stress.y += shear_y;
+        }
+        const double watt = D.mat_windatt[imat];
+        wind_stress.x *= (watt * watt) / (3 * D.density);
+        wind_stress.y *= (watt * watt) / (3 * D.density);
+        total_stress.x = wind_stress.x + 0.0;   // + wave stress
+        total_stress.y = wind_stress.y + 0.0;
+        double sx[3] = {0., 0., 0.}, sy[3] = {0., 0., 0.};
+        t1 = total_stress.x * djac * one_3;     // integrate_triangle_phi (fe_elem2d_integrations.c:159-164)
+        for (int i = 0; i < 3; i++) sx[i] += t1;
+        t1 = total_stress.y * djac * one_3;
+        for (int i = 0; i < 3; i++) sy[i] += t1;
+        for (int i = 0; i < 3; i++) {
+            elem_rhs[i].x_eq -= dt * sx[i];
+            elem_rhs[i].y_eq -= dt * sy[i];
+        }
+    }
 
     // rainfall :728-754
     double src = 0.;

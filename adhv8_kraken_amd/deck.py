@@ -53,6 +53,22 @@ class Series:
         vals = np.interp(pts, t, v)
         return float(np.sum(0.5 * (vals[1:] + vals[:-1]) * np.diff(pts)) / (t1 - t0))
 
+    def value_at(self, t: float, col: int = 0) -> float:
+        """tc_eval_series at t (src/tools/tc_eval_series.c:5-41) over the
+        interval sseries_get_interval picks (src/structs/sseries.c:693-719)."""
+        small = 2.220446049250313e-16
+        ilast = len(self.times) - 1
+        i = 0
+        while i < ilast and t >= self.times[i] * (1.0 - small):
+            i += 1
+        i -= 1
+        if i == ilast:
+            return float(self.values[ilast, col])
+        ltime, rtime = float(self.times[i]), float(self.times[i + 1])
+        lv, rv = float(self.values[i, col]), float(self.values[i + 1, col])
+        theta = (rtime - t) / (rtime - ltime)
+        return theta * lv + (1.0 - theta) * rv
+
     def is_constant(self, col: int = 0) -> bool:
         return bool(np.all(self.values[:, col] == self.values[0, col]))
 
@@ -149,10 +165,11 @@ def read_hot(path, nnodes):
 
 # ------------------------------------------------------------------ .bc
 # card groups the reference's .bc parser dispatches on (src/initio/read_bc.c)
-_CARD_GROUPS = {"OP", "IP", "MP", "FR", "NB", "DB", "EGS", "NDS", "MTS", "SERIES", "TC", "TEST", "SOUT", "END", "CN",
+_CARD_GROUPS = {"OP", "IP", "MP", "FR", "NB", "DB", "EGS", "NDS", "MTS", "SERIES", "TC", "TEST", "SOUT", "FOUT", "END",
+                "CN",
                 "PC", "NDP", "FLX", "FST", "WER", "WRS", "FLP", "FGT", "SLS", "SDR", "BR", "OFF", "ON", "MID"}
 _IGNORED = {("OP", "SW2"), ("OP", "TRN"), ("OP", "INC"), ("MP", "ML"), ("MP", "SRT"), ("SOUT", None),
-            ("SERIES", "AWRITE"), ("SERIES", "OUTPUT")}
+            ("SERIES", "AWRITE"), ("SERIES", "OUTPUT"), ("PC", None), ("FOUT", None)}
 
 
 def _series_card(tok, lines, i):
@@ -182,7 +199,9 @@ def read_deck(stem):
 
     op = dict(nblock=1, prec=1, tau_pg=0.5, tau_temporal=0.0)
     ip = dict(tol_nonlin=None, inc_nonlin=None, max_nonlin_it=None, max_lin_it=500)
-    mp = dict(g=9.8, density=1000.0, viscosity=1.0e-6, muc=1.0, dtl=0.0)
+    mp = dict(g=9.8, density=1000.0, viscosity=1.0e-6, muc=1.0, dtl=0.0, wind=False)
+    wind_flag = {}   # material -> wind stress law (MP WND STR)
+    windatt = {}     # material -> attenuation (MP WND ATT)
     mat_string = {}
     ev = {}
     eev = {}      # material -> (coef, mode)
@@ -259,6 +278,16 @@ def read_deck(stem):
                 mp["g"] = float(tok[2])
             elif c1 == "DTL":
                 mp["dtl"] = max(float(tok[2]), 0.0)
+            elif c1 == "WND":
+                # read_bc_MP.c:241-276: any MP WND card switches winds on
+                mp["wind"] = True
+                c2 = tok[2].upper() if len(tok) > 2 else None
+                if c2 == "STR":
+                    wind_flag[int(tok[3]) - 1] = int(tok[4])
+                elif c2 == "ATT":
+                    windatt[int(tok[3]) - 1] = float(tok[4])
+                else:
+                    raise DeckError(f"MP WND must be followed by STR or ATT: {raw.strip()}")
             else:
                 raise DeckError(f"unsupported card: {raw.strip()}")
         elif c0 == "FR":
@@ -365,6 +394,16 @@ def read_deck(stem):
     if dt_series is None or t0 is None or tf is None:
         raise DeckError("deck needs SERIES DT, TC T0 and TC TF")
 
+    # wind stations (SERIES WIND, read_bc_SERIES.c:101-106): one station
+    # weighs 1 on every node (station_node_contrib.c:26-32)
+    wind_series = [s for s in series.values() if s.kind == "WIND"]
+    if wind_series:
+        mp["wind"] = True
+    if len(wind_series) > 1:
+        raise DeckError("more than one wind station is not supported")
+    if any(f != 0 for f in wind_flag.values()):
+        raise DeckError("only wind stress law 0 (MP WND STR <mat> 0) is supported")
+
     p = _mesh.SW2Problem(
         name=stem.rsplit("/", 1)[-1],
         x=np.ascontiguousarray(xyz[:, 0]), y=np.ascontiguousarray(xyz[:, 1]), z=np.ascontiguousarray(xyz[:, 2]),
@@ -378,6 +417,8 @@ def read_deck(stem):
         g=mp["g"], dt=dt_series.values[0, 0], tau_temporal=op["tau_temporal"], tau_pg=op["tau_pg"],
         drying_lower=mp["dtl"], drying_upper=mp["dtl"], viscosity=mp["viscosity"],
         head0=head, vel0=vel, nsteps=0,
+        wind=int(mp["wind"]), density=mp["density"],
+        mat_windatt=np.array([windatt.get(m, 1.0) for m in range(nmat)], dtype=np.float64),
         meta=dict(old_head=old_head, old_vel=old_vel, older_head=older_head, older_vel=older_vel, deck=stem,
                   latitude=[cor.get(m, 0.0) for m in range(nmat)]),
     )
@@ -391,6 +432,20 @@ def read_deck(stem):
     deck.test_args = test_args
     set_edge_values(deck, t0, t0 + p.dt)
     return deck
+
+
+def wind_values(deck: Deck, t: float) -> np.ndarray:
+    """Node wind values (x, y per node) at time t, as meteor_update sets them
+    from the wind station series (src/meteor/meteor_update.c:30-50,
+    meteor_update_series.c:24-54: the station value times its node weight,
+    here 1, added to 0)."""
+    N = deck.problem.nnodes
+    out = np.zeros(2 * N)
+    for s in (s for s in deck.series.values() if s.kind == "WIND"):
+        if deck.density > 0.0:
+            out[0::2] += s.value_at(t, 0) * 1.0
+            out[1::2] += s.value_at(t, 1) * 1.0
+    return out
 
 
 def set_edge_values(deck: Deck, t0: float, t1: float):

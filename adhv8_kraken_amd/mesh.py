@@ -107,6 +107,9 @@ class SW2Problem:
     drying_lower: float = 0.0
     drying_upper: float = 0.0
     viscosity: float = 1.0e-6
+    wind: int = 0              # winds on (MP WND / SERIES WIND)
+    mat_windatt: np.ndarray = None   # [M] wind attenuation, None = 1.0
+    density: float = 1000.0    # MP RHO
     head0: np.ndarray = None   # [N]
     vel0: np.ndarray = None    # [2N] AoS
     nsteps: int = 1

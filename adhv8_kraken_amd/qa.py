@@ -140,3 +140,108 @@ def error_water_source(p, head, density, djac, time):
     analytic_mass = analytic_depth * total_area * density
     err = abs(grid_mass - analytic_mass)
     return np.array([err, err / grid_mass])
+
+
+SMALL = 2.220446049250313e-16
+
+
+def _dry_wet_factornew(x, y, h, djac):
+    """fe_3m2d_dry_wet_factornew (prototype src/tools/tl_grid_mass.c:5, body
+    src/NEW_ADH/tools/tl_grid_mass.c:442-521): the wet fraction of a
+    triangle cut at the zero-depth line, and the number of dry vertices."""
+    passcode = 0
+    total = 0
+    for i in range(3):
+        d = 1 if h[i] < 0.0 else 0
+        total += d
+        passcode += d << i
+    if total == 3:
+        return 0.0, 3
+    if total == 0:
+        return 1.0, 0
+    if total != 1:
+        passcode = 7 - passcode
+    only = passcode // 2
+    nx, ny = [0.0] * 3, [0.0] * 3
+    for i in range(3):
+        if i != only:
+            xsi = h[i] / (h[i] - h[only])
+            nx[i] = x[i] + xsi * (x[only] - x[i])      # FIXED_POS, macro.h:756-759
+            ny[i] = y[i] + xsi * (y[only] - y[i])
+        else:
+            nx[i], ny[i] = x[i], y[i]
+    elem_djac = djac
+    v0x, v0y = nx[1] - nx[0], ny[1] - ny[0]            # TRI_AREA, macro.h:771-776
+    v1x, v1y = nx[2] - nx[0], ny[2] - ny[0]
+    area = (v0x * v1y - v0y * v1x) / 2.0
+    if area / elem_djac < SMALL:
+        i = total % 2
+        return float(i), (0 if i == 1 else 3)
+    factor = float((total >> 1) - (total % 2))
+    factor *= area / elem_djac
+    return factor + (1.0 if factor < 0 else 0.0), total
+
+
+def grid_mass(p, head, density, djac):
+    """tl_find_grid_mass_elem2d (src/tools/tl_grid_mass.c:20-96): density x
+    the water volume, partially dry triangles integrated over their wet
+    part with the negative depths counted as positive mass."""
+    e = p.e2d.reshape(-1, 3)
+    m = 0.0
+    for ie in range(e.shape[0]):
+        nd = e[ie]
+        eh = [float(head[n]) for n in nd]
+        neg = [0.0, 0.0, 0.0]
+        factor, num_dry = _dry_wet_factornew([float(p.x[n]) for n in nd], [float(p.y[n]) for n in nd], eh,
+                                             float(djac[ie]))
+        if num_dry == 1:
+            for i in range(3):
+                if eh[i] < 0.0:
+                    neg[i] = eh[i]
+            f1, f2 = 1.0, -1.0 * (1.0 - factor)
+        elif num_dry == 2:
+            for i in range(3):
+                if eh[i] < 0.0:
+                    neg[i] = eh[i]
+                    eh[i] = 0.0
+            f1, f2 = factor, 0.0
+        else:
+            f1, f2 = factor, 0.0
+        dj = float(djac[ie])
+        # integrate_triangle_f (fe_elem2d_integrations.c:60-62)
+        m += (f1 * dj * (eh[0] + eh[1] + eh[2]) / 3.0) + (f2 * dj * (neg[0] + neg[1] + neg[2]) / 3.0)
+    return m * density
+
+
+def error_wet_dry(p, head, vel, initial_depth, density, djac, initial_mass):
+    """write_testcase_error_wet_dry (test_cases.c:297-335): mean |u|, mean
+    |v| (the water starts and should end at rest), mean |h - h0|, each
+    relative to its scale (1 for the velocities, max |h0| for the depth),
+    and the grid mass error |mass - mass0| (a closed basin: no boundary or
+    source flux, tl_grid_mass.c:99-150) with its ratio to the initial mass."""
+    N = p.nnodes
+    ex = float(np.sum(np.abs(vel[0::2])))
+    ey = float(np.sum(np.abs(vel[1::2])))
+    eh = float(np.sum(np.abs(head - initial_depth)))
+    max_head = float(np.max(np.abs(initial_depth)))
+    if max_head < 1e-6:
+        max_head = 1.0
+    gm_err = abs(grid_mass(p, head, density, djac) - 0.0 - initial_mass)
+    return np.array([ex / N, ex / N / 1.0, ey / N, ey / N / 1.0, eh / N, eh / N / max_head, gm_err,
+                     gm_err / initial_mass])
+
+
+def error_winds(p, head, wind, node1, node2, g, density):
+    """write_testcase_error_winds_and_waves (test_cases.c:1844-1892), 2D:
+    mean over the transect node1-node2 of |g h_avg dh/ds - |tau|/rho|, the
+    wind setup balance, with |tau| from the node wind values."""
+    ids = slice_nodes_x(p, node1, node2)
+    n = len(ids)
+    err = 0.0
+    for i in range(n - 1):
+        a, b = ids[i + 1], ids[i]
+        slope = math.sqrt(wind[2 * a] ** 2 + wind[2 * a + 1] ** 2)
+        dist = math.sqrt((p.x[a] - p.x[b]) ** 2 + (p.y[a] - p.y[b]) ** 2)
+        dhdx = (head[a] - head[b]) / dist
+        err += abs(g * 0.5 * (head[a] + head[b]) * dhdx - slope / density)
+    return np.array([err / (n - 1), float(n)])

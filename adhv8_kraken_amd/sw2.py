@@ -41,7 +41,8 @@ class Mesh(C.Structure):
                 ("mat_cor_speed", PD),
                 ("gravity", C.c_double), ("dt", C.c_double), ("tau_temporal", C.c_double), ("tau_pg", C.c_double),
                 ("drying_lower_limit", C.c_double), ("drying_upper_limit", C.c_double),
-                ("viscosity", C.c_double)]
+                ("viscosity", C.c_double),
+                ("wind", C.c_int), ("mat_wind_flag", PI), ("mat_windatt", PD), ("density", C.c_double)]
 
 
 class Opts(C.Structure):
@@ -113,6 +114,8 @@ def lib():
         L.adh_sw2_hip_batch_set_dt.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.adh_sw2_hip_batch_timestep.argtypes = [C.c_void_p, PI, C.POINTER(Report), PI, C.c_int]
         L.adh_sw2_hip_batch_get_dt.argtypes = [C.c_void_p, C.c_int, PD]
+        L.adh_sw2_hip_set_wind.argtypes = [C.c_void_p, PD]
+        L.adh_sw2_hip_batch_set_wind.argtypes = [C.c_void_p, C.c_int, PD]
         L.adh_sw2_hip_batch_set_tol.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.adh_sw2_hip_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.adh_sw2_hip_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
@@ -177,6 +180,10 @@ def _mesh_struct(p, keep):
     m.mat_cor_speed = _pd(k(p.mat_cor_speed, np.float64))
     m.gravity, m.dt, m.tau_temporal, m.tau_pg = p.g, p.dt, p.tau_temporal, p.tau_pg
     m.drying_lower_limit, m.drying_upper_limit, m.viscosity = p.drying_lower, p.drying_upper, p.viscosity
+    m.wind = int(getattr(p, "wind", 0) or 0)
+    m.density = float(getattr(p, "density", 1000.0))
+    if m.wind and getattr(p, "mat_windatt", None) is not None:
+        m.mat_windatt = _pd(k(p.mat_windatt, np.float64))
     return m
 
 
@@ -237,6 +244,10 @@ class SW2Model(_Tunable):
 
     def set_dt(self, dt):
         _check(self.L.adh_sw2_hip_set_dt(self.h, dt))
+
+    def set_wind(self, stress_xy):
+        """Node wind values (x, y per node) for the next time step."""
+        _check(self.L.adh_sw2_hip_set_wind(self.h, _pd(np.ascontiguousarray(stress_xy, dtype=np.float64))))
 
     def set_state(self, head, vel, old_head, old_vel, older_head, older_vel):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (head, vel, old_head, old_vel, older_head, older_vel)]
@@ -389,6 +400,9 @@ class SW2Ensemble(_Tunable):
 
     def set_dt(self, i, dt):
         _check(self.L.adh_sw2_hip_batch_set_dt(self.h, i, dt))
+
+    def set_wind(self, i, stress_xy):
+        _check(self.L.adh_sw2_hip_batch_set_wind(self.h, i, _pd(np.ascontiguousarray(stress_xy, dtype=np.float64))))
 
     def get_dt(self, i):
         dt = C.c_double(0.0)

@@ -10,8 +10,11 @@ taken from the DT series, doubled from the last one but capped by the series
 and shortened to land on the final time (src/tools/tc_init.c:4-27,
 src/tools/tc_end.c:5-50).
 
-The model is any object with set_dt(dt), shift() and newton_ok() -> bool;
-`SW2Model` (HIP library) and the CPU oracle wrapper used in tests both fit.
+The model is any object with set_dt(dt), shift() and newton_ok() -> bool,
+and optionally begin_step(t, dt), called before every attempt as the
+reference updates its series averages and wind values there
+(src/main/adh_func.c:538-546); `SW2Model` (HIP library) and the CPU oracle
+wrapper used in tests both fit.
 """
 from __future__ import annotations
 
@@ -51,6 +54,8 @@ def run(model, t0, tf, dt_times, dt_values, on_step=None, max_attempts=30):
         while True:
             attempts += 1
             model.set_dt(dt)
+            if hasattr(model, "begin_step"):
+                model.begin_step(t, dt)
             model.shift()
             ok, info = model.newton_ok()
             if ok:

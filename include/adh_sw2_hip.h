@@ -70,6 +70,15 @@ typedef struct {
     const double *mat_ev;             /* [3M] user eddy viscosity xx, yy, xy */
     const double *mat_cor_speed;      /* [M] get_coriolis_angular_speed(coriolis) */
     double gravity, dt, tau_temporal, tau_pg, drying_lower_limit, drying_upper_limit, viscosity;
+    /* Winds (MP WND STR / ATT, SERIES WIND): surface stress on fully wet
+     * triangles, src/fe/sw2/fe_sw2_body_resid.c:674-719 with
+     * swind_elem2d_get_local_stress (src/structs/smeteor.c:241-308).
+     * wind = 0 switches them off (the node values then need not be set). */
+    int wind;
+    const int *mat_wind_flag;         /* [M] stress law (SMAT_SW.wind_flag): only 0, the node value is the
+                                         stress, is supported; NULL = 0 */
+    const double *mat_windatt;        /* [M] attenuation (SMAT_SW.windatt); NULL = 1.0 */
+    double density;                   /* MP RHO (SMODEL.density) */
 } adh_sw2_mesh;
 
 /* Solver settings.  Defaults (adh_sw2_hip_default_opts) reproduce the
@@ -163,6 +172,12 @@ int adh_sw2_hip_newton(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict);
  * fe_sw2_solve, retried with dt*0.25 on failure, src/main/adh_func.c:535-549). */
 int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int max_retries);
 
+/* Node wind values SWIND.stress, x and y per node ([2N]), as the reference's
+ * meteor_update sets them before each time-step attempt from the wind
+ * station series (src/meteor/meteor_update.c:36-43, called at
+ * src/main/adh_func.c:545).  Only for a model created with wind = 1. */
+int adh_sw2_hip_set_wind(adh_sw2_hip *h, const double *stress_xy);
+
 /* ---- Ensembles: n independent models advanced together -------------------
  * No counterpart in the reference, whose fe_newton runs one model at a time
  * (src/fe/fe_newton.c:214); this is the additive batched entry beside the
@@ -188,6 +203,8 @@ int adh_sw2_hip_batch_get_state(adh_sw2_hip *h, int i, double *head, double *vel
 int adh_sw2_hip_batch_set_dt(adh_sw2_hip *h, int i, double dt);
 /* current dt of model i (after any cuts the last time step made) */
 int adh_sw2_hip_batch_get_dt(adh_sw2_hip *h, int i, double *dt);
+/* node wind values of model i (as adh_sw2_hip_set_wind over its own nodes) */
+int adh_sw2_hip_batch_set_wind(adh_sw2_hip *h, int i, const double *stress_xy);
 /* nonlinear tolerance of model i (the reference's NTL is one absolute value
  * per model); models without one use opts.tol_nonlin */
 int adh_sw2_hip_batch_set_tol(adh_sw2_hip *h, int i, double tol_nonlin);

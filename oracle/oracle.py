@@ -44,7 +44,8 @@ class ODesc(C.Structure):
                 ("mat_turb", PI), ("mat_eev_mode", PI), ("mat_eev_coef", PD), ("mat_ev", PD),
                 ("mat_cor_speed", PD),
                 ("g", C.c_double), ("dt", C.c_double), ("tau_temporal", C.c_double), ("tau_pg", C.c_double),
-                ("drying_lower", C.c_double), ("drying_upper", C.c_double), ("viscosity", C.c_double)]
+                ("drying_lower", C.c_double), ("drying_upper", C.c_double), ("viscosity", C.c_double),
+                ("wind", C.c_int), ("mat_windatt", PD), ("density", C.c_double)]
 
 
 class OOpts(C.Structure):
@@ -72,6 +73,7 @@ def _setup(L):
     L.o_set_state.argtypes = [C.c_void_p] + [PD] * 6
     L.o_get_state.argtypes = [C.c_void_p] + [PD] * 6
     L.o_set_dt.argtypes = [C.c_void_p, C.c_double]
+    L.o_set_wind.argtypes = [C.c_void_p, PD]
     L.o_wdflag.argtypes = [C.c_void_p, PI]
     L.o_init.argtypes = [C.c_void_p, PI]
     L.o_body_resid.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, PD]
@@ -152,6 +154,10 @@ class Oracle:
         d.mat_cor_speed = _pd(k(p.mat_cor_speed, np.float64))
         d.g, d.dt, d.tau_temporal, d.tau_pg = p.g, p.dt, p.tau_temporal, p.tau_pg
         d.drying_lower, d.drying_upper, d.viscosity = p.drying_lower, p.drying_upper, p.viscosity
+        d.wind = int(getattr(p, "wind", 0) or 0)
+        d.density = float(getattr(p, "density", 1000.0))
+        if d.wind and getattr(p, "mat_windatt", None) is not None:
+            d.mat_windatt = _pd(k(p.mat_windatt, np.float64))
         self.desc = d
         self.h = L.o_create(C.byref(d))
         self.L = L
@@ -180,6 +186,9 @@ class Oracle:
 
     def set_dt(self, dt):
         self.L.o_set_dt(self.h, dt)
+
+    def set_wind(self, stress_xy):
+        self.L.o_set_wind(self.h, _pd(np.ascontiguousarray(stress_xy, dtype=np.float64)))
 
     def geometry(self):
         p = self.p

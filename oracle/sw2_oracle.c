@@ -76,6 +76,8 @@ struct o_model {
     int prof_nblock;
     /* block-Jacobi inverses */
     double *bjinv;
+    /* node wind values (x, y per node) */
+    double *wind;
 };
 
 /* ---------------------------------------------------------------------- */
@@ -805,10 +807,40 @@ static void body_resid(o_model *m, dof3 *elem_rhs, int ie, double perturbation, 
         elem_rhs[i].y_eq += rhs_y_eq[i];
     }
 
-    /* :674-719 wind and wave stress: no wind/wave libraries, total stress is
-       zero and its residual contribution (elem_rhs -= dt * 0.0) is an exact
-       no-op, so it is not restated. */
+    /* :674-719 wind and wave stress on fully wet triangles.  Without winds
+       (and there are never waves here) the contribution elem_rhs -= dt * 0.0
+       is an exact no-op and is skipped. */
     v2 total_stress = {0., 0.};
+    if (wd_flag == 0 && d->wind) {
+        /* swind_elem2d_get_local_stress, src/structs/smeteor.c:241-308, stress law 0
+           (the node value is the shear); pow(windatt, 2.0) == windatt * windatt
+           for the attenuations the decks use (1.0) */
+        v2 wind_stress = {0., 0.};
+        for (i = 0; i < 3; i++) {
+            double shear_x = 0.0, shear_y = 0.0;
+            if (d->density > 0.0) {
+                shear_x = m->wind[2 * nd[i]];
+                shear_y = m->wind[2 * nd[i] + 1];
+            }
+            wind_stress.x += shear_x;
+            wind_stress.y += shear_y;
+        }
+        double watt = d->mat_windatt ? d->mat_windatt[imat] : 1.0;
+        wind_stress.x *= (watt * watt) / (3 * d->density);
+        wind_stress.y *= (watt * watt) / (3 * d->density);
+        total_stress.x = wind_stress.x + 0.0;   /* + wave stress */
+        total_stress.y = wind_stress.y + 0.0;
+        for (i = 0; i < 3; i++) { rhs_x_eq[i] = 0.; rhs_y_eq[i] = 0.; }
+        /* integrate_triangle_phi, fe_elem2d_integrations.c:159-164 */
+        t1 = total_stress.x * djac * one_3;
+        for (i = 0; i < 3; i++) rhs_x_eq[i] += t1;
+        t1 = total_stress.y * djac * one_3;
+        for (i = 0; i < 3; i++) rhs_y_eq[i] += t1;
+        for (i = 0; i < 3; i++) {
+            elem_rhs[i].x_eq -= dt * rhs_x_eq[i];
+            elem_rhs[i].y_eq -= dt * rhs_y_eq[i];
+        }
+    }
 
     /* :728-754 rainfall */
     double elem_water_source = 0.;
@@ -1074,6 +1106,7 @@ o_model *o_create(const o_desc *d) {
     m->matrix = xcalloc(N, sizeof(spv));
     m->node_block = xcalloc(N, sizeof(int));
     m->bjinv = xcalloc(9 * N, 8);
+    m->wind = xcalloc(2 * N, 8);
     m->dt = d->dt;
     return m;
 }
@@ -1087,7 +1120,7 @@ void o_destroy(o_model *m) {
     free(m->djac); free(m->grad); free(m->djac1d); free(m->nrml1d);
     free(m->head); free(m->vel); free(m->old_head); free(m->old_vel); free(m->older_head); free(m->older_vel);
     free(m->wd_flag); free(m->bc_mask); free(m->residual); free(m->sol); free(m->diagonal); free(m->scale_vect);
-    free(m->matrix); free(m->node_block); free(m->bjinv);
+    free(m->matrix); free(m->node_block); free(m->bjinv); free(m->wind);
     free(m);
 }
 
@@ -1099,6 +1132,8 @@ void o_get_geometry(const o_model *m, double *djac, double *grad, double *djac1d
 }
 
 void o_set_dt(o_model *m, double dt) { m->dt = dt; }
+
+void o_set_wind(o_model *m, const double *stress_xy) { memcpy(m->wind, stress_xy, 16 * (size_t)m->N); }
 
 void o_set_state(o_model *m, const double *head, const double *vel, const double *old_head, const double *old_vel,
                  const double *older_head, const double *older_vel) {

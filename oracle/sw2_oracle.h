@@ -57,6 +57,10 @@ typedef struct {
     const double *mat_cor_speed;    /* [M] coriolis angular speed 2*Omega*sin(lat) */
     /* scalars */
     double g, dt, tau_temporal, tau_pg, drying_lower, drying_upper, viscosity;
+    /* winds (MP WND, SERIES WIND): wind = 0 switches them off */
+    int wind;
+    const double *mat_windatt;      /* [M] attenuation (MP WND ATT, default 1.0) */
+    double density;                 /* MP RHO */
 } o_desc;
 
 typedef struct {
@@ -96,6 +100,9 @@ void o_set_state(o_model *m, const double *head, const double *vel, const double
 void o_get_state(const o_model *m, double *head, double *vel, double *old_head, double *old_vel,
                  double *older_head, double *older_vel);
 void o_set_dt(o_model *m, double dt);
+/* node wind values SWIND.stress (x, y per node) as meteor_update sets them
+   (src/meteor/meteor_update.c:36-43) */
+void o_set_wind(o_model *m, const double *stress_xy);
 
 void o_wdflag(o_model *m, int *wd_flag_out);
 void o_init(o_model *m, int *bc_mask_out);

@@ -7,7 +7,7 @@ error.out.last, which the reference's test-case writer
 (src/initio/test_cases.c) produced at the end of the run.
 
 Run here, where the reference tree is mounted:
-    python tests/golden/make_qa_fixtures.py [/root/reference]
+    python tests/golden/make_qa_fixtures.py [/root/reference [case ...]]
 """
 import os
 import re
@@ -25,6 +25,8 @@ CASES = {
     "angle_coriolis": "qa/sw2/angle_coriolis/tri/2D-coriolis_X",
     "angle_dam": "qa/sw2/angle_dam/tri/angle_dam",
     "angle_source": "qa/sw2/angle_source/tri/angle_source",
+    "wet_dry": "qa/sw2/wet_dry/tri/wd",
+    "angle_wind": "qa/sw2/angle_wind/tri/angle_wind",
 }
 
 FIELDS = ["x", "y", "z", "e2d", "e2d_mat", "e2d_ks", "e2d_src_flag", "e2d_src", "e1d", "e1d_adj", "e1d_bc",
@@ -39,8 +41,10 @@ def expected_errors(path):
     return np.array([float(v) for v in re.findall(r"[-+]?\d+\.\d+e[-+]\d+", txt)], dtype=np.float64), txt
 
 
-def main(ref):
+def main(ref, only=None):
     for name, rel in CASES.items():
+        if only and name not in only:
+            continue
         stem = os.path.join(ref, rel)
         d = deck.read_deck(stem)
         p = d.problem
@@ -62,10 +66,15 @@ def main(ref):
         out["expected"] = vals
         out["expected_text"] = np.array(txt)
         out["test_case"] = np.array(d.test_case or "")
+        out["wind"] = np.int32(p.wind)
+        out["mat_windatt"] = p.mat_windatt
+        winds = [sr for sr in d.series.values() if sr.kind == "WIND"]
+        out["wind_series"] = (np.concatenate([winds[0].times[:, None], winds[0].values[:, :2]], axis=1)
+                              if winds else np.zeros((0, 3)))
         dst = os.path.join(HERE, f"qa_{name}.npz")
         np.savez_compressed(dst, **out)
         print(dst, os.path.getsize(dst), "bytes")
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference", sys.argv[2:] or None)

@@ -43,6 +43,14 @@ void *he_create(const double *x, const double *y, const double *z, const double 
 
 void he_destroy(void *p) { delete (HostElem *)p; }
 
+// winds: node values (x, y per node), attenuation per material, density
+void he_set_wind(void *p, const double *wind, const double *mat_windatt, double density) {
+    HostElem *h = (HostElem *)p;
+    h->D.wind = wind;
+    h->D.mat_windatt = mat_windatt;
+    h->D.density = density;
+}
+
 void he_body(void *p, int ie, double pert, int pn, int pv, int ps, int dry_rule, double *out) {
     d3 r[3];
     const ElemData &D = ((HostElem *)p)->D;

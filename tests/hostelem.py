@@ -31,6 +31,7 @@ def lib():
         L.he_create.argtypes = [PD] * 12 + [PI, PI, PI, PD, PD, PI, PD, PI, PD, PI, PI, PD, PD, PI, PD, PI, PI, PD,
                                             PD, PD] + [C.c_double] * 7 + [C.c_int]
         L.he_destroy.argtypes = [C.c_void_p]
+        L.he_set_wind.argtypes = [C.c_void_p, PD, PD, C.c_double]
         L.he_body.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
         L.he_body_wet.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, PD]
         L.he_boundary.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
@@ -72,6 +73,10 @@ class HostElem:
         self.h = self.L.he_create(*args, p.g, p.dt, p.tau_temporal, p.tau_pg, p.drying_lower, p.drying_upper,
                                   p.viscosity, int(p.nelems2d))
         self.p = p
+
+    def set_wind(self, wind, mat_windatt, density):
+        self._wind = [_d(wind), _d(mat_windatt)]
+        self.L.he_set_wind(self.h, self._wind[0].ctypes.data_as(PD), self._wind[1].ctypes.data_as(PD), density)
 
     def __del__(self):
         try:

@@ -19,6 +19,8 @@ def load(name):
     kw = {f: z[f] for f in FIELDS}
     kw.update({s: float(z[s]) for s in ["g", "dt", "tau_temporal", "tau_pg", "drying_lower", "drying_upper",
                                         "viscosity"]})
+    if "wind" in z.files:
+        kw.update(wind=int(z["wind"]), mat_windatt=z["mat_windatt"], density=float(z["density"]))
     p = mesh.SW2Problem(name=name, **kw)
     p.meta = {k: z[k] for k in ("old_head", "old_vel", "older_head", "older_vel")}
     tol, itl, nit, mit, prec, nblock = z["newton"]
@@ -26,8 +28,22 @@ def load(name):
                prec=int(prec), nblock=int(nblock), t0=float(z["t0_tf"][0]), tf=float(z["t0_tf"][1]),
                dt_times=z["dt_series"][0], dt_values=z["dt_series"][1], expected=z["expected"],
                test_case=str(z["test_case"]), test_args=z["test_args"], density=float(z["density"]),
-               latitude=z["latitude"])
+               latitude=z["latitude"],
+               wind_series=z["wind_series"] if "wind_series" in z.files else np.zeros((0, 3)))
     return p, ctl
+
+
+def wind_at(p, ctl, t):
+    """Node wind values at t (one station, weight 1): deck.Series.value_at."""
+    from adhv8_kraken_amd.deck import Series
+    ws = ctl["wind_series"]
+    out = np.zeros(2 * p.nnodes)
+    if len(ws):
+        s = Series("WIND", 0, ws[:, 0], ws[:, 1:])
+        if ctl["density"] > 0.0:
+            out[0::2] += s.value_at(t, 0) * 1.0
+            out[1::2] += s.value_at(t, 1) * 1.0
+    return out
 
 
 def _initial_state(p):
@@ -47,6 +63,14 @@ def make_report(name, p, ctl, djac=None):
         return lambda h, v, t, dt: rep(p, h, v, t, dt)
     if name == "angle_source":
         return lambda h, v, t, dt: qa.error_water_source(p, h, ctl["density"], djac, t + dt)
+    if name == "angle_wind":
+        n1, n2 = int(ctl["test_args"][0]) - 1, int(ctl["test_args"][1]) - 1
+        return lambda h, v, t, dt: qa.error_winds(p, h, wind_at(p, ctl, t), n1, n2, p.g, ctl["density"])
+    if name == "wet_dry":
+        # testcase_prep_wet_dry keeps the hot-started depth; the initial mass
+        # is taken from it as the reference does at start-up
+        m0 = qa.grid_mass(p, p.head0, ctl["density"], djac)
+        return lambda h, v, t, dt: qa.error_wet_dry(p, h, v, p.head0, ctl["density"], djac, m0)
     raise KeyError(name)
 
 
@@ -75,6 +99,10 @@ def run_oracle(name, p, ctl):
         def set_dt(self, dt):
             o.set_dt(dt)
 
+        def begin_step(self, t, dt):
+            if p.wind:
+                o.set_wind(wind_at(p, ctl, t))
+
         def shift(self):
             o.shift()
 
@@ -100,6 +128,10 @@ def run_library(name, p, ctl, **opt):
     class Stepper:
         def set_dt(self, dt):
             m.set_dt(dt)
+
+        def begin_step(self, t, dt):
+            if p.wind:
+                m.set_wind(wind_at(p, ctl, t))
 
         def shift(self):
             m.shift()

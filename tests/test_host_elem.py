@@ -9,8 +9,10 @@ from hostelem import HostElem
 from oracle.oracle import Oracle, opts, set_log_mode
 
 
-def _stepped(p, newton_its=2):
+def _stepped(p, newton_its=2, wind=None):
     o = Oracle(p)
+    if wind is not None:
+        o.set_wind(wind)
     o.shift()
     o.init()
     R0 = np.abs(o.resid()).max()
@@ -22,11 +24,13 @@ def _stepped(p, newton_its=2):
     return o
 
 
-def _compare(o, p, elems):
+def _compare(o, p, elems, wind=None):
     st = o.get_state()
     wd = o.wdflag()
     dj, gr, dj1, nr1 = o.geometry()
     H = HostElem(p, st, wd, dj, gr, dj1, nr1)
+    if wind is not None:
+        H.set_wind(wind, p.mat_windatt, p.density)
     bad = []
     for ie in elems:
         if not np.array_equal(o.body_resid(ie).view(np.int64), H.body(ie).view(np.int64)):
@@ -114,3 +118,40 @@ def test_wet_variant_matches_general_code_on_wet_triangles():
                     if not np.array_equal(a.view(np.int64), b.view(np.int64)):
                         bad.append(("pert", ie, pn, pv, ps))
     assert not bad, bad[:10]
+
+
+def _windy(p, seed):
+    """Winds on, with node values varying in space (the stress law 0 takes
+    them as the surface stress) and an attenuation below 1 on material 0."""
+    p.wind = 1
+    p.density = 1000.0
+    p.mat_windatt = np.full(p.nmat, 0.75)
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-2.0, 2.0, 2 * p.nnodes)
+
+
+def test_wind_stress_bit_exact():
+    """Surface wind stress on fully wet triangles (fe_sw2_body_resid.c:674-719,
+    smeteor.c:241-308) and in the SUPG strong residual, wet channel and
+    wet/dry floodplain (partially wet triangles take no wind)."""
+    for p, elems in ((mesh.bumpy_channel(30, 12, seed=9), None), (mesh.floodplain(20, 10, nsteps=1), None)):
+        w = _windy(p, 4)
+        o = _stepped(p, wind=w)
+        bad, wd = _compare(o, p, range(0, p.nelems2d, 2), wind=w)
+        assert not bad, bad[:10]
+
+
+def test_wind_changes_the_residual():
+    """The wind term is live: the same state with and without winds gives
+    different momentum residuals on a wet triangle."""
+    p = mesh.bumpy_channel(12, 6, seed=3)
+    w = _windy(p, 5)
+    o = _stepped(p, wind=w)
+    st = o.get_state()
+    wd = o.wdflag()
+    dj, gr, dj1, nr1 = o.geometry()
+    H0 = HostElem(p, st, wd, dj, gr, dj1, nr1)
+    H1 = HostElem(p, st, wd, dj, gr, dj1, nr1)
+    H1.set_wind(w, p.mat_windatt, p.density)
+    a, b = H0.body(0), H1.body(0)
+    assert not np.array_equal(a.reshape(3, 3)[:, :2], b.reshape(3, 3)[:, :2])
