@@ -94,6 +94,27 @@ def spmv_roofline(g, p, precision):
                 avg_us=round(ms * 1e3, 2), algo_bytes=int(algo))
 
 
+def jacobian_roofline(g, p):
+    """FD Jacobian (wet list kernel): FP64 flops per launch, counted with
+    rocprofv3 FP64 instruction counters on this workload (committed summary),
+    over its live HIP-event launch time, against the FP64 vector peak."""
+    import re
+    ms = g.time_kernel(1, reps=5)
+    flops = None
+    if (p.nnodes, p.nelems2d) == (501501, 1000000):
+        try:
+            txt = open(os.path.join(ROOT, "profiles", "counters_k_jac2d_list_channel1000x500.md")).read()
+            m = re.search(r"= \*\*([0-9.]+) GFLOP\*\* per launch", txt)
+            flops = float(m.group(1)) * 1e9 if m else None
+        except OSError:
+            pass
+    out = dict(bound="fp64-valu", kernel="k_jac2d_list", avg_us=round(ms * 1e3, 1), peak=78.6, unit="TFLOP/s")
+    if flops:
+        ach = flops / (ms * 1e-3) / 1e12
+        out.update(achieved=round(ach, 2), frac=round(ach / 78.6, 4), flops_per_launch=flops)
+    return out
+
+
 def cpu_baseline(seconds_budget=20.0):
     """The oracle (C restatement of the reference's serial path, skyline-LU
     preconditioned BiCGSTAB) on a bounded sample: one time step of the same
@@ -307,6 +328,8 @@ def main():
         res[prec] = r
         roof[prec] = spmv_roofline(g, p, prec)
         roof[prec]["jac_us"] = round(g.time_kernel(1, reps=5) * 1e3, 1)
+        if prec == 1:
+            jroof = jacobian_roofline(g, p)
         g.close()
     cpu = None if args.no_cpu else cpu_baseline()
     r32, r64 = res[1], res[0]
@@ -343,6 +366,7 @@ def main():
                      "max_rel_error_vs_fp64": err32, "roofline": roof[2],
                      "form": "FP32 matrix and Krylov vectors; FP64 residual, update"},
         "roofline": roof[1],
+        "jacobian_roofline": jroof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
