@@ -361,6 +361,24 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
                 slot_src[sfill[slot_of(m->e1d[2 * e + i], m->e1d[2 * e + j])]++] = 9 * E + 4 * e + 2 * i + j;
     rows.clear();
     rows.shrink_to_fit();
+    {   // device slot order: diagonal slots by node, then the off-diagonal slots by
+        // their SELL position, so the assembly's writes run along the sliced layouts
+        // (a slot's contributors keep their order, so every sum is unchanged)
+        std::vector<int> offs(c.nnz);
+        for (int q = 0; q < c.nnz; q++) offs[q] = q;
+        std::sort(offs.begin(), offs.end(), [&](int a, int b) { return c.h_offd_pos[a] < c.h_offd_pos[b]; });
+        std::vector<int> dptr(nslots + 1, 0), dsrc(slot_src.size());
+        c.h_slot_q.assign(c.nnz > 0 ? c.nnz : 1, 0);
+        size_t w = 0;
+        for (int ds = 0; ds < nslots; ds++) {
+            const int ps = ds < N ? ds : N + offs[ds - N];
+            for (int k = slot_ptr[ps]; k < slot_ptr[ps + 1]; k++) dsrc[w++] = slot_src[k];
+            dptr[ds + 1] = (int)w;
+            if (ds >= N) c.h_slot_q[ds - N] = c.h_offd_pos[offs[ds - N]];
+        }
+        slot_ptr.swap(dptr);
+        slot_src.swap(dsrc);
+    }
 
     for (int k = 0; k < 3; k++) c.loc_bc[k] = (m->node_bc && k < N) ? m->node_bc[k] : 0;
 
@@ -428,7 +446,7 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
     UP(col, c.h_col.data(), c.nnz);
     UP(sell_ptr, sell_ptr.data(), c.nslice + 1);
     UP(sell_col, sell_col.data(), c.nsell);
-    UP(offd_pos, c.h_offd_pos.data(), c.nnz);
+    UP(slot_q, c.h_slot_q.data(), c.nnz);
     UP(slot_ptr, slot_ptr.data(), nslots + 1);
     UP(slot_src, slot_src.data(), slot_src.size());
     {   // element block -> its position in the slot-ordered contributor list

@@ -69,7 +69,7 @@ struct Ctx {
     int *n2b_ptr, *n2b;    // node -> (edge*2 + local)
     // pattern + assembly
     int *row_ptr, *col;    // off-diagonal blocks, insertion order
-    int *slot_ptr, *slot_src;  // N diagonal slots then nnz off-diagonal slots
+    int *slot_ptr, *slot_src;  // N diagonal slots (node order) then nnz off-diagonal slots (SELL order)
     int *blkpos;               // element block (9e+3i+j, then 9E+4b+2i+j) -> contributor position
     std::vector<int> h_blkpos;
     std::vector<int> h_row_ptr, h_col;
@@ -83,8 +83,10 @@ struct Ctx {
     int nslice;
     int kuni;              // > 0: every slice holds kuni slots (sell_ptr[s] = s * kuni)
     size_t nsell;          // 64 * sell_ptr[nslice]
-    int *sell_ptr, *sell_col, *offd_pos;  // [nslice+1], [nsell], [nnz] (CSR block -> slot)
-    std::vector<int> h_offd_pos;
+    int *sell_ptr, *sell_col;   // [nslice+1], [nsell]
+    std::vector<int> h_offd_pos;  // CSR off-diagonal block -> SELL position
+    int *slot_q;                  // [nnz] SELL position of each off-diagonal slot (device slot order)
+    std::vector<int> h_slot_q;
     // work
     int *elist_wet, *elist_gen, *ecount;  // FD-Jacobian triangle lists [E], [E], counts [2]
     int *esplit;                          // per-block counts / offsets of the split [E/256 + 1]

@@ -361,45 +361,52 @@ void launch_jacobian(Ctx *c) {
                            c->loc_bc[1], c->blkpos + 9 * (size_t)c->E, c->EM);
 }
 
-// Ordered block assembly: thread per (slot, entry).  The element kernels store
-// each 3x3 block at its contributor position (slot order; within a slot the
+// Ordered block assembly, thread per slot.  The element kernels store each
+// 3x3 block at its contributor position (slot order; within a slot the
 // contributors in ascending element order, 2D before 1D), so a slot's sum
-// reads one contiguous run.
-__global__ void k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_src, const int *offd_pos,
-                           const double *EM, double *diag, double *offd) {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)nslots * 9) return;
-    int s = (int)(t / 9), k = (int)(t % 9);
-    double a = 0.0;
-    for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) a += EM[9 * (size_t)p + k];
-    if (s < N) diag[didx9(s, k)] = a;
-    else offd[sell_vidx((size_t)offd_pos[s - N], k)] = a;
-}
-
-// diagonal floor and residual tweak, fe_sw2_load.c:218-236
-__global__ void k_diag_floor(int N, double *diag, double *R, const double *u, const double *v) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// reads one contiguous run; slots are laid out diagonal-by-node then
+// off-diagonal-by-SELL-position, so consecutive threads write consecutive
+// sliced entries.  Diagonal slots then apply the diagonal floor and residual
+// tweak of fe_sw2_load.c:218-236.
+__global__ void __launch_bounds__(256) k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_q,
+                                                  const double *EM, double *diag, double *offd, double *R,
+                                                  const double *u, const double *v) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = 0.0;
+    for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) {
+        const double *b = EM + 9 * (size_t)p;
+#pragma unroll
+        for (int k = 0; k < 9; k++) a[k] += b[k];
+    }
+    if (s >= N) {
+        const size_t q = (size_t)slot_q[s - N];
+#pragma unroll
+        for (int k = 0; k < 9; k++) offd[sell_vidx(q, k)] = a[k];
+        return;
+    }
     const double f = NOT_QUITE_SMALL;
-    double &d0 = diag[didx9(n, 0)], &d4 = diag[didx9(n, 4)], &d8 = diag[didx9(n, 8)];
-    double &r0 = R[vidx3(n, 0)], &r1 = R[vidx3(n, 1)];
-    if (fabs(d0) < f) {
-        d0 = f;
-        if (fabs(r0) < f) r0 = -f * u[n];
+    if (fabs(a[0]) < f) {
+        a[0] = f;
+        double &r0 = R[vidx3(s, 0)];
+        if (fabs(r0) < f) r0 = -f * u[s];
     }
-    if (fabs(d4) < f) {
-        d4 = f;
-        if (fabs(r1) < f) r1 = -f * v[n];
+    if (fabs(a[4]) < f) {
+        a[4] = f;
+        double &r1 = R[vidx3(s, 1)];
+        if (fabs(r1) < f) r1 = -f * v[s];
     }
-    if (fabs(d8) < f) d8 = f;
+    if (fabs(a[8]) < f) a[8] = f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) diag[didx9(s, k)] = a[k];
 }
 
 void launch_assemble(Ctx *c) {
-    int nslots = c->N + c->nnz;
-    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(9 * nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
-                       c->slot_ptr, c->slot_src, c->offd_pos, c->EM, c->diag, c->offd);
-    hipLaunchKernelGGL(k_diag_floor, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->diag, c->R,
-                       c->u, c->v);
+    const int nslots = c->N + c->nnz;
+    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
+                       c->slot_ptr, c->slot_q, c->EM, c->diag, c->offd, c->R, c->u, c->v);
 }
 
 void launch_jac2d_only(Ctx *c) {
@@ -416,11 +423,7 @@ void launch_resid2d_only(Ctx *c) {
     launch_resid2d_split(c, D);
 }
 
-void launch_assemble_only(Ctx *c) {
-    int nslots = c->N + c->nnz;
-    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(9 * nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
-                       c->slot_ptr, c->slot_src, c->offd_pos, c->EM, c->diag, c->offd);
-}
+void launch_assemble_only(Ctx *c) { launch_assemble(c); }
 
 // ------------------------------------------------------- init / wet-dry flags
 // fe_sw2_init (fe_sw2_init.c:33-79)
