@@ -97,10 +97,28 @@ struct ElemData {
 HDF double elem_dt(const ElemData &D, int ie) { return D.dt_inst ? D.dt_inst[D.elem_inst[ie]] : D.dt; }
 HDF double edge_dt(const ElemData &D, int ie) { return D.dt_inst ? D.dt_inst[D.edge_inst[ie]] : D.dt; }
 
+// Correctly rounded x / b from y = RN(1/b): one multiply and two FMAs
+// instead of a full division.  Markstein's correction step (P. Markstein,
+// "Computation of elementary functions on the IBM RISC System/6000
+// processor", IBM J. Res. Develop. 34(1), 1990, 111-119; J.-M. Muller et al.,
+// Handbook of Floating-Point Arithmetic, 2nd ed., 2018, sec. 4.9): with
+// q = RN(x y) and the exact remainder r = x - q b (one FMA), RN(q + r y) is
+// the correctly rounded quotient for normal quotients; a zero x returns
+// x * y, a zero of x's sign.  Valid where quotients stay normal (no subnormal
+// or overflowing x / b), which holds at the call sites: sums of residual
+// terms and FD differences over 2 eps >= 3e-8.  The host tests compare every
+// use against the oracle's divisions.
+HDF double div_rn(double x, double b, double y) {
+    const double q = x * y;
+    const double r = fma(-q, b, x);
+    return x == 0.0 ? q : fma(r, y, q);
+}
+constexpr double RN_ONE_THIRD = 1.0 / 3.0;   // RN(1/3)
+
 // ---------------------------------------------------------------- integrals
-// fe_elem2d_integrations.c:60-62
+// fe_elem2d_integrations.c:60-62 (c * djac * sum / 3.0, the division exact-rounded)
 HDF double int_tri_f(double djac, double c, const double *f) {
-    return (c * djac * (f[0] + f[1] + f[2]) / 3.0);
+    return div_rn(c * djac * (f[0] + f[1] + f[2]), 3.0, RN_ONE_THIRD);
 }
 // :119-121
 HDF double int_tri_f_f(double djac, double c, const double *f) {
@@ -190,10 +208,10 @@ template <int FN>
 HDF void wd_apply(const v3 *x, const double *h, const v2 *gp, v2 vconst, const v2 *vwd, const double *fwd, double djac,
                   double dt, double c, d3 *rhs) {
     if (FN == WD_INTEGRATE_F) {  // :95-116
-        double t1 = (h[0] + h[1] + h[2]) / 3.0, t2 = 0.0, t3 = 0.0;
+        double t1 = div_rn(h[0] + h[1] + h[2], 3.0, RN_ONE_THIRD), t2 = 0.0, t3 = 0.0;
         if (vwd) {
-            t2 = (vwd[0].x + vwd[1].x + vwd[2].x) / 3.0;
-            t3 = (vwd[0].y + vwd[1].y + vwd[2].y) / 3.0;
+            t2 = div_rn(vwd[0].x + vwd[1].x + vwd[2].x, 3.0, RN_ONE_THIRD);
+            t3 = div_rn(vwd[0].y + vwd[1].y + vwd[2].y, 3.0, RN_ONE_THIRD);
         }
         rhs[0].c_eq += djac * t1;
         rhs[0].x_eq += djac * t2;
@@ -1006,10 +1024,11 @@ HD void jac_column(const ElemData &D, int ie, int j, int var, double *col) {
         boundary_resid(D, ie, eps, j, pv, +1, false, P);
         boundary_resid(D, ie, eps, j, pv, -1, false, M);
     }
+    const double inv = 1.0 / eps2;
     for (int i = 0; i < NN; i++) {
-        col[3 * i + 0] = (P[i].x_eq - M[i].x_eq) / eps2;
-        col[3 * i + 1] = (P[i].y_eq - M[i].y_eq) / eps2;
-        col[3 * i + 2] = (P[i].c_eq - M[i].c_eq) / eps2;
+        col[3 * i + 0] = div_rn(P[i].x_eq - M[i].x_eq, eps2, inv);
+        col[3 * i + 1] = div_rn(P[i].y_eq - M[i].y_eq, eps2, inv);
+        col[3 * i + 2] = div_rn(P[i].c_eq - M[i].c_eq, eps2, inv);
     }
 }
 

@@ -141,14 +141,15 @@ __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D
     d3 r[3];
     body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, eps, j, pv, sgn, false, r, nodes);
     double col[9];
+    const double inv = 1.0 / eps2;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         // even lanes hold P (+eps), odd lanes M (-eps); both lanes form P - M
         double mx = __shfl_xor(r[i].x_eq, 1, 64), my = __shfl_xor(r[i].y_eq, 1, 64),
                mc = __shfl_xor(r[i].c_eq, 1, 64);
-        col[3 * i + 0] = (r[i].x_eq - mx) / eps2;
-        col[3 * i + 1] = (r[i].y_eq - my) / eps2;
-        col[3 * i + 2] = (r[i].c_eq - mc) / eps2;
+        col[3 * i + 0] = div_rn(r[i].x_eq - mx, eps2, inv);
+        col[3 * i + 1] = div_rn(r[i].y_eq - my, eps2, inv);
+        col[3 * i + 2] = div_rn(r[i].c_eq - mc, eps2, inv);
     }
     if (t & 1) continue;
     const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.elem_inst[ie] : nullptr;

@@ -43,6 +43,11 @@ void *he_create(const double *x, const double *y, const double *z, const double 
 
 void he_destroy(void *p) { delete (HostElem *)p; }
 
+// div_rn over n pairs (quotients out), for the division test
+void he_div_rn(int n, const double *x, const double *b, double *q) {
+    for (int i = 0; i < n; i++) q[i] = div_rn(x[i], b[i], 1.0 / b[i]);
+}
+
 // winds: node values (x, y per node), attenuation per material, density
 void he_set_wind(void *p, const double *wind, const double *mat_windatt, double density) {
     HostElem *h = (HostElem *)p;

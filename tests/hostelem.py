@@ -32,6 +32,7 @@ def lib():
                                             PD, PD] + [C.c_double] * 7 + [C.c_int]
         L.he_destroy.argtypes = [C.c_void_p]
         L.he_set_wind.argtypes = [C.c_void_p, PD, PD, C.c_double]
+        L.he_div_rn.argtypes = [C.c_int, PD, PD, PD]
         L.he_body.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
         L.he_body_wet.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, PD]
         L.he_boundary.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, PD]
@@ -40,6 +41,14 @@ def lib():
         L.he_cr_log.restype = C.c_double
         _lib = L
     return _lib
+
+
+def div_rn(x, b):
+    """The element code's reciprocal-corrected division, evaluated on the host."""
+    x, b = _d(x), _d(b)
+    q = np.empty_like(x)
+    lib().he_div_rn(int(x.size), x.ctypes.data_as(PD), b.ctypes.data_as(PD), q.ctypes.data_as(PD))
+    return q
 
 
 def _d(a):
