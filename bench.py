@@ -147,7 +147,7 @@ def cpu_baseline(seconds_budget=20.0):
                        f"{its} Newton its in {dt:.2f}s; rate scaled by 40k/1M triangles")
 
 
-def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2):
+def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2, autotune=True):
     """Every instance of the ensemble advanced `steps` time steps together
     (one launch per phase for all instances); each instance's nonlinear
     tolerance is rtol times its own opening residual."""
@@ -161,7 +161,7 @@ def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2):
         ens.set_tol(i, rtol * r0[i])
     for _ in range(warmup):
         ens.timestep(max_retries=max_retries)
-    tuning = ens.autotune(reps=2)
+    tuning = ens.autotune(reps=2) if autotune else None
     its = np.zeros(len(ps), dtype=np.int64)
     lin = np.zeros(len(ps), dtype=np.int64)
     fails = 0
@@ -224,7 +224,7 @@ def ensemble_mode(args):
     res = {}
     for prec in forms:
         log(f"form precision={prec}")
-        res[prec] = run_ensemble(ps, prec, steps, args.warmup, 1e-8)
+        res[prec] = run_ensemble(ps, prec, steps, args.warmup, 1e-8, autotune=not args.no_autotune)
         log(f"  {res[prec]['tri_newton_per_s'] / 1e6:.1f} M triangle-Newton/s")
     err = None
     if 0 in res and 1 in res:
@@ -295,6 +295,8 @@ def main():
     ap.add_argument("--ncy", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=0, help="1: replay captured HIP graphs of Krylov chunks")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes of the element list kernels (for clean kernel traces)")
     ap.add_argument("--workload", default="channel", choices=["channel", "channel-ensemble", "floodplain-ensemble"],
                     help="channel: the 1M-triangle headline; the ensembles print their own JSON line")
     ap.add_argument("--compare", type=int, default=0,
@@ -324,7 +326,7 @@ def main():
     roof = {}
     for prec in (0, 1, 2):
         log(f"form precision={prec}")
-        g, r = run_form(p, prec, args.steps, args.warmup, 1e-8, args.graphs)
+        g, r = run_form(p, prec, args.steps, args.warmup, 1e-8, args.graphs, autotune=not args.no_autotune)
         res[prec] = r
         roof[prec] = spmv_roofline(g, p, prec)
         roof[prec]["jac_us"] = round(g.time_kernel(1, reps=5) * 1e3, 1)
