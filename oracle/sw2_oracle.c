@@ -12,11 +12,17 @@
  *
  * Scope of the restatement: triangles only (NDONTRI), SW2 physics with
  * EVS/EEV eddy viscosity, bed friction from the equivalent roughness height,
- * Coriolis, rainfall source strings, wet/dry clipping, SUPG, node-string
- * Dirichlet flags, and the 1D boundary types NB VEL, NB OTW and NB OUT.
- * Baroclinic density, lids, wind/wave stress, vorticity, hydraulic
- * structures and quadrilaterals are not represented (their terms are exactly
- * zero for the decks this oracle is used with; see the notes inline).
+ * Coriolis, rainfall source strings, wind surface stress (stress law 0, one
+ * station), wet/dry clipping, SUPG, node-string Dirichlet flags, and the 1D
+ * boundary types NB VEL, NB OTW and NB OUT.  Baroclinic density, lids, wave
+ * stress, vorticity, hydraulic structures and quadrilaterals are not
+ * represented (their terms are exactly zero for the decks this oracle is used
+ * with; see the notes inline).
+ *
+ * The wind term is pinned by the wind setup balance g h dh/ds = |tau|/rho
+ * that the angle_wind deck's transect error measures and that a long run
+ * approaches (tests/test_qa_golden.py); the golden figures of the two wind
+ * decks (angle_wind, wet_dry) are approached but not reproduced to rounding.
  */
 #include "sw2_oracle.h"
 
