@@ -318,6 +318,7 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
         // SpMV then runs a fully unrolled, branch-free slot loop
         int kglob = 0;
         for (int s = 0; s < c.nslice; s++) kglob = std::max(kglob, sell_ptr[s + 1] - sell_ptr[s]);
+        c.kmax_row = kglob;
         c.kuni = 0;
         if (kglob >= 1 && kglob <= KUNI_MAX &&
             (double)kglob * c.nslice <= 1.25 * (double)sell_ptr[c.nslice]) {
@@ -361,23 +362,24 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
                 slot_src[sfill[slot_of(m->e1d[2 * e + i], m->e1d[2 * e + j])]++] = 9 * E + 4 * e + 2 * i + j;
     rows.clear();
     rows.shrink_to_fit();
-    {   // device slot order: diagonal slots by node, then the off-diagonal slots by
-        // their SELL position, so the assembly's writes run along the sliced layouts
-        // (a slot's contributors keep their order, so every sum is unchanged)
-        std::vector<int> offs(c.nnz);
-        for (int q = 0; q < c.nnz; q++) offs[q] = q;
-        std::sort(offs.begin(), offs.end(), [&](int a, int b) { return c.h_offd_pos[a] < c.h_offd_pos[b]; });
-        std::vector<int> dptr(nslots + 1, 0), dsrc(slot_src.size());
-        c.h_slot_q.assign(c.nnz > 0 ? c.nnz : 1, 0);
+    {   // device slot order: the N diagonal slots by node, then one slot per SELL
+        // position (empty for padding), so the assembly's thread for node n finds
+        // its row's blocks at N + q(n, j) and every run of 64 lanes reads and writes
+        // contiguously (a slot's contributors keep their order: sums unchanged)
+        std::vector<int> qinv(c.nsell > 0 ? c.nsell : 1, -1);
+        for (int p = 0; p < c.nnz; p++) qinv[c.h_offd_pos[p]] = p;
+        const size_t ndslots = (size_t)N + c.nsell;
+        std::vector<int> dptr(ndslots + 1, 0), dsrc(slot_src.size());
         size_t w = 0;
-        for (int ds = 0; ds < nslots; ds++) {
-            const int ps = ds < N ? ds : N + offs[ds - N];
-            for (int k = slot_ptr[ps]; k < slot_ptr[ps + 1]; k++) dsrc[w++] = slot_src[k];
+        for (size_t ds = 0; ds < ndslots; ds++) {
+            const int ps = ds < (size_t)N ? (int)ds : (qinv[ds - N] >= 0 ? N + qinv[ds - N] : -1);
+            if (ps >= 0)
+                for (int k = slot_ptr[ps]; k < slot_ptr[ps + 1]; k++) dsrc[w++] = slot_src[k];
             dptr[ds + 1] = (int)w;
-            if (ds >= N) c.h_slot_q[ds - N] = c.h_offd_pos[offs[ds - N]];
         }
         slot_ptr.swap(dptr);
         slot_src.swap(dsrc);
+        c.ndslots = ndslots;
     }
 
     for (int k = 0; k < 3; k++) c.loc_bc[k] = (m->node_bc && k < N) ? m->node_bc[k] : 0;
@@ -446,8 +448,7 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
     UP(col, c.h_col.data(), c.nnz);
     UP(sell_ptr, sell_ptr.data(), c.nslice + 1);
     UP(sell_col, sell_col.data(), c.nsell);
-    UP(slot_q, c.h_slot_q.data(), c.nnz);
-    UP(slot_ptr, slot_ptr.data(), nslots + 1);
+    UP(slot_ptr, slot_ptr.data(), c.ndslots + 1);
     UP(slot_src, slot_src.data(), slot_src.size());
     {   // element block -> its position in the slot-ordered contributor list
         c.h_blkpos.assign(slot_src.size(), 0);
@@ -855,6 +856,7 @@ extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, co
     int rc;
     if ((rc = nodes_from_host(c, c.diag, diagonal, 9))) return rc;
     if (c.nnz && (rc = offd_from_host(c, offdiag))) return rc;
+    c.scale_valid = 0;
     if ((rc = nodes_from_host(c, c.R, rhs, 3))) return rc;
     HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.Np));
     return 0;

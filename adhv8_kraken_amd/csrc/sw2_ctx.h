@@ -12,6 +12,7 @@
 
 namespace sw2 {
 
+constexpr double SOLV_TOL = 2.220446049250313e-16;  // SMALL, src/include/define.h:177 (Jacobi scale floor)
 // 18 * triangles (the FD Jacobian's lane count) plus two grid strides must fit an int
 constexpr long long MAX_TRIANGLES = 115000000LL;
 constexpr int KUNI_MAX = 10;                              // longest uniform slice of the unrolled SpMV
@@ -69,7 +70,8 @@ struct Ctx {
     int *n2b_ptr, *n2b;    // node -> (edge*2 + local)
     // pattern + assembly
     int *row_ptr, *col;    // off-diagonal blocks, insertion order
-    int *slot_ptr, *slot_src;  // N diagonal slots (node order) then nnz off-diagonal slots (SELL order)
+    int *slot_ptr, *slot_src;  // N diagonal slots (node order) then one slot per SELL position
+    int scale_valid;           // the assembly computed `scale` for the current matrix
     int *blkpos;               // element block (9e+3i+j, then 9E+4b+2i+j) -> contributor position
     std::vector<int> h_blkpos;
     std::vector<int> h_row_ptr, h_col;
@@ -82,11 +84,11 @@ struct Ctx {
     // of slot j touches 64 consecutive values.
     int nslice;
     int kuni;              // > 0: every slice holds kuni slots (sell_ptr[s] = s * kuni)
+    int kmax_row;          // longest row (off-diagonal blocks)
     size_t nsell;          // 64 * sell_ptr[nslice]
     int *sell_ptr, *sell_col;   // [nslice+1], [nsell]
     std::vector<int> h_offd_pos;  // CSR off-diagonal block -> SELL position
-    int *slot_q;                  // [nnz] SELL position of each off-diagonal slot (device slot order)
-    std::vector<int> h_slot_q;
+    size_t ndslots;               // N diagonal slots + one slot per SELL position
     // work
     int *elist_wet, *elist_gen, *ecount;  // FD-Jacobian triangle lists [E], [E], counts [2]
     int *esplit;                          // per-block counts / offsets of the split [E/256 + 1]

@@ -362,52 +362,92 @@ void launch_jacobian(Ctx *c) {
                            c->loc_bc[1], c->blkpos + 9 * (size_t)c->E, c->EM);
 }
 
-// Ordered block assembly, thread per slot.  The element kernels store each
-// 3x3 block at its contributor position (slot order; within a slot the
-// contributors in ascending element order, 2D before 1D), so a slot's sum
-// reads one contiguous run; slots are laid out diagonal-by-node then
-// off-diagonal-by-SELL-position, so consecutive threads write consecutive
-// sliced entries.  Diagonal slots then apply the diagonal floor and residual
-// tweak of fe_sw2_load.c:218-236.
-__global__ void __launch_bounds__(256) k_assemble(int nslots, int N, const int *slot_ptr, const int *slot_q,
-                                                  const double *EM, double *diag, double *offd, double *R,
-                                                  const double *u, const double *v) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    double a[9];
+// Ordered block assembly.  The element kernels store each 3x3 block at its
+// contributor position (slot order; within a slot the contributors in
+// ascending element order, 2D before 1D), so a block's sum reads one
+// contiguous run.  Slots are the N diagonal blocks by node, then one per SELL
+// position.  A workgroup takes one 64-row slice: wave 0 the diagonal blocks,
+// wave w >= 1 the row slots j = w - 1 (+ stride), so every wave reads one
+// contiguous stretch of element blocks and writes 64 consecutive sliced
+// entries.  The diagonal block gets the floor and residual tweak of
+// fe_sw2_load.c:218-236; the waves' row maxima, combined in LDS, give the
+// Jacobi scale of solv_linear_sys_setup (the same values as k_scale_rows,
+// which the set-up then skips).
+__device__ __forceinline__ void sum_run(const int *slot_ptr, size_t slot, const double *EM, double (&a)[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k] = 0.0;
-    for (int p = slot_ptr[s]; p < slot_ptr[s + 1]; p++) {
+    for (int p = slot_ptr[slot]; p < slot_ptr[slot + 1]; p++) {
         const double *b = EM + 9 * (size_t)p;
 #pragma unroll
         for (int k = 0; k < 9; k++) a[k] += b[k];
     }
-    if (s >= N) {
-        const size_t q = (size_t)slot_q[s - N];
+}
+
+__global__ void __launch_bounds__(1024) k_assemble(int N, const int *slot_ptr, const int *row_ptr,
+                                                   const int *sell_ptr, const double *EM, double *diag, double *offd,
+                                                   double *R, const double *u, const double *v, double *sv) {
+    __shared__ double smax[16][3][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int n = 64 * blockIdx.x + lane;
+    double s[3] = {0.0, 0.0, 0.0};
+    double a[9];
+    if (n < N) {
+        if (w == 0) {
+            sum_run(slot_ptr, (size_t)n, EM, a);
+            const double f = NOT_QUITE_SMALL;
+            if (fabs(a[0]) < f) {
+                a[0] = f;
+                double &r0 = R[vidx3(n, 0)];
+                if (fabs(r0) < f) r0 = -f * u[n];
+            }
+            if (fabs(a[4]) < f) {
+                a[4] = f;
+                double &r1 = R[vidx3(n, 1)];
+                if (fabs(r1) < f) r1 = -f * v[n];
+            }
+            if (fabs(a[8]) < f) a[8] = f;
 #pragma unroll
-        for (int k = 0; k < 9; k++) offd[sell_vidx(q, k)] = a[k];
-        return;
-    }
-    const double f = NOT_QUITE_SMALL;
-    if (fabs(a[0]) < f) {
-        a[0] = f;
-        double &r0 = R[vidx3(s, 0)];
-        if (fabs(r0) < f) r0 = -f * u[s];
-    }
-    if (fabs(a[4]) < f) {
-        a[4] = f;
-        double &r1 = R[vidx3(s, 1)];
-        if (fabs(r1) < f) r1 = -f * v[s];
-    }
-    if (fabs(a[8]) < f) a[8] = f;
+            for (int k = 0; k < 9; k++) {
+                diag[didx9(n, k)] = a[k];
+                const double x = fabs(a[k]);
+                if (x > s[k / 3]) s[k / 3] = x;
+            }
+        } else {
+            const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + lane;
+            const int len = row_ptr[n + 1] - row_ptr[n];
+            for (int j = w - 1; j < len; j += nw - 1) {
+                const size_t q = q0 + 64 * (size_t)j;
+                sum_run(slot_ptr, (size_t)N + q, EM, a);
 #pragma unroll
-    for (int k = 0; k < 9; k++) diag[didx9(s, k)] = a[k];
+                for (int k = 0; k < 9; k++) {
+                    offd[sell_vidx(q, k)] = a[k];
+                    const double x = fabs(a[k]);
+                    if (x > s[k / 3]) s[k / 3] = x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) smax[w][j][lane] = s[j];
+    __syncthreads();
+    if (w != 0 || n >= N) return;
+    const double small_factor = 1.0 / sqrt(SOLV_TOL);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        double m = smax[0][j][lane];
+        for (int x = 1; x < nw; x++) {
+            const double y = smax[x][j][lane];
+            if (y > m) m = y;
+        }
+        sv[vidx3(n, j)] = m < SOLV_TOL ? small_factor : 1.0 / sqrt(fabs(m));
+    }
 }
 
 void launch_assemble(Ctx *c) {
-    const int nslots = c->N + c->nnz;
-    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(nslots, 256)), dim3(256), 0, c->stream, nslots, c->N,
-                       c->slot_ptr, c->slot_q, c->EM, c->diag, c->offd, c->R, c->u, c->v);
+    const int nw = 1 + std::min(std::max(c->kmax_row, 1), 15);   // waves per 64-row slice
+    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(c->N, 64)), dim3(64 * nw), 0, c->stream, c->N, c->slot_ptr,
+                       c->row_ptr, c->sell_ptr, c->EM, c->diag, c->offd, c->R, c->u, c->v, c->scale);
+    c->scale_valid = 1;
 }
 
 void launch_jac2d_only(Ctx *c) {

@@ -30,7 +30,6 @@ namespace sw2 {
 #define NFIN(c) ((c)->ninst ? (c)->ninst : 1)
 
 
-constexpr double SOLV_TOL = SMALL;  // src/include/define.h:177
 
 // ------------------------------------------------------------- Jacobi scaling
 // spv_jacobi 3-DOF branch (src/solver/spv_jacobi.c:222-316)
@@ -144,8 +143,10 @@ __global__ void k_to_f32(size_t n, const double *a, float *b) {
 
 void launch_setup(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
-    hipLaunchKernelGGL(k_scale_rows, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
-                       c->sell_ptr, c->scale);
+    if (!c->scale_valid)   // a matrix the assembly did not produce: its row maxima first
+        hipLaunchKernelGGL(k_scale_rows, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
+                           c->sell_ptr, c->scale);
+    c->scale_valid = 0;    // the scaling below consumes it (and rescales the FP64 matrix in place)
     if (c->opts.preconditioner == 1) {  // profile LU of the node blocks (FP64)
         hipLaunchKernelGGL((k_scale_apply<false, false>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
                            c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
