@@ -65,11 +65,30 @@ __global__ void k_resid_gather(int N, const int *n2e_ptr, const int *n2e, const 
     if (n >= N) return;
     bool dry = old_head[n] <= 0.;
     double rx = 0.0, ry = 0.0, rc = 0.0;
-    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
-        int s = n2e[k];
-        const double *o = ER + 9 * (size_t)(s / 3) + 3 * (s % 3);
-        if (!dry) { rx -= o[0]; ry -= o[1]; }
-        rc -= o[2];
+    const int k0 = n2e_ptr[n], k1 = n2e_ptr[n + 1];
+    int k = k0;
+    // batches of up to 8 contributions: the indices, then the values, are
+    // loaded before the (in-order) sums, so their latencies overlap
+    for (; k < k1; k += 8) {
+        const int m = min(8, k1 - k);
+        int sidx[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) sidx[t] = t < m ? n2e[k + t] : 0;
+        double vx[8], vy[8], vc[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const double *o = ER + 9 * (size_t)(sidx[t] / 3) + 3 * (sidx[t] % 3);
+            vx[t] = t < m ? o[0] : 0.0;
+            vy[t] = t < m ? o[1] : 0.0;
+            vc[t] = t < m ? o[2] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            if (t < m) {
+                if (!dry) { rx -= vx[t]; ry -= vy[t]; }
+                rc -= vc[t];
+            }
+        }
     }
     for (int k = n2b_ptr[n]; k < n2b_ptr[n + 1]; k++) {
         int s = n2b[k];
