@@ -395,7 +395,20 @@ void launch_jacobian(Ctx *c) {
 __device__ __forceinline__ void sum_run(const int *slot_ptr, size_t slot, const double *EM, double (&a)[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k] = 0.0;
-    for (int p = slot_ptr[slot]; p < slot_ptr[slot + 1]; p++) {
+    const int p0 = slot_ptr[slot], p1 = slot_ptr[slot + 1];
+    // pairs of blocks: both loaded before the in-order sums
+    int p = p0;
+    for (; p + 1 < p1; p += 2) {
+        const double *b0 = EM + 9 * (size_t)p, *b1 = b0 + 9;
+        double x0[9], x1[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) { x0[k] = b0[k]; x1[k] = b1[k]; }
+#pragma unroll
+        for (int k = 0; k < 9; k++) a[k] += x0[k];
+#pragma unroll
+        for (int k = 0; k < 9; k++) a[k] += x1[k];
+    }
+    if (p < p1) {
         const double *b = EM + 9 * (size_t)p;
 #pragma unroll
         for (int k = 0; k < 9; k++) a[k] += b[k];
