@@ -244,26 +244,48 @@ __global__ void __launch_bounds__(256) k_split_count(int E, const int *e2d, cons
     }
 }
 
-// exclusive scans of the two block-count arrays (one block); counts[0] = wet
-// total, counts[1] = general total
+// exclusive scans of the two block-count arrays (one block of 256 threads,
+// each owning a contiguous run of counts); counts[0] = wet total, counts[1] =
+// general total
 __global__ void __launch_bounds__(256) k_split_scan(int nb, int *bcount, int *counts) {
-    __shared__ int tot[256];
+    __shared__ int wsum[2][4];
     const int per = (nb + 255) / 256, b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int a = 0; a < 2; a++) {
         int *bc = bcount + a * nb;
         int s = 0;
-        for (int b = b0; b < b1; b++) s += bc[b];
-        tot[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int t = 0; t < 256; t++) { int v = tot[t]; tot[t] = run; run += v; }
-            counts[a] = run;
+        for (int b = b0; b < b1; b += 8) {  // 8 loads in flight per round trip
+            int v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = b + u < b1 ? bc[b + u] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += v[u];
         }
+        // inclusive scan of the per-thread sums: within the wave, then over the waves
+        int x = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[a][wid] = x;
         __syncthreads();
-        int run = tot[threadIdx.x];
-        for (int b = b0; b < b1; b++) { int v = bc[b]; bc[b] = run; run += v; }
-        __syncthreads();
+        int run = x - s, total = 0;
+        for (int w = 0; w < 4; w++) {
+            if (w < wid) run += wsum[a][w];
+            total += wsum[a][w];
+        }
+        if (threadIdx.x == 0) counts[a] = total;
+        for (int b = b0; b < b1; b += 8) {  // the run's exclusive prefixes, 8 counts per round trip
+            int v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = b + u < b1 ? bc[b + u] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (b + u < b1) {
+                    bc[b + u] = run;
+                    run += v[u];
+                }
+        }
     }
 }
 
@@ -706,10 +728,21 @@ __global__ void __launch_bounds__(256) k_norms_fin(int nparts, int N, const doub
     if (inst_nodes) N = inst_nodes[m];
     out += m;
     double a = 0.0, b = 0.0, cc = 0.0;
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        a = fmax(a, part[3 * k]);
-        b += part[3 * k + 1];
-        cc = fmax(cc, part[3 * k + 2]);
+    // 8 strided partials per thread loaded together, folded in block order
+    const int bd = blockDim.x;
+    for (int k = k0 + threadIdx.x; k < k1; k += 8 * bd) {
+        double x[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) x[u][e] = k + u * bd < k1 ? part[3 * (k + u * bd) + e] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (k + u * bd < k1) {
+                a = fmax(a, x[u][0]);
+                b += x[u][1];
+                cc = fmax(cc, x[u][2]);
+            }
     }
     smax[threadIdx.x] = a; ssum[threadIdx.x] = b; sinc[threadIdx.x] = cc;
     __syncthreads();

@@ -92,7 +92,9 @@ __device__ __forceinline__ void inv3(const double (&d)[9], double (&inv)[9]) {
 // matrix and inverses are stored only as the FP32 copies the FP32 Krylov
 // reads (rounded from the same FP64 values); BJ = false (profile-LU
 // preconditioner): no inverses.
-template <bool F32, bool BJ>
+// K > 0: every slice has K slots (padding slots hold zeros and a valid
+// column), run as a fully unrolled loop with all column scales gathered first.
+template <bool F32, bool BJ, int K>
 __global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double *offd, const int *row_ptr,
                                                      const int *sell_ptr, const int *sell_col, const double *sv,
                                                      double *R, double *sol, double *dinv, float *diag32,
@@ -121,18 +123,36 @@ __global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double
         }
     }
     const size_t q0 = 64 * (size_t)sell_ptr[n >> 6] + (n & 63);
-    const int len = row_ptr[n + 1] - row_ptr[n];
-    for (int b = 0; b < len; b++) {
-        const size_t q = q0 + 64 * (size_t)b;
-        const int cn = sell_col[q];
-        const double sc[3] = {sv[vidx3(cn, 0)], sv[vidx3(cn, 1)], sv[vidx3(cn, 2)]};
+    auto scale_slot = [&](size_t q, const double (&sc)[3]) {
+#pragma unroll
         for (int j = 0; j < 3; j++)
+#pragma unroll
             for (int k = 0; k < 3; k++) {
                 const size_t i = sell_vidx(q, 3 * j + k);
                 const double v = offd[i] * (rf[j] * sc[k]);
                 if constexpr (F32) offd32[i] = (float)v;
                 else offd[i] = v;
             }
+    };
+    if constexpr (K > 0) {
+        int cn[K];
+#pragma unroll
+        for (int b = 0; b < K; b++) cn[b] = sell_col[q0 + 64 * b];
+        double sc[K][3];
+#pragma unroll
+        for (int b = 0; b < K; b++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) sc[b][k] = sv[vidx3(cn[b], k)];
+#pragma unroll
+        for (int b = 0; b < K; b++) scale_slot(q0 + 64 * (size_t)b, sc[b]);
+    } else {
+        const int len = row_ptr[n + 1] - row_ptr[n];
+        for (int b = 0; b < len; b++) {
+            const size_t q = q0 + 64 * (size_t)b;
+            const int cn = sell_col[q];
+            const double sc[3] = {sv[vidx3(cn, 0)], sv[vidx3(cn, 1)], sv[vidx3(cn, 2)]};
+            scale_slot(q, sc);
+        }
     }
 }
 
@@ -141,6 +161,20 @@ __global__ void k_to_f32(size_t n, const double *a, float *b) {
     if (i < n) b[i] = (float)a[i];
 }
 
+// k_scale_apply dispatched on the uniform slice width (0 = per-row lengths)
+#define SW2_SCALE_LAUNCH(F32, BJ, K)                                                                          \
+    hipLaunchKernelGGL((k_scale_apply<F32, BJ, K>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, \
+                       c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,       \
+                       c->offd32, c->dinv32)
+#define SW2_SCALE_SWITCH(F32, BJ)                           \
+    switch (c->kuni) {                                      \
+        case 5: SW2_SCALE_LAUNCH(F32, BJ, 5); break;        \
+        case 6: SW2_SCALE_LAUNCH(F32, BJ, 6); break;        \
+        case 7: SW2_SCALE_LAUNCH(F32, BJ, 7); break;        \
+        case 8: SW2_SCALE_LAUNCH(F32, BJ, 8); break;        \
+        default: SW2_SCALE_LAUNCH(F32, BJ, 0); break;       \
+    }
+
 void launch_setup(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     if (!c->scale_valid)   // a matrix the assembly did not produce: its row maxima first
@@ -148,20 +182,12 @@ void launch_setup(Ctx *c) {
                            c->sell_ptr, c->scale);
     c->scale_valid = 0;    // the scaling below consumes it (and rescales the FP64 matrix in place)
     if (c->opts.preconditioner == 1) {  // profile LU of the node blocks (FP64)
-        hipLaunchKernelGGL((k_scale_apply<false, false>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
-                           c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
-                           c->offd32, c->dinv32);
+        SW2_SCALE_SWITCH(false, false)
         launch_band_setup(c);
         return;
     }
-    if (c->opts.precision >= 1)
-        hipLaunchKernelGGL((k_scale_apply<true, true>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
-                           c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
-                           c->offd32, c->dinv32);
-    else
-        hipLaunchKernelGGL((k_scale_apply<false, true>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd,
-                           c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,
-                           c->offd32, c->dinv32);
+    if (c->opts.precision >= 1) { SW2_SCALE_SWITCH(true, true) }
+    else { SW2_SCALE_SWITCH(false, true) }
 }
 
 // ------------------------------------------------------------ reductions
@@ -206,12 +232,22 @@ __device__ __forceinline__ void reduce_partials(int k0, int k1, const double *pa
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) acc[k] = 0.0;
-    for (int b = k0 + threadIdx.x; b < k1; b += blockDim.x)
+    // U strided partials per thread are loaded together (one memory round
+    // trip instead of U), then folded in ascending block order
+    constexpr int U = 8;
+    const int bd = blockDim.x;
+    for (int b0 = k0 + threadIdx.x; b0 < k1; b0 += U * bd) {
+        double x[U][K];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            double x = part[K * b + k];
-            acc[k] = op[k] ? fmax(acc[k], x) : acc[k] + x;
-        }
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++) x[u][k] = b0 + u * bd < k1 ? part[K * (b0 + u * bd) + k] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b0 + u * bd < k1)
+#pragma unroll
+                for (int k = 0; k < K; k++) acc[k] = op[k] ? fmax(acc[k], x[u][k]) : acc[k] + x[u][k];
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double x = acc[k];

@@ -226,6 +226,59 @@ def test_newton_history(name):
         assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(a).max())
 
 
+def test_large_mesh_jacobian_is_directional_derivative():
+    """At 540,000 triangles (2,110 blocks of 256 triangles, so each thread of
+    the list scan owns more than 8 block counts, and 4,236 assembly
+    workgroups) the assembled Jacobian applied to a random direction equals
+    -dR along it, by a central difference of the device residual
+    (fe_sw2_load.c:34-269 against fe_sw2_resid.c:29-216).  The element
+    blocks are first filled at a far-away state, so a triangle missing from
+    the element lists, or a slot the assembly skipped, keeps blocks that are
+    wrong by O(1) in its rows."""
+    import scipy.sparse
+    p = mesh.bumpy_channel(600, 450, seed=5)
+    N = p.nnodes
+    say(f"large-mesh Jacobian: {p.nelems2d} triangles, {N} nodes")
+    g = SW2Model(p)
+    g.shift(); g.init()
+    r0 = np.abs(g.resid()).max()
+    g.set_opts(default_opts(tol_nonlin=1e-3 * r0, max_nonlin_it=3))
+    g.newton()                                  # a state with flow
+    st = g.get_state()
+    g.set_state(1.5 * st[0], 3.0 * st[1], *st[2:])
+    g.resid(); g.load()                         # element blocks of another state
+    g.set_state(*st)
+    g.resid(); g.load()
+    diag, offd, _ = g.get_system()
+    rp, col = g.pattern()
+    bi = np.concatenate([np.arange(N), np.repeat(np.arange(N), np.diff(rp))])
+    bj = np.concatenate([np.arange(N), col])
+    rr = np.broadcast_to(3 * bi[:, None, None] + np.arange(3)[None, :, None], (len(bi), 3, 3))
+    cc = np.broadcast_to(3 * bj[:, None, None] + np.arange(3)[None, None, :], (len(bi), 3, 3))
+    A = scipy.sparse.csr_matrix((np.concatenate([diag, offd]), (rr.ravel(), cc.ravel())), shape=(3 * N, 3 * N))
+    rng = np.random.default_rng(1)
+    d = rng.standard_normal(3 * N)
+    d[2::3] *= 1e-2
+    t = 1e-6
+
+    def R_at(s):
+        h = st[0] + s * d[2::3]
+        vel = st[1].copy()
+        vel[0::2] += s * d[0::3]
+        vel[1::2] += s * d[1::3]
+        g.set_state(h, vel, *st[2:])
+        return g.resid()
+
+    fd = -(R_at(t) - R_at(-t)) / (2 * t)
+    Jd = A @ d
+    g.close()
+    worst = np.abs(Jd - fd).max() / np.abs(fd).max()
+    l2 = np.linalg.norm(Jd - fd) / np.linalg.norm(fd)
+    say(f"  max row error / max |dR| = {worst:.3e}, l2 {l2:.3e}")
+    assert l2 < 1e-6
+    assert worst < 1e-4
+
+
 def test_newton_fp32_refined():
     """FP32 matrix/preconditioner/Krylov with FP64 residual and update reaches
     the same FP64 nonlinear tolerance; state within 1e-7 relative of FP64."""
