@@ -279,6 +279,95 @@ def test_large_mesh_jacobian_is_directional_derivative():
     assert worst < 1e-4
 
 
+def _hub_basin(nring=20, seed=3):
+    """A 110 m square basin whose centre node has `nring` neighbours (a ring
+    of nodes around it, Delaunay-triangulated with a 10 m grid outside),
+    flume-style inflow and tailwater strings, a wavy bed and random
+    velocities: rows longer than the assembly's 15 off-diagonal waves and
+    than the unrolled SpMV widths."""
+    import scipy.spatial
+    rng = np.random.default_rng(seed)
+    gx, gy = np.meshgrid(np.arange(12) * 10.0, np.arange(12) * 10.0, indexing="ij")
+    pts = np.stack([gx.ravel(), gy.ravel()], 1)
+    c = np.array([55.0, 55.0])
+    pts = pts[np.linalg.norm(pts - c, axis=1) > 25.0]
+    ang = 2 * np.pi * (np.arange(nring) + 0.5) / nring
+    ring = c + 12.0 * np.stack([np.cos(ang), np.sin(ang)], 1)
+    pts = np.concatenate([c[None], ring, pts])
+    tri = scipy.spatial.Delaunay(pts).simplices.astype(np.int64)
+    a, b, cc = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    cross = (b[:, 0] - a[:, 0]) * (cc[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (cc[:, 0] - a[:, 0])
+    tri[cross < 0] = tri[cross < 0][:, [0, 2, 1]]
+    N, E = len(pts), len(tri)
+    x, y = pts[:, 0].copy(), pts[:, 1].copy()
+    z = -(1e-3 * x) + 0.05 * np.sin(x / 17.0) * np.cos(y / 13.0)
+    # flume-style strings: NB VEL inflow on the west edge, NB OTW tailwater on
+    # the east edge, each edge from north to south, with its triangle
+    def edges(side_x):
+        nodes = np.where(np.isclose(x, side_x))[0]
+        nodes = nodes[np.argsort(-y[nodes])]
+        e = np.stack([nodes[:-1], nodes[1:]], 1)
+        adj = [int(np.where((tri == a).any(1) & (tri == b).any(1))[0][0]) for a, b in e]
+        return e, np.array(adj)
+    ew, aw = edges(0.0)
+    ee, ae = edges(110.0)
+    tail = float(z[np.isclose(x, 110.0)].max()) + 1.0
+    e1d = np.concatenate([ew, ee]).astype(np.int32).ravel()
+    e1d_adj = np.concatenate([aw, ae]).astype(np.int32)
+    e1d_bc = np.concatenate([np.full(len(ew), mesh.BCT_VEL_NEU), np.full(len(ee), mesh.BCT_PRS_NEU)]).astype(np.int32)
+    e1d_val = np.concatenate([np.full(len(ew), 0.05), np.full(len(ee), tail)])
+    p = mesh.SW2Problem(
+        name="hub-basin", x=x, y=y, z=z, e2d=np.ascontiguousarray(tri.ravel().astype(np.int32)),
+        e2d_mat=np.zeros(E, dtype=np.int32), e2d_ks=np.full(E, mesh.manning_to_ks(0.03)),
+        e2d_src_flag=np.zeros(E, dtype=np.int32), e2d_src=np.zeros(E),
+        e1d=np.ascontiguousarray(e1d), e1d_adj=np.ascontiguousarray(e1d_adj), e1d_bc=np.ascontiguousarray(e1d_bc),
+        e1d_val=np.ascontiguousarray(e1d_val), node_bc=np.zeros(N, dtype=np.int32), node_bc_val=np.zeros(3 * N),
+        mat_turb=np.array([mesh.TURB_EVS], dtype=np.int32), mat_eev_mode=np.array([0], dtype=np.int32),
+        mat_eev_coef=np.array([0.0]), mat_ev=np.zeros(3), mat_cor_speed=np.array([0.0]), dt=5.0,
+        head0=np.ascontiguousarray(tail - z), vel0=np.zeros(2 * N))
+    vel = np.zeros(2 * N)
+    vel[0::2] = 0.1 + 0.02 * rng.standard_normal(N)
+    vel[1::2] = 0.02 * rng.standard_normal(N)
+    return p, vel
+
+
+def test_high_valence_mesh():
+    """Residual, element Jacobians, assembled blocks and scaled system
+    bit-exact, and Newton within the FP64 tolerances, on a mesh with a node
+    of 20 neighbours (the assembly's strided slot loop; the per-slice SpMV
+    and scaling loops instead of the unrolled uniform widths)."""
+    p, vel = _hub_basin()
+    say(f"high-valence mesh: {p.nnodes} nodes, {p.nelems2d} triangles")
+    st = (p.head0, vel, p.head0, vel, p.head0, vel)
+    o, g = pair(p, st)
+    rp, _ = g.pattern()
+    assert np.diff(rp).max() == 20
+    o.shift(); g.shift()
+    o.init(); g.init()
+    assert nmismatch(o.resid(), g.resid()) == 0
+    d_o, f_o = o.load()
+    d_g, f_g, rr_g = g.load()
+    assert nmismatch(d_o, d_g) == 0 and nmismatch(f_o, f_g) == 0
+    assert bits_equal(o.residual(), rr_g)
+    em, _ = g.elem_jacobians()
+    for ie in range(p.nelems2d):
+        assert bits_equal(o.elem_jacobian(ie), em[81 * ie: 81 * ie + 81])
+    assert bits_equal(o.linear_setup(oopts(prec=2)), g.linear_sys_setup())
+    o, g = pair(p, st)
+    o.shift(); g.shift(); o.init(); g.init()
+    r0 = np.abs(o.resid()).max()
+    tol = 1e-8 * r0
+    g.set_opts(default_opts(tol_nonlin=tol, max_nonlin_it=20))
+    res_o = o.newton(oopts(tol_nonlin=tol, prec=2, max_nonlin_it=20))
+    res_g = g.newton()
+    say(f"  newton its oracle={res_o['iterations']} gpu={res_g['iterations']}")
+    assert res_g["verdict"] == YES and res_o["ok"] == 1
+    assert res_g["iterations"] == res_o["iterations"]
+    assert np.allclose(res_g["hist_max"], res_o["hist_max"], rtol=0, atol=1e-8 * r0)
+    for a, b in zip(o.get_state(), g.get_state()):
+        assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(a).max())
+
+
 def test_newton_fp32_refined():
     """FP32 matrix/preconditioner/Krylov with FP64 residual and update reaches
     the same FP64 nonlinear tolerance; state within 1e-7 relative of FP64."""
