@@ -24,13 +24,27 @@ static void launch_resid2d_split(Ctx *c, const ElemData &D);
 // ------------------------------------------------------------------ residual
 // The same over a triangle list (grid-stride over the device-side count):
 // WET lists hold triangles with positive depths, run without clipping code.
+// Software pipeline: the triangle id two iterations ahead and its node ids
+// one iteration ahead are loaded while the current triangle is computed.
 template <bool WET, int BS>
 __global__ void __launch_bounds__(BS) k_resid2d_list(ElemData D, const int *list, const int *count, double *ER) {
     const int n = *count;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x) {
-        const int ie = list[li];
+    const int stride = gridDim.x * blockDim.x;
+    int li = blockIdx.x * blockDim.x + threadIdx.x;
+    int ie_a = li < n ? list[li] : 0;
+    int ie_b = li + stride < n ? list[li + stride] : 0;
+    int nd_a[3] = {0, 0, 0};
+    if (li < n)
+        for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
+    for (; li < n; li += stride) {
+        const int ie = ie_a;
+        const int nodes[3] = {nd_a[0], nd_a[1], nd_a[2]};
+        ie_a = ie_b;
+        ie_b = li + 2 * stride < n ? list[li + 2 * stride] : 0;
+        if (li + stride < n)
+            for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
         d3 r[3];
-        body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, 0.0, 0, PERTURB_NONE, 0, true, r);
+        body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, 0.0, 0, PERTURB_NONE, 0, true, r, nodes);
         double *o = ER + 9 * (size_t)ie;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
