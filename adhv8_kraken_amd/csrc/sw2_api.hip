@@ -816,17 +816,21 @@ static int krylov(Ctx &c, int *lin_it, int *verdict) {
     // first chunk: the previous solve's iteration count rounded up to a
     // multiple of 4 (consecutive Newton systems converge alike), then chunks
     // of 4; chunk lengths stay in a small set so few graphs are captured.
-    // The device flag makes iterations launched past convergence return at entry.
+    // The device flag makes iterations launched past convergence return at
+    // entry, so the first chunk follows the prologue without a host check.
+    // (The epilogue adds the initial guess held in sol, so it runs once, after
+    // the last chunk.)
     int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
+    int left = lin_it_cap(c);
     for (int chunk = 0;; chunk++) {
-        HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal), hipMemcpyDeviceToHost, c.stream));
-        HIPCHK(hipStreamSynchronize(c.stream));
-        if (c.h_ksc->done) break;
-        int left = lin_it_cap(c) - c.h_ksc->it;
         int want = chunk == 0 ? first : 4;
         int k = std::max(1, std::min(want, left));
         int rc = run_iters(c, k);
         if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        if (c.h_ksc->done) break;
+        left = lin_it_cap(c) - c.h_ksc->it;
     }
     c.last_lin_it = c.h_ksc->it;
     launch_krylov_epilogue(&c);
@@ -1112,12 +1116,17 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
     }
     HIPCHK(hipMemcpyAsync(c.ksc, init.data(), sizeof(KScal) * n, hipMemcpyHostToDevice, c.stream));
     launch_krylov_prologue(&c);
+    // chunks as in krylov(): no host check between the prologue and the first chunk
     int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
     int maxit = 0;
+    int left = cap;
     for (int chunk = 0;; chunk++) {
+        int want = chunk == 0 ? first : 4;
+        int rc = run_iters(c, std::max(1, std::min(want, left)));
+        if (rc) return rc;
         HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal) * n, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
-        int left = 0;
+        left = 0;
         bool all_done = true;
         for (int i = 0; i < n; i++)
             if (!c.h_ksc[i].done) {
@@ -1125,9 +1134,6 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
                 left = std::max(left, cap - c.h_ksc[i].it);
             }
         if (all_done) break;
-        int want = chunk == 0 ? first : 4;
-        int rc = run_iters(c, std::max(1, std::min(want, left)));
-        if (rc) return rc;
     }
     launch_krylov_epilogue(&c);
     HIPCHK(hipGetLastError());
