@@ -806,7 +806,10 @@ static int run_iters(Ctx &c, int k) {
     return 0;
 }
 
-static int krylov(Ctx &c, int *lin_it, int *verdict) {
+// zero_guess: sol holds a zero initial guess (the Newton step zeroes it), so
+// the epilogue is idempotent and is queued behind every chunk before the
+// convergence flag is read; otherwise it runs once, after the last chunk.
+static int krylov(Ctx &c, int *lin_it, int *verdict, bool zero_guess) {
     KScal init{};
     init.atol = c.opts.lin_atol;
     init.rtol = c.opts.lin_rtol;
@@ -818,8 +821,6 @@ static int krylov(Ctx &c, int *lin_it, int *verdict) {
     // of 4; chunk lengths stay in a small set so few graphs are captured.
     // The device flag makes iterations launched past convergence return at
     // entry, so the first chunk follows the prologue without a host check.
-    // (The epilogue adds the initial guess held in sol, so it runs once, after
-    // the last chunk.)
     int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
     int left = lin_it_cap(c);
     for (int chunk = 0;; chunk++) {
@@ -827,13 +828,14 @@ static int krylov(Ctx &c, int *lin_it, int *verdict) {
         int k = std::max(1, std::min(want, left));
         int rc = run_iters(c, k);
         if (rc) return rc;
+        if (zero_guess) launch_krylov_epilogue(&c, true);
         HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         if (c.h_ksc->done) break;
         left = lin_it_cap(c) - c.h_ksc->it;
     }
     c.last_lin_it = c.h_ksc->it;
-    launch_krylov_epilogue(&c);
+    if (!zero_guess) launch_krylov_epilogue(&c, false);
     HIPCHK(hipGetLastError());
     if (lin_it) *lin_it = c.h_ksc->it;
     *verdict = (c.h_ksc->it == lin_it_cap(c)) ? ADH_SW2_NO : ADH_SW2_YES;
@@ -845,7 +847,7 @@ extern "C" int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *li
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     int vd = ADH_SW2_NO;
-    int rc = krylov(c, lin_it, &vd);
+    int rc = krylov(c, lin_it, &vd, false);
     if (rc) return rc;
     if ((rc = sync_check(c))) return rc;
     if (verdict) *verdict = vd;
@@ -996,7 +998,7 @@ static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
         (void)hipEventRecord(c.ev[4], c.stream);
         launch_setup(&c);
         (void)hipEventRecord(c.ev[5], c.stream);
-        if ((rc = krylov(c, &lin_it, &vd))) return rc;
+        if ((rc = krylov(c, &lin_it, &vd, true))) return rc;   // sol zeroed above
         (void)hipEventRecord(c.ev[6], c.stream);
         // FP64: the reference's rule, a Krylov failure fails the Newton step.
         // FP32-refined forms: an unconverged inner solve is still a
@@ -1116,7 +1118,9 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
     }
     HIPCHK(hipMemcpyAsync(c.ksc, init.data(), sizeof(KScal) * n, hipMemcpyHostToDevice, c.stream));
     launch_krylov_prologue(&c);
-    // chunks as in krylov(): no host check between the prologue and the first chunk
+    // chunks as in krylov() with a zero guess (newton_batch zeroes sol): no
+    // host check between the prologue and the first chunk, the idempotent
+    // epilogue queued behind every chunk before the flags are read
     int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
     int maxit = 0;
     int left = cap;
@@ -1124,6 +1128,7 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
         int want = chunk == 0 ? first : 4;
         int rc = run_iters(c, std::max(1, std::min(want, left)));
         if (rc) return rc;
+        launch_krylov_epilogue(&c, true);
         HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal) * n, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         left = 0;
@@ -1135,7 +1140,6 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
             }
         if (all_done) break;
     }
-    launch_krylov_epilogue(&c);
     HIPCHK(hipGetLastError());
     lin_it.assign(n, 0);
     vd.assign(n, ADH_SW2_YES);

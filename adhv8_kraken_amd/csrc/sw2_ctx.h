@@ -187,7 +187,7 @@ void launch_norms(Ctx *c);
 void launch_setup(Ctx *c);
 void launch_krylov_prologue(Ctx *c);
 void launch_krylov_iters(Ctx *c, int k);
-void launch_krylov_epilogue(Ctx *c);
+void launch_krylov_epilogue(Ctx *c, bool zero_guess);
 void launch_geometry(Ctx *c);
 void launch_zero(Ctx *c, double *p, size_t n);
 void launch_copy(Ctx *c, const double *src, double *dst, size_t n);

@@ -641,13 +641,16 @@ __global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, doubl
     block_partials<3>(v, op, part);
 }
 
-// solv_return_old (src/solver/solv_return_old.c:23-30): sol = (u + u0) * scale
-template <typename T>
+// solv_return_old (src/solver/solv_return_old.c:23-30): sol = (u + u0) * scale,
+// u0 the initial guess held in sol.  ZERO: the caller zeroed the guess (the
+// Newton step does), so u0 = +0 enters as a constant and the kernel gives the
+// same sol however often it runs.
+template <typename T, bool ZERO>
 __global__ void k_bcg_epi(int N3, const T *u, const double *scale, double *sol) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N3) return;
     double x = (double)u[i];
-    x += sol[i];
+    x += ZERO ? 0.0 : sol[i];
     x *= scale[i];
     sol[i] = x;
 }
@@ -786,14 +789,16 @@ void launch_krylov_iters(Ctx *c, int k) {
     else iters<PrecF64>(c, k);
 }
 
-void launch_krylov_epilogue(Ctx *c) {
+void launch_krylov_epilogue(Ctx *c, bool zero_guess) {
     int n3 = 3 * c->Np;
-    if (c->opts.precision == 2)
-        hipLaunchKernelGGL(k_bcg_epi<float>, dim3(nblocks_for(n3, 256)), dim3(256), 0, c->stream, n3, c->fu, c->scale,
-                           c->sol);
-    else
-        hipLaunchKernelGGL(k_bcg_epi<double>, dim3(nblocks_for(n3, 256)), dim3(256), 0, c->stream, n3, c->ku, c->scale,
-                           c->sol);
+    dim3 g(nblocks_for(n3, 256));
+    if (c->opts.precision == 2) {
+        if (zero_guess) hipLaunchKernelGGL((k_bcg_epi<float, true>), g, dim3(256), 0, c->stream, n3, c->fu, c->scale, c->sol);
+        else hipLaunchKernelGGL((k_bcg_epi<float, false>), g, dim3(256), 0, c->stream, n3, c->fu, c->scale, c->sol);
+    } else {
+        if (zero_guess) hipLaunchKernelGGL((k_bcg_epi<double, true>), g, dim3(256), 0, c->stream, n3, c->ku, c->scale, c->sol);
+        else hipLaunchKernelGGL((k_bcg_epi<double, false>), g, dim3(256), 0, c->stream, n3, c->ku, c->scale, c->sol);
+    }
 }
 
 }  // namespace sw2
