@@ -717,9 +717,16 @@ extern "C" int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
     HIPCHK(hipStreamSynchronize(c.stream));
-    // the device keeps element blocks in slot order (k_assemble reads them contiguously)
+    // the device keeps element blocks in slot order (k_assemble reads them
+    // contiguously), in FP32 after a refined-form Jacobian
     std::vector<double> ems(9 * c.h_blkpos.size());
-    HIPCHK(hipMemcpy(ems.data(), c.EM, sizeof(double) * ems.size(), hipMemcpyDeviceToHost));
+    if (c.em32) {
+        std::vector<float> f(ems.size());
+        HIPCHK(hipMemcpy(f.data(), c.EM, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); i++) ems[i] = (double)f[i];
+    } else {
+        HIPCHK(hipMemcpy(ems.data(), c.EM, sizeof(double) * ems.size(), hipMemcpyDeviceToHost));
+    }
     const size_t nb2 = 9 * (size_t)c.E;
     for (size_t blk = 0; blk < c.h_blkpos.size(); blk++) {
         double *dst = blk < nb2 ? (em2d ? em2d + 9 * blk : nullptr) : (em1d ? em1d + 9 * (blk - nb2) : nullptr);

@@ -21,6 +21,11 @@ int nblocks_for(int n, int bs) { return n <= 0 ? 1 : (n + bs - 1) / bs; }
 
 static void launch_resid2d_split(Ctx *c, const ElemData &D);
 
+// The refined forms (precision >= 1) store the element Jacobian blocks in
+// FP32 (the matrix they assemble is stored in FP32 anyway); the FP64 form in
+// FP64.  Returns the FP32 view of the slab, or null for FP64 storage.
+static float *em32_of(Ctx *c) { return c->opts.precision >= 1 ? (float *)c->EM : nullptr; }
+
 // ------------------------------------------------------------------ residual
 // The same over a triangle list (grid-stride over the device-side count):
 // WET lists hold triangles with positive depths, run without clipping code.
@@ -143,7 +148,7 @@ void launch_resid(Ctx *c) {
 template <bool WET, int BS, int OCC>
 __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D, const int *list, const int *count,
                                                                       int b0, int b1, int b2, const int *blkpos,
-                                                                      double *EM) {
+                                                                      double *EM, float *EM32) {
     const int nt = 18 * *count;
     const int stride = gridDim.x * blockDim.x;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,14 +195,18 @@ __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D
     jac_column_db<3>(D, nodes, lb, j, var, col);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        double *o = EM + 9 * (size_t)blkpos[9 * ie + 3 * i + j];
+        const size_t o = 9 * (size_t)blkpos[9 * ie + 3 * i + j];
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++) o[3 * rr + var] = col[3 * i + rr];
+        for (int rr = 0; rr < 3; rr++) {
+            if (EM32) EM32[o + 3 * rr + var] = (float)col[3 * i + rr];
+            else EM[o + 3 * rr + var] = col[3 * i + rr];
+        }
     }
     }
 }
 
-__global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, const int *blkpos1, double *EM) {
+__global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, const int *blkpos1, double *EM,
+                                               float *EM32) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * B) return;
     int ie = t / 6, q = t % 6, j = q / 3, var = q % 3;
@@ -209,9 +218,12 @@ __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1
     jac_column_db<2>(D, nodes, lb, j, var, col);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-        double *o = EM + 9 * (size_t)blkpos1[4 * ie + 2 * i + j];
+        const size_t o = 9 * (size_t)blkpos1[4 * ie + 2 * i + j];
 #pragma unroll
-        for (int r = 0; r < 3; r++) o[3 * r + var] = col[3 * i + r];
+        for (int r = 0; r < 3; r++) {
+            if (EM32) EM32[o + 3 * r + var] = (float)col[3 * i + r];
+            else EM[o + 3 * r + var] = col[3 * i + r];
+        }
     }
 }
 
@@ -343,7 +355,8 @@ static void launch_split(Ctx *c) {
 template <bool WET, int BS, int OCC>
 static void jac_list_occ(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
     hipLaunchKernelGGL((k_jac2d_list<WET, BS, OCC>), dim3(std::min(grid, nblocks_for(18 * c->E, BS))), dim3(BS), 0,
-                       c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM);
+                       c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
+                       em32_of(c));
 }
 // the wet list carries almost all triangles: it also gets register-budget variants
 template <bool WET, int BS>
@@ -411,10 +424,11 @@ void launch_jacobian(Ctx *c) {
     if (!c->pre_valid) launch_old_terms(c);
     ElemData D = c->ed;
     D.first_elem = nullptr;
+    c->em32 = em32_of(c) != nullptr;
     launch_jac2d_split(c, D);
     if (c->B)
         hipLaunchKernelGGL(k_jac1d, dim3(nblocks_for(6 * c->B, 128)), dim3(128), 0, c->stream, D, c->B, c->loc_bc[0],
-                           c->loc_bc[1], c->blkpos + 9 * (size_t)c->E, c->EM);
+                           c->loc_bc[1], c->blkpos + 9 * (size_t)c->E, c->EM, em32_of(c));
 }
 
 // Ordered block assembly.  The element kernels store each 3x3 block at its
@@ -428,31 +442,33 @@ void launch_jacobian(Ctx *c) {
 // fe_sw2_load.c:218-236; the waves' row maxima, combined in LDS, give the
 // Jacobi scale of solv_linear_sys_setup (the same values as k_scale_rows,
 // which the set-up then skips).
-__device__ __forceinline__ void sum_run(const int *slot_ptr, size_t slot, const double *EM, double (&a)[9]) {
+template <typename TM>
+__device__ __forceinline__ void sum_run(const int *slot_ptr, size_t slot, const TM *EM, double (&a)[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k] = 0.0;
     const int p0 = slot_ptr[slot], p1 = slot_ptr[slot + 1];
     // pairs of blocks: both loaded before the in-order sums
     int p = p0;
     for (; p + 1 < p1; p += 2) {
-        const double *b0 = EM + 9 * (size_t)p, *b1 = b0 + 9;
+        const TM *b0 = EM + 9 * (size_t)p, *b1 = b0 + 9;
         double x0[9], x1[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) { x0[k] = b0[k]; x1[k] = b1[k]; }
+        for (int k = 0; k < 9; k++) { x0[k] = (double)b0[k]; x1[k] = (double)b1[k]; }
 #pragma unroll
         for (int k = 0; k < 9; k++) a[k] += x0[k];
 #pragma unroll
         for (int k = 0; k < 9; k++) a[k] += x1[k];
     }
     if (p < p1) {
-        const double *b = EM + 9 * (size_t)p;
+        const TM *b = EM + 9 * (size_t)p;
 #pragma unroll
-        for (int k = 0; k < 9; k++) a[k] += b[k];
+        for (int k = 0; k < 9; k++) a[k] += (double)b[k];
     }
 }
 
+template <typename TM>
 __global__ void __launch_bounds__(1024) k_assemble(int N, const int *slot_ptr, const int *row_ptr,
-                                                   const int *sell_ptr, const double *EM, double *diag, double *offd,
+                                                   const int *sell_ptr, const TM *EM, double *diag, double *offd,
                                                    double *R, const double *u, const double *v, double *sv) {
     __shared__ double smax[16][3][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -513,8 +529,14 @@ __global__ void __launch_bounds__(1024) k_assemble(int N, const int *slot_ptr, c
 
 void launch_assemble(Ctx *c) {
     const int nw = 1 + std::min(std::max(c->kmax_row, 1), 15);   // waves per 64-row slice
-    hipLaunchKernelGGL(k_assemble, dim3(nblocks_for(c->N, 64)), dim3(64 * nw), 0, c->stream, c->N, c->slot_ptr,
-                       c->row_ptr, c->sell_ptr, c->EM, c->diag, c->offd, c->R, c->u, c->v, c->scale);
+    if (c->em32)   // element blocks stored in FP32 by the refined forms, summed in FP64
+        hipLaunchKernelGGL(k_assemble<float>, dim3(nblocks_for(c->N, 64)), dim3(64 * nw), 0, c->stream, c->N,
+                           c->slot_ptr, c->row_ptr, c->sell_ptr, (const float *)c->EM, c->diag, c->offd, c->R, c->u,
+                           c->v, c->scale);
+    else
+        hipLaunchKernelGGL(k_assemble<double>, dim3(nblocks_for(c->N, 64)), dim3(64 * nw), 0, c->stream, c->N,
+                           c->slot_ptr, c->row_ptr, c->sell_ptr, (const double *)c->EM, c->diag, c->offd, c->R, c->u,
+                           c->v, c->scale);
     c->scale_valid = 1;
 }
 
