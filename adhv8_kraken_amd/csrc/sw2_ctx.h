@@ -30,7 +30,7 @@ struct KScal {
 
 // Reduction slots: per-block partials and completion counters.
 struct Red {
-    double *part;      // [3 * maxblocks]
+    double *part;      // [4 * maxblocks]: up to 3 partials per block, then K1's (1 per block)
     unsigned *count;   // [8] counters, one per reduction site
     int maxblocks;
 };

@@ -10,7 +10,8 @@
 //
 // Each kernel writes block partials of its dot products, and a one-block
 // finisher kernel reduces them in block order and updates the device
-// scalars, so a BiCGSTAB iteration is 6 launches with no host round trip; the
+// scalars (K1's partials are reduced at the start of K2 instead), so a
+// BiCGSTAB iteration is 5 launches with no host round trip; the
 // host reads the convergence flag every opts.check_every iterations and
 // kernels launched past convergence return at entry.  The same kernels run in FP64 or FP32
 // (precision 0), with FP32 matrix and preconditioner and FP64 vectors
@@ -480,15 +481,36 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *se
     block_partials<1>(v, op, part);
 }
 
+// K2 also finishes K1's reduction: every block reduces K1's block partials of
+// its model (part1, in the one-block finisher's order, so every block gets
+// the finisher's gamma bit for bit) and forms alpha = rho / gamma itself; the
+// model's first block stores gamma and alpha for K3.  This saves the finisher
+// launch between K1 and K2.
 template <typename P, int K>
-__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part, const int *blk_inst, KScal *sc) {
+__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
+                                                const int *blk_inst, KScal *sc, const double *part1, int nparts,
+                                                const int *inst_blk0) {
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
+    int m = 0;
+    if (blk_inst) { m = blk_inst[2 * blockIdx.x]; sc += m; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
+    __shared__ double s_alpha;
+    {
+        const int k0 = inst_blk0 ? inst_blk0[m] : 0, k1 = inst_blk0 ? inst_blk0[m + 1] : nparts;
+        const int op[1] = {0};
+        double g[1];
+        reduce_partials<1>(k0, k1, part1, op, g);
+        if (threadIdx.x == 0) {
+            const double alpha = sc->rho / g[0];
+            s_alpha = alpha;
+            if ((int)blockIdx.x == k0) { sc->gamma = g[0]; sc->alpha = alpha; }
+        }
+        __syncthreads();
+    }
     int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const T nalpha = (T)(-sc->alpha);
+    const T nalpha = (T)(-s_alpha);
     double v[3] = {0.0, 0.0, 0.0};
     if (n < nend) {
         VecS<T> sf{V.r, V.Ap, nalpha};
@@ -686,12 +708,12 @@ template <typename P, int K>
 static void iters_k(Ctx *c, int k) {
     int nb = nblocks_for(c->N, 256);
     KVecs<P> V = vecs<P>(c);
+    double *part1 = c->red.part + 3 * (size_t)c->red.maxblocks;   // K1's partials, reduced inside K2
     for (int i = 0; i < k; i++) {
         hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                           c->red.part, BI(c), c->ksc);
-        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
+                           part1, BI(c), c->ksc);
         hipLaunchKernelGGL((k_bcg_k2<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                           c->red.part, BI(c), c->ksc);
+                           c->red.part, BI(c), c->ksc, part1, nb, IB0(c));
         hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
         hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
@@ -703,7 +725,7 @@ template <typename P, int K>
 static void k1_only_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, vecs<P>(c), c->sell_ptr,
-                       c->sell_col, c->red.part, BI(c), c->ksc);
+                       c->sell_col, c->red.part + 3 * (size_t)c->red.maxblocks, BI(c), c->ksc);
 }
 
 // dispatch on the uniform slice width (0 = per-slice widths, looped)

@@ -368,6 +368,32 @@ def test_high_valence_mesh():
         assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(a).max())
 
 
+def test_refined_element_blocks_are_fp32_roundings(stepped):
+    """The refined forms store each FD element block as the FP32 rounding of
+    the FP64 block that the FP64 form stores (the same FP64 arithmetic, then
+    one rounding), and the assembly sums them in FP64: the assembled matrix
+    stays within FP32 rounding of the FP64 form's."""
+    name = "bumpy-60x30"
+    p = CASES[name]
+    say(f"refined element blocks: {name}")
+    out = {}
+    for prec in (0, 1):
+        g = SW2Model(p, default_opts(precision=prec))
+        g.set_state(*stepped[name])
+        g.shift(); g.init(); g.resid()
+        d, f, _ = g.load()
+        em2, em1 = g.elem_jacobians()
+        out[prec] = (d, f, em2, em1)
+        g.close()
+    (d0, f0, e0, b0), (d1, f1, e1, b1) = out[0], out[1]
+    assert bits_equal(e1, e0.astype(np.float32).astype(np.float64))
+    assert bits_equal(b1, b0.astype(np.float32).astype(np.float64))
+    big = max(np.abs(d0).max(), np.abs(f0).max())
+    err = max(np.abs(d1 - d0).max(), np.abs(f1 - f0).max()) / big
+    say(f"  assembled refined vs FP64: {err:.2e} of the largest entry")
+    assert err <= 1e-6
+
+
 def test_newton_fp32_refined():
     """FP32 matrix/preconditioner/Krylov with FP64 residual and update reaches
     the same FP64 nonlinear tolerance; state within 1e-7 relative of FP64."""
