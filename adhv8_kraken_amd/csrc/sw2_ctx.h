@@ -22,7 +22,7 @@ constexpr size_t BAND_MAX_BYTES = (size_t)8 << 30;       // band storage limit
 
 // Device-side scalars of one BiCGSTAB solve (solv_bcgstab.c:60-70).
 struct KScal {
-    double rho, rhop, alpha, omega, beta, gamma;
+    double rho, rhop, alpha, omega, beta, gamma, rho_next;
     double rnorm, bnorm, tol;
     double atol, rtol;
     int it, max_it, done, pad;
