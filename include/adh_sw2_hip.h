@@ -92,11 +92,13 @@ typedef struct {
     int max_cuts;        /* 5 line-search cuts */
     double lin_rtol;     /* BiCGSTAB stop: ||r||inf <= lin_atol + lin_rtol*||M^-1 b||inf */
     double lin_atol;
-    int precision;       /* 0: FP64; 1: FP32 matrix + preconditioner, FP64 Krylov vectors; 2: FP32 matrix and
-                            Krylov; 1 and 2 keep the residual, update and Newton tests in FP64 */
+    int precision;       /* 0: FP64; 1: FP32 element blocks, matrix and preconditioner, FP64 Krylov vectors;
+                            2: FP32 matrix and Krylov; 1 and 2 form the FD Jacobian in FP64 and keep the
+                            residual, update and Newton tests in FP64 */
     int preconditioner;  /* 2: 3x3 block Jacobi; 1: profile LU of node blocks (OP PRE 1 with OP BLK
                             nblock, src/solver/solv_blk_*.c), FP64 only */
-    int check_every;     /* Krylov iterations between host convergence checks */
+    int check_every;     /* Krylov iterations before the first host convergence check of a solve that has
+                            no previous count (later solves start from the previous count, rounded up to 4) */
     int nblock;          /* OP BLK: node blocks of the profile-LU preconditioner (default 1) */
     int graphs;          /* 1: replay captured HIP graphs of Krylov iteration chunks; 0 (default): launch eagerly */
 } adh_sw2_opts;
@@ -149,7 +151,9 @@ int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual);
  * tweak) optional. */
 int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdiag, double *residual);
 /* Element Jacobians as assembled (after Dirichlet edits): [81*E] in block order
- * (row node i, col node j) -> 9 values; for parity tests. */
+ * (row node i, col node j) -> 9 values; for parity tests.  The refined forms
+ * (precision >= 1) keep them in FP32: the values are the FP32 roundings of
+ * the FP64 form's. */
 int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *em1d);
 /* fe_sw2_inc (src/fe/sw2/fe_sw2_inc.c:24-57); sol NULL = the device solution. */
 int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol);
