@@ -22,7 +22,7 @@ constexpr size_t BAND_MAX_BYTES = (size_t)8 << 30;       // band storage limit
 
 // Device-side scalars of one BiCGSTAB solve (solv_bcgstab.c:60-70).
 struct KScal {
-    double rho, rhop, alpha, omega, beta, gamma, rho_next;
+    double rho, rhop, alpha, omega, beta, gamma;
     double rnorm, bnorm, tol;
     double atol, rtol;
     int it, max_it, done, pad;
@@ -30,7 +30,7 @@ struct KScal {
 
 // Reduction slots: per-block partials and completion counters.
 struct Red {
-    double *part;      // [4 * maxblocks]: up to 3 partials per block, then K1's (1 per block)
+    double *part;      // [3 * maxblocks]
     unsigned *count;   // [8] counters, one per reduction site
     int maxblocks;
 };

@@ -10,8 +10,7 @@
 //
 // Each kernel writes block partials of its dot products, and a one-block
 // finisher kernel reduces them in block order and updates the device
-// scalars (K1's and K2's partials are reduced at the start of K2 and K3
-// instead), so a BiCGSTAB iteration is 4 launches with no host round trip; the
+// scalars, so a BiCGSTAB iteration is 6 launches with no host round trip; the
 // host reads the convergence flag every opts.check_every iterations and
 // kernels launched past convergence return at entry.  The same kernels run in FP64 or FP32
 // (precision 0), with FP32 matrix and preconditioner and FP64 vectors
@@ -270,7 +269,7 @@ __device__ __forceinline__ void reduce_partials(int k0, int k1, const double *pa
 }
 
 // finisher stages of the BiCGSTAB reductions (solv_bcgstab.c:117-290)
-enum { FIN_P1, FIN_P2, FIN_K1, FIN_K3 };
+enum { FIN_P1, FIN_P2, FIN_K1, FIN_K2, FIN_K3 };
 
 // one block per model (blockIdx.x), over that model's 256-node blocks
 template <int STAGE>
@@ -278,12 +277,13 @@ __global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KSc
     const int m = blockIdx.x;
     const int k0 = inst_blk0 ? inst_blk0[m] : 0, k1 = inst_blk0 ? inst_blk0[m + 1] : nparts;
     sc += m;
-    if ((STAGE == FIN_K1 || STAGE == FIN_K3) && sc->done) return;
-    constexpr int K = STAGE == FIN_P2 ? 2 : 1;
+    if ((STAGE == FIN_K1 || STAGE == FIN_K2 || STAGE == FIN_K3) && sc->done) return;
+    constexpr int K = STAGE == FIN_P2 ? 2 : (STAGE == FIN_K2 ? 3 : 1);
     int op[K];
     if constexpr (STAGE == FIN_P1 || STAGE == FIN_K3) op[0] = 1;
     else if constexpr (STAGE == FIN_P2) { op[0] = 1; op[1] = 0; }
-    else op[0] = 0;   // FIN_K1
+    else if constexpr (STAGE == FIN_K1) op[0] = 0;
+    else { op[0] = 0; op[1] = 0; op[2] = 0; }
     double out[K];
     reduce_partials<K>(k0, k1, part, op, out);
     if (threadIdx.x != 0) return;
@@ -301,9 +301,12 @@ __global__ void __launch_bounds__(256) k_fin(int nparts, const double *part, KSc
     } else if constexpr (STAGE == FIN_K1) {
         sc->gamma = out[0];
         sc->alpha = sc->rho / sc->gamma;
+    } else if constexpr (STAGE == FIN_K2) {
+        double omega = out[0] / out[1];
+        sc->omega = omega;
+        sc->rhop = sc->rho;
+        sc->rho = -1.0 * omega * out[2];
     } else {
-        sc->rhop = sc->rho;      // the rho update K3 formed (see k_bcg_k3)
-        sc->rho = sc->rho_next;
         sc->rnorm = out[0];
         sc->it += 1;
         sc->done = !(sc->rnorm > sc->tol && sc->it < sc->max_it);
@@ -477,36 +480,15 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *se
     block_partials<1>(v, op, part);
 }
 
-// K2 also finishes K1's reduction: every block reduces K1's block partials of
-// its model (part1, in the one-block finisher's order, so every block gets
-// the finisher's gamma bit for bit) and forms alpha = rho / gamma itself; the
-// model's first block stores gamma and alpha for K3.  This saves the finisher
-// launch between K1 and K2.
 template <typename P, int K>
-__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part,
-                                                const int *blk_inst, KScal *sc, const double *part1, int nparts,
-                                                const int *inst_blk0) {
+__global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col, double *part, const int *blk_inst, KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    int m = 0;
-    if (blk_inst) { m = blk_inst[2 * blockIdx.x]; sc += m; nend = blk_inst[2 * blockIdx.x + 1]; }
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
-    __shared__ double s_alpha;
-    {
-        const int k0 = inst_blk0 ? inst_blk0[m] : 0, k1 = inst_blk0 ? inst_blk0[m + 1] : nparts;
-        const int op[1] = {0};
-        double g[1];
-        reduce_partials<1>(k0, k1, part1, op, g);
-        if (threadIdx.x == 0) {
-            const double alpha = sc->rho / g[0];
-            s_alpha = alpha;
-            if ((int)blockIdx.x == k0) { sc->gamma = g[0]; sc->alpha = alpha; }
-        }
-        __syncthreads();
-    }
     int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const T nalpha = (T)(-s_alpha);
+    const T nalpha = (T)(-sc->alpha);
     double v[3] = {0.0, 0.0, 0.0};
     if (n < nend) {
         VecS<T> sf{V.r, V.Ap, nalpha};
@@ -526,38 +508,16 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *se
     block_partials<3>(v, op, part);
 }
 
-// K3 also finishes K2's reduction: every block reduces K2's three block
-// partials of its model (part2, in the finisher's order) and forms omega,
-// the new rho and beta itself.  The model's first block stores omega and the
-// new rho in rho_next; FIN_K3 moves rho to rhop and rho_next to rho once no
-// K3 block reads the old rho any more.
 template <typename P>
-__global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc,
-                                                const double *part2, int nparts, const int *inst_blk0) {
+__global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc) {
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    int m = 0;
-    if (blk_inst) { m = blk_inst[2 * blockIdx.x]; sc += m; nend = blk_inst[2 * blockIdx.x + 1]; }
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (sc->done) return;
-    __shared__ double s_sc[2];
-    {
-        const int k0 = inst_blk0 ? inst_blk0[m] : 0, k1 = inst_blk0 ? inst_blk0[m + 1] : nparts;
-        const int op[3] = {0, 0, 0};
-        double o[3];
-        reduce_partials<3>(k0, k1, part2, op, o);
-        if (threadIdx.x == 0) {
-            const double omega = o[0] / o[1];
-            const double rho = -1.0 * omega * o[2];
-            s_sc[0] = omega;
-            s_sc[1] = (rho * sc->alpha) / (sc->rho * omega);   // beta, with rhop = the old rho
-            if ((int)blockIdx.x == k0) { sc->omega = omega; sc->rho_next = rho; }
-        }
-        __syncthreads();
-    }
     int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const double alpha = sc->alpha, omega = s_sc[0];
-    const double beta = s_sc[1];
+    const double alpha = sc->alpha, omega = sc->omega;
+    const double beta = (sc->rho * alpha) / (sc->rhop * omega);
     const T ta = (T)alpha, to = (T)omega, tno = (T)(-omega), tb = (T)beta;
     double v[1] = {0.0};
     if (n < nend) {
@@ -726,16 +686,16 @@ template <typename P, int K>
 static void iters_k(Ctx *c, int k) {
     int nb = nblocks_for(c->N, 256);
     KVecs<P> V = vecs<P>(c);
-    double *part1 = c->red.part + 3 * (size_t)c->red.maxblocks;   // K1's partials, reduced inside K2
     for (int i = 0; i < k; i++) {
         hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                           part1, BI(c), c->ksc);
+                           c->red.part, BI(c), c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
         hipLaunchKernelGGL((k_bcg_k2<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                           c->red.part, BI(c), c->ksc, part1, nb, IB0(c));
-        // K3 reduces K2's partials itself and writes its own to the (consumed) K1 slots
-        hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, part1, BI(c), c->ksc,
-                           (const double *)c->red.part, nb, IB0(c));
-        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, part1, c->ksc, IB0(c));
+                           c->red.part, BI(c), c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
+        hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
+                           c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     }
 }
 
@@ -743,7 +703,7 @@ template <typename P, int K>
 static void k1_only_k(Ctx *c) {
     int nb = nblocks_for(c->N, 256);
     hipLaunchKernelGGL((k_bcg_k1<P, K>), dim3(nb), dim3(256), 0, c->stream, c->N, vecs<P>(c), c->sell_ptr,
-                       c->sell_col, c->red.part + 3 * (size_t)c->red.maxblocks, BI(c), c->ksc);
+                       c->sell_col, c->red.part, BI(c), c->ksc);
 }
 
 // dispatch on the uniform slice width (0 = per-slice widths, looped)
@@ -792,10 +752,10 @@ static void pl_iters_k(Ctx *c, int k) {
         launch_band_solve(c, V.As, c->ksc);
         hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
-        double *part1 = c->red.part + 3 * (size_t)c->red.maxblocks;
-        hipLaunchKernelGGL(k_bcg_k3<PrecF64>, dim3(nb), dim3(256), 0, c->stream, c->N, V, part1, BI(c), c->ksc,
-                           (const double *)c->red.part, nb, IB0(c));
-        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, part1, c->ksc, IB0(c));
+        hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
+        hipLaunchKernelGGL(k_bcg_k3<PrecF64>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
+                           BI(c), c->ksc);
+        hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     }
 }
 
