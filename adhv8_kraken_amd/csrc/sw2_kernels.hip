@@ -270,49 +270,79 @@ __global__ void __launch_bounds__(256) k_split_count(int E, const int *e2d, cons
     }
 }
 
-// exclusive scans of the two block-count arrays (one block of 256 threads,
-// each owning a contiguous run of counts); counts[0] = wet total, counts[1] =
-// general total
-__global__ void __launch_bounds__(256) k_split_scan(int nb, int *bcount, int *counts) {
-    __shared__ int wsum[2][4];
-    const int per = (nb + 255) / 256, b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+// exclusive scans of the two block-count arrays (one block of 1024 threads,
+// each owning a contiguous run of up to SCAN_R counts of both arrays, held in
+// registers from one load to the write-back, so the kernel waits on memory
+// once); counts[0] = wet total, counts[1] = general total.  Longer runs
+// (more than 8,192 count blocks, i.e. over 2M triangles) loop over chunks.
+constexpr int SCAN_T = 1024, SCAN_R = 8;
+__device__ __forceinline__ void scan_pair(int s0, int s1, int &run0, int &run1, int &tot0, int &tot1) {
+    __shared__ int wsum[2][SCAN_T / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int a = 0; a < 2; a++) {
-        int *bc = bcount + a * nb;
-        int s = 0;
-        for (int b = b0; b < b1; b += 8) {  // 8 loads in flight per round trip
-            int v[8];
+    int x0 = s0, x1 = s1;   // inclusive scans within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y0 = __shfl_up(x0, o, 64), y1 = __shfl_up(x1, o, 64);
+        if (lane >= o) { x0 += y0; x1 += y1; }
+    }
+    if (lane == 63) { wsum[0][wid] = x0; wsum[1][wid] = x1; }
+    __syncthreads();
+    run0 = x0 - s0; run1 = x1 - s1; tot0 = 0; tot1 = 0;
+    for (int w = 0; w < SCAN_T / 64; w++) {
+        if (w < wid) { run0 += wsum[0][w]; run1 += wsum[1][w]; }
+        tot0 += wsum[0][w];
+        tot1 += wsum[1][w];
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_split_scan(int nb, int *bcount, int *counts) {
+    int *bc0 = bcount, *bc1 = bcount + nb;
+    const int per = (nb + SCAN_T - 1) / SCAN_T, b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
+    int run0, run1, tot0, tot1;
+    if (per <= SCAN_R) {
+        int v0[SCAN_R], v1[SCAN_R], s0 = 0, s1 = 0;
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = b + u < b1 ? bc[b + u] : 0;
-#pragma unroll
-            for (int u = 0; u < 8; u++) s += v[u];
+        for (int u = 0; u < SCAN_R; u++) {
+            v0[u] = b0 + u < b1 ? bc0[b0 + u] : 0;
+            v1[u] = b0 + u < b1 ? bc1[b0 + u] : 0;
         }
-        // inclusive scan of the per-thread sums: within the wave, then over the waves
-        int x = s;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[a][wid] = x;
-        __syncthreads();
-        int run = x - s, total = 0;
-        for (int w = 0; w < 4; w++) {
-            if (w < wid) run += wsum[a][w];
-            total += wsum[a][w];
-        }
-        if (threadIdx.x == 0) counts[a] = total;
-        for (int b = b0; b < b1; b += 8) {  // the run's exclusive prefixes, 8 counts per round trip
-            int v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = b + u < b1 ? bc[b + u] : 0;
+        for (int u = 0; u < SCAN_R; u++) { s0 += v0[u]; s1 += v1[u]; }
+        scan_pair(s0, s1, run0, run1, tot0, tot1);
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+        for (int u = 0; u < SCAN_R; u++)
+            if (b0 + u < b1) {
+                bc0[b0 + u] = run0; run0 += v0[u];
+                bc1[b0 + u] = run1; run1 += v1[u];
+            }
+    } else {
+        int s0 = 0, s1 = 0;
+        for (int b = b0; b < b1; b += SCAN_R) {
+            int v0[SCAN_R], v1[SCAN_R];
+#pragma unroll
+            for (int u = 0; u < SCAN_R; u++) {
+                v0[u] = b + u < b1 ? bc0[b + u] : 0;
+                v1[u] = b + u < b1 ? bc1[b + u] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < SCAN_R; u++) { s0 += v0[u]; s1 += v1[u]; }
+        }
+        scan_pair(s0, s1, run0, run1, tot0, tot1);
+        for (int b = b0; b < b1; b += SCAN_R) {
+            int v0[SCAN_R], v1[SCAN_R];
+#pragma unroll
+            for (int u = 0; u < SCAN_R; u++) {
+                v0[u] = b + u < b1 ? bc0[b + u] : 0;
+                v1[u] = b + u < b1 ? bc1[b + u] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < SCAN_R; u++)
                 if (b + u < b1) {
-                    bc[b + u] = run;
-                    run += v[u];
+                    bc0[b + u] = run0; run0 += v0[u];
+                    bc1[b + u] = run1; run1 += v1[u];
                 }
         }
     }
+    if (threadIdx.x == 0) { counts[0] = tot0; counts[1] = tot1; }
 }
 
 __global__ void __launch_bounds__(256) k_split_write(int E, const int *e2d, const double *head, const int *elem_inst,
@@ -343,7 +373,7 @@ static void launch_split(Ctx *c) {
     const int *mask = c->ninst ? c->inst_mask : nullptr;
     hipLaunchKernelGGL(k_split_count, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->elem_inst, mask,
                        c->esplit);
-    hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(256), 0, c->stream, nb, c->esplit, c->ecount);
+    hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(SCAN_T), 0, c->stream, nb, c->esplit, c->ecount);
     hipLaunchKernelGGL(k_split_write, dim3(nb), dim3(256), 0, c->stream, c->E, c->e2d, c->head, c->elem_inst, mask,
                        c->esplit, c->elist_wet, c->elist_gen);
     c->split_valid = 1;

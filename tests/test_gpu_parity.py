@@ -227,16 +227,17 @@ def test_newton_history(name):
 
 
 def test_large_mesh_jacobian_is_directional_derivative():
-    """At 540,000 triangles (2,110 blocks of 256 triangles, so each thread of
-    the list scan owns more than 8 block counts, and 4,236 assembly
-    workgroups) the assembled Jacobian applied to a random direction equals
-    -dR along it, by a central difference of the device residual
-    (fe_sw2_load.c:34-269 against fe_sw2_resid.c:29-216).  The element
-    blocks are first filled at a far-away state, so a triangle missing from
-    the element lists, or a slot the assembly skipped, keeps blocks that are
-    wrong by O(1) in its rows."""
+    """At 2.2M triangles (8,594 blocks of 256 triangles, so each of the 1,024
+    threads of the list scan owns more than 8 block counts: its chunked
+    path; 17,221 assembly workgroups) the assembled Jacobian applied to a
+    random direction equals -dR along it, by a central difference of the
+    device residual (fe_sw2_load.c:34-269 against fe_sw2_resid.c:29-216).
+    The element blocks are first filled at a far-away state, so a triangle
+    missing from the element lists, or a slot the assembly skipped, keeps
+    blocks that are wrong by O(1) in its rows.  (Meshes below 2.1M
+    triangles run the scan's register path: every other test.)"""
     import scipy.sparse
-    p = mesh.bumpy_channel(600, 450, seed=5)
+    p = mesh.bumpy_channel(1100, 1000, seed=5)
     N = p.nnodes
     say(f"large-mesh Jacobian: {p.nelems2d} triangles, {N} nodes")
     g = SW2Model(p)
