@@ -198,6 +198,18 @@ void launch_setup(Ctx *c) {
 // boundary publishes the partials; an in-kernel "last block" scheme would
 // need a device-scope fence per block, which on a multi-XCD part writes back
 // the XCD's L2 and costs more than the SpMV itself.
+// XCD-aware order of the 256-node blocks of the Krylov kernels: the hardware
+// deals consecutive workgroups to the 8 XCDs in turn, so workgroup b runs the
+// logical block that gives each XCD one contiguous range of blocks, and a
+// row's neighbour gathers (a mesh column away: a few blocks) hit that XCD's
+// own L2.  A bijection on [0, gridDim.x): partial slots, the reduction order
+// and every result are those of the plain order.
+__device__ __forceinline__ int lblock() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return x * q + min(x, r) + i;
+}
+
 template <int K>
 __device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K], double *part) {
     __shared__ double sh[K][256 / 64];
@@ -217,7 +229,7 @@ __device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K
         for (int k = 0; k < K; k++) {
             double x = sh[k][0];
             for (int w = 1; w < nw; w++) x = op[k] ? fmax(x, sh[k][w]) : x + sh[k][w];
-            part[K * blockIdx.x + k] = x;
+            part[K * lblock() + k] = x;
         }
     }
 }
@@ -412,8 +424,9 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double 
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
+    int n = lb * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < nend) {
         const MT *M = V.dinv + didx9(n, 0);
@@ -438,8 +451,9 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *se
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
+    int n = lb * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (n < nend) {
         T y[3];
@@ -463,9 +477,10 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *se
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = lb * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < nend) {
         T y[3];
@@ -485,9 +500,10 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *se
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = lb * blockDim.x + threadIdx.x;
     const T nalpha = (T)(-sc->alpha);
     double v[3] = {0.0, 0.0, 0.0};
     if (n < nend) {
@@ -513,9 +529,10 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part,
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = lb * blockDim.x + threadIdx.x;
     const double alpha = sc->alpha, omega = sc->omega;
     const double beta = (sc->rho * alpha) / (sc->rhop * omega);
     const T ta = (T)alpha, to = (T)omega, tno = (T)(-omega), tb = (T)beta;
@@ -575,8 +592,9 @@ __global__ void k_pl_copy(size_t n, const double *src, double *dst) {
 // u = b' (= M^-1 b), ||b'||inf
 __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
+    const int n = lb * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < nend)
         for (int j = 0; j < 3; j++) {
@@ -592,8 +610,9 @@ __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *
 // r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
 __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
+    const int n = lb * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (n < nend)
         for (int j = 0; j < 3; j++) {
@@ -612,9 +631,10 @@ __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *
 // gamma = (q, Ap), alpha = rho / gamma
 __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = lb * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (n < nend)
         for (int j = 0; j < 3; j++) v[0] += V.q[vidx3(n, j)] * V.Ap[vidx3(n, j)];
@@ -625,9 +645,10 @@ __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, doubl
 // omega = (As, s) / (As, As), rho' = -omega (As, q)
 __global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
+    const int lb = lblock();   // XCD-aware block order
+    if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = lb * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     if (n < nend)
         for (int j = 0; j < 3; j++) {
