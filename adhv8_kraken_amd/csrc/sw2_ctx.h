@@ -175,6 +175,17 @@ __host__ __device__ __forceinline__ size_t vidx3(int n, int k) {
 // entry e (0..8, row-major 3x3) of node n's diagonal block
 __host__ __device__ __forceinline__ size_t didx9(int n, int e) { return sell_vidx((size_t)n, e); }
 
+// XCD-aware workgroup order: the hardware deals consecutive workgroups to the
+// 8 XCDs in turn; workgroup b runs logical block xcd_block(), which gives each
+// XCD one contiguous range of blocks, so data that neighbouring blocks share
+// (a node's neighbours, a triangle's nodes) stays in that XCD's own L2.  A
+// bijection on [0, gridDim.x): which thread computes what changes, no result.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return x * q + min(x, r) + i;
+}
+
 // launchers (sw2_kernels.hip / sw2_solver.hip)
 void launch_old_terms(Ctx *c);
 void launch_resid(Ctx *c);

@@ -35,7 +35,7 @@ template <bool WET, int BS>
 __global__ void __launch_bounds__(BS) k_resid2d_list(ElemData D, const int *list, const int *count, double *ER) {
     const int n = *count;
     const int stride = gridDim.x * blockDim.x;
-    int li = blockIdx.x * blockDim.x + threadIdx.x;
+    int li = xcd_block() * blockDim.x + threadIdx.x;   // neighbouring triangles share nodes: same XCD
     int ie_a = li < n ? list[li] : 0;
     int ie_b = li + stride < n ? list[li + stride] : 0;
     int nd_a[3] = {0, 0, 0};
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(128) k_resid1d(ElemData D, int B, double *ER1)
 __global__ void k_resid_gather(int N, const int *n2e_ptr, const int *n2e, const int *n2b_ptr, const int *n2b,
                                const double *ER, const double *ER1, const double *old_head, double *u, double *v,
                                double *R) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = xcd_block() * blockDim.x + threadIdx.x;   // a node's elements are shared with its neighbours
     if (n >= N) return;
     bool dry = old_head[n] <= 0.;
     double rx = 0.0, ry = 0.0, rc = 0.0;
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D
                                                                       double *EM, float *EM32) {
     const int nt = 18 * *count;
     const int stride = gridDim.x * blockDim.x;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;   // neighbouring triangles share nodes: same XCD
     // Grid-stride over (triangle, column, sign); the stride is a multiple of
     // the wavefront, so both lanes of a pair run and leave together.
     // Software pipeline: the triangle id two iterations ahead and the node ids

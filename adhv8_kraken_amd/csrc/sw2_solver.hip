@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double
                                                      const int *sell_ptr, const int *sell_col, const double *sv,
                                                      double *R, double *sol, double *dinv, float *diag32,
                                                      float *offd32, float *dinv32) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = xcd_block() * blockDim.x + threadIdx.x;   // neighbours' scales stay in this XCD's L2
     if (n >= N) return;
     double rf[3] = {sv[vidx3(n, 0)], sv[vidx3(n, 1)], sv[vidx3(n, 2)]};
     for (int j = 0; j < 3; j++) {
@@ -198,17 +198,11 @@ void launch_setup(Ctx *c) {
 // boundary publishes the partials; an in-kernel "last block" scheme would
 // need a device-scope fence per block, which on a multi-XCD part writes back
 // the XCD's L2 and costs more than the SpMV itself.
-// XCD-aware order of the 256-node blocks of the Krylov kernels: the hardware
-// deals consecutive workgroups to the 8 XCDs in turn, so workgroup b runs the
-// logical block that gives each XCD one contiguous range of blocks, and a
-// row's neighbour gathers (a mesh column away: a few blocks) hit that XCD's
-// own L2.  A bijection on [0, gridDim.x): partial slots, the reduction order
-// and every result are those of the plain order.
-__device__ __forceinline__ int lblock() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
-    return x * q + min(x, r) + i;
-}
+// The Krylov kernels run their 256-node blocks in XCD-aware order (xcd_block):
+// each XCD gets a contiguous range of blocks, so a row's neighbour gathers (a
+// mesh column away: a few blocks) hit that XCD's own L2.  Partial slots, the
+// reduction order and every result are those of the plain order.
+__device__ __forceinline__ int lblock() { return xcd_block(); }
 
 template <int K>
 __device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K], double *part) {
