@@ -198,14 +198,16 @@ void launch_setup(Ctx *c) {
 // boundary publishes the partials; an in-kernel "last block" scheme would
 // need a device-scope fence per block, which on a multi-XCD part writes back
 // the XCD's L2 and costs more than the SpMV itself.
-// The Krylov kernels run their 256-node blocks in XCD-aware order (xcd_block):
-// each XCD gets a contiguous range of blocks, so a row's neighbour gathers (a
-// mesh column away: a few blocks) hit that XCD's own L2.  Partial slots, the
-// reduction order and every result are those of the plain order.
-__device__ __forceinline__ int lblock() { return xcd_block(); }
+// A single model's Krylov kernels run their 256-node blocks in XCD-aware
+// order (xcd_block): each XCD gets a contiguous range of blocks, so a row's
+// neighbour gathers (a mesh column away: a few blocks) hit that XCD's own L2.
+// Ensembles keep the plain order, which spreads the blocks of the models still
+// iterating evenly over the XCDs.  Partial slots, the reduction order and
+// every result are those of the plain order.
+__device__ __forceinline__ int lblock(const int *blk_inst) { return blk_inst ? (int)blockIdx.x : xcd_block(); }
 
 template <int K>
-__device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K], double *part) {
+__device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K], double *part, int lb) {
     __shared__ double sh[K][256 / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
@@ -223,7 +225,7 @@ __device__ __forceinline__ void block_partials(double (&v)[K], const int (&op)[K
         for (int k = 0; k < K; k++) {
             double x = sh[k][0];
             for (int w = 1; w < nw; w++) x = op[k] ? fmax(x, sh[k][w]) : x + sh[k][w];
-            part[K * lblock() + k] = x;
+            part[K * lb + k] = x;
         }
     }
 }
@@ -418,7 +420,7 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double 
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     int n = lb * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -436,7 +438,7 @@ __global__ void __launch_bounds__(256) k_bcg_p1(int N, KVecs<P> V, const double 
         }
     }
     const int op[1] = {1};
-    block_partials<1>(v, op, part);
+    block_partials<1>(v, op, part, lb);
 }
 
 // prologue part 2: r = b' - M^-1 A u, q = r, p = r, rho = (r,q), ||r||inf  (:138-188)
@@ -445,7 +447,7 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *se
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     int n = lb * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
@@ -463,7 +465,7 @@ __global__ void __launch_bounds__(256) k_bcg_p2(int N, KVecs<P> V, const int *se
         }
     }
     const int op[2] = {1, 0};
-    block_partials<2>(v, op, part);
+    block_partials<2>(v, op, part, lb);
 }
 
 template <typename P, int K>
@@ -471,7 +473,7 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *se
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
     int n = lb * blockDim.x + threadIdx.x;
@@ -486,7 +488,7 @@ __global__ void __launch_bounds__(256) k_bcg_k1(int N, KVecs<P> V, const int *se
         }
     }
     const int op[1] = {0};
-    block_partials<1>(v, op, part);
+    block_partials<1>(v, op, part, lb);
 }
 
 template <typename P, int K>
@@ -494,7 +496,7 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *se
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
     int n = lb * blockDim.x + threadIdx.x;
@@ -515,7 +517,7 @@ __global__ void __launch_bounds__(256) k_bcg_k2(int N, KVecs<P> V, const int *se
         }
     }
     const int op[3] = {0, 0, 0};
-    block_partials<3>(v, op, part);
+    block_partials<3>(v, op, part, lb);
 }
 
 template <typename P>
@@ -523,7 +525,7 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part,
     using T = typename P::V;
     using MT = typename P::M;
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
     int n = lb * blockDim.x + threadIdx.x;
@@ -549,7 +551,7 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part,
         }
     }
     const int op[1] = {1};
-    block_partials<1>(v, op, part);
+    block_partials<1>(v, op, part, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,7 +588,7 @@ __global__ void k_pl_copy(size_t n, const double *src, double *dst) {
 // u = b' (= M^-1 b), ||b'||inf
 __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     const int n = lb * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -598,13 +600,13 @@ __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *
             v[0] = fmax(v[0], fabs(y));
         }
     const int op[1] = {1};
-    block_partials<1>(v, op, part);
+    block_partials<1>(v, op, part, lb);
 }
 
 // r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
 __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     const int n = lb * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
@@ -619,13 +621,13 @@ __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *
             v[1] += r * r;
         }
     const int op[2] = {1, 0};
-    block_partials<2>(v, op, part);
+    block_partials<2>(v, op, part, lb);
 }
 
 // gamma = (q, Ap), alpha = rho / gamma
 __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
     const int n = lb * blockDim.x + threadIdx.x;
@@ -633,13 +635,13 @@ __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, doubl
     if (n < nend)
         for (int j = 0; j < 3; j++) v[0] += V.q[vidx3(n, j)] * V.Ap[vidx3(n, j)];
     const int op[1] = {0};
-    block_partials<1>(v, op, part);
+    block_partials<1>(v, op, part, lb);
 }
 
 // omega = (As, s) / (As, As), rho' = -omega (As, q)
 __global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
-    const int lb = lblock();   // XCD-aware block order
+    const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
     if (sc->done) return;
     const int n = lb * blockDim.x + threadIdx.x;
@@ -653,7 +655,7 @@ __global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, doubl
             v[2] += y * V.q[i];
         }
     const int op[3] = {0, 0, 0};
-    block_partials<3>(v, op, part);
+    block_partials<3>(v, op, part, lb);
 }
 
 // solv_return_old (src/solver/solv_return_old.c:23-30): sol = (u + u0) * scale,
