@@ -35,7 +35,7 @@ template <bool WET, int BS>
 __global__ void __launch_bounds__(BS) k_resid2d_list(ElemData D, const int *list, const int *count, double *ER) {
     const int n = *count;
     const int stride = gridDim.x * blockDim.x;
-    int li = xcd_block() * blockDim.x + threadIdx.x;   // neighbouring triangles share nodes: same XCD
+    int li = blockIdx.x * blockDim.x + threadIdx.x;
     int ie_a = li < n ? list[li] : 0;
     int ie_b = li + stride < n ? list[li + stride] : 0;
     int nd_a[3] = {0, 0, 0};
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D
                                                                       double *EM, float *EM32) {
     const int nt = 18 * *count;
     const int stride = gridDim.x * blockDim.x;
-    int t = xcd_block() * blockDim.x + threadIdx.x;   // neighbouring triangles share nodes: same XCD
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
     // Grid-stride over (triangle, column, sign); the stride is a multiple of
     // the wavefront, so both lanes of a pair run and leave together.
     // Software pipeline: the triangle id two iterations ahead and the node ids

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double
                                                      const int *sell_ptr, const int *sell_col, const double *sv,
                                                      double *R, double *sol, double *dinv, float *diag32,
                                                      float *offd32, float *dinv32) {
-    int n = xcd_block() * blockDim.x + threadIdx.x;   // neighbours' scales stay in this XCD's L2
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     double rf[3] = {sv[vidx3(n, 0)], sv[vidx3(n, 1)], sv[vidx3(n, 2)]};
     for (int j = 0; j < 3; j++) {
