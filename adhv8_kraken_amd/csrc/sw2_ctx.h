@@ -177,13 +177,24 @@ __host__ __device__ __forceinline__ size_t didx9(int n, int e) { return sell_vid
 
 // XCD-aware workgroup order: the hardware deals consecutive workgroups to the
 // 8 XCDs in turn; workgroup b runs logical block xcd_block(), which gives each
-// XCD one contiguous range of blocks, so data that neighbouring blocks share
-// (a node's neighbours, a triangle's nodes) stays in that XCD's own L2.  A
-// bijection on [0, gridDim.x): which thread computes what changes, no result.
+// XCD contiguous chunks of XCD_CHUNK blocks (in every run of 8 * XCD_CHUNK
+// blocks, one chunk per XCD; a short tail is split contiguously), so data that
+// neighbouring blocks share (a node's neighbours, a row's gathered columns a
+// mesh column away) stays in that XCD's own L2 while all XCDs still stream
+// the same region of memory.  A bijection on [0, gridDim.x): which workgroup
+// computes what changes, no result.
+constexpr int XCD_CHUNK = 64;
 __device__ __forceinline__ int xcd_block() {
+    constexpr int C = XCD_CHUNK;
     const int nb = gridDim.x, b = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
-    return x * q + min(x, r) + i;
+    const int full = (nb / (8 * C)) * (8 * C);
+    if (b < full) {
+        const int sb = b / (8 * C), w = b % (8 * C);
+        return sb * 8 * C + (w & 7) * C + (w >> 3);
+    }
+    const int rem = nb - full, bb = b - full;
+    const int q = rem >> 3, r = rem & 7, x = bb & 7, i = bb >> 3;
+    return full + x * q + min(x, r) + i;
 }
 
 // launchers (sw2_kernels.hip / sw2_solver.hip)
