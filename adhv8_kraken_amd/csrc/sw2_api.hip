@@ -813,6 +813,17 @@ static int run_iters(Ctx &c, int k) {
     return 0;
 }
 
+// Krylov chunk lengths between host convergence checks: the first from the
+// previous solve's count (opts.check_every for a model's first solve), then
+// `more` at a time; with graph replay both are multiples of 4, so few graphs
+// are captured.
+static void chunk_plan(const Ctx &c, int &first, int &more) {
+    const bool graphs = c.opts.graphs != 0;
+    if (c.last_lin_it <= 0) first = c.opts.check_every;
+    else first = graphs ? ((c.last_lin_it + 3) / 4) * 4 : c.last_lin_it;
+    more = graphs ? 4 : 2;
+}
+
 // zero_guess: sol holds a zero initial guess (the Newton step zeroes it), so
 // the epilogue is idempotent and is queued behind every chunk before the
 // convergence flag is read; otherwise it runs once, after the last chunk.
@@ -823,15 +834,15 @@ static int krylov(Ctx &c, int *lin_it, int *verdict, bool zero_guess) {
     init.max_it = lin_it_cap(c);
     HIPCHK(hipMemcpyAsync(c.ksc, &init, sizeof(KScal), hipMemcpyHostToDevice, c.stream));
     launch_krylov_prologue(&c);
-    // first chunk: the previous solve's iteration count rounded up to a
-    // multiple of 4 (consecutive Newton systems converge alike), then chunks
-    // of 4; chunk lengths stay in a small set so few graphs are captured.
-    // The device flag makes iterations launched past convergence return at
-    // entry, so the first chunk follows the prologue without a host check.
-    int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
+    // first chunk: the previous solve's iteration count (consecutive Newton
+    // systems converge alike), then chunks of 2 (chunk_plan).  The device flag
+    // makes iterations launched past convergence return at entry, so the
+    // first chunk follows the prologue without a host check.
+    int first, more;
+    chunk_plan(c, first, more);
     int left = lin_it_cap(c);
     for (int chunk = 0;; chunk++) {
-        int want = chunk == 0 ? first : 4;
+        int want = chunk == 0 ? first : more;
         int k = std::max(1, std::min(want, left));
         int rc = run_iters(c, k);
         if (rc) return rc;
@@ -1128,11 +1139,12 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
     // chunks as in krylov() with a zero guess (newton_batch zeroes sol): no
     // host check between the prologue and the first chunk, the idempotent
     // epilogue queued behind every chunk before the flags are read
-    int first = c.last_lin_it > 0 ? ((c.last_lin_it + 3) / 4) * 4 : c.opts.check_every;
+    int first, more;
+    chunk_plan(c, first, more);
     int maxit = 0;
     int left = cap;
     for (int chunk = 0;; chunk++) {
-        int want = chunk == 0 ? first : 4;
+        int want = chunk == 0 ? first : more;
         int rc = run_iters(c, std::max(1, std::min(want, left)));
         if (rc) return rc;
         launch_krylov_epilogue(&c, true);

@@ -98,7 +98,8 @@ typedef struct {
     int preconditioner;  /* 2: 3x3 block Jacobi; 1: profile LU of node blocks (OP PRE 1 with OP BLK
                             nblock, src/solver/solv_blk_*.c), FP64 only */
     int check_every;     /* Krylov iterations before the first host convergence check of a solve that has
-                            no previous count (later solves start from the previous count, rounded up to 4) */
+                            no previous count (later solves start from the previous count, then check every
+                            2; with graph replay both are rounded up to multiples of 4) */
     int nblock;          /* OP BLK: node blocks of the profile-LU preconditioner (default 1) */
     int graphs;          /* 1: replay captured HIP graphs of Krylov iteration chunks; 0 (default): launch eagerly */
 } adh_sw2_opts;
