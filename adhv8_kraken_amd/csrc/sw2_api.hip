@@ -684,7 +684,13 @@ extern "C" int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual) {
 // off-diagonal blocks between the reference's row order (host) and SELL-64 (device)
 static int offd_to_host(Ctx &c, double *out) {
     std::vector<double> sv(9 * c.nsell);
-    HIPCHK(hipMemcpy(sv.data(), c.offd, sizeof(double) * sv.size(), hipMemcpyDeviceToHost));
+    if (c.mat32) {   // refined forms: the FP32 matrix (unscaled until the set-up scales it in place)
+        std::vector<float> f(sv.size());
+        HIPCHK(hipMemcpy(f.data(), c.offd32, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); i++) sv[i] = (double)f[i];
+    } else {
+        HIPCHK(hipMemcpy(sv.data(), c.offd, sizeof(double) * sv.size(), hipMemcpyDeviceToHost));
+    }
     for (int p = 0; p < c.nnz; p++)
         for (int e = 0; e < 9; e++) out[9 * (size_t)p + e] = sv[sell_vidx((size_t)c.h_offd_pos[p], e)];
     return 0;
@@ -881,6 +887,7 @@ extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, co
     if ((rc = nodes_from_host(c, c.diag, diagonal, 9))) return rc;
     if (c.nnz && (rc = offd_from_host(c, offdiag))) return rc;
     c.scale_valid = 0;
+    c.mat32 = 0;   // the system is now the FP64 one given here
     if ((rc = nodes_from_host(c, c.R, rhs, 3))) return rc;
     HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.Np));
     return 0;

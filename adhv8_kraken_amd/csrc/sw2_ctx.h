@@ -73,6 +73,7 @@ struct Ctx {
     int *slot_ptr, *slot_src;  // N diagonal slots (node order) then one slot per SELL position
     int scale_valid;           // the assembly computed `scale` for the current matrix
     int em32;                  // EM holds FP32 blocks (the last Jacobian ran in a refined form)
+    int mat32;                 // the assembled off-diagonal blocks are in offd32 (unscaled until set-up)
     int *blkpos;               // element block (9e+3i+j, then 9E+4b+2i+j) -> contributor position
     std::vector<int> h_blkpos;
     std::vector<int> h_row_ptr, h_col;

@@ -496,9 +496,11 @@ __device__ __forceinline__ void sum_run(const int *slot_ptr, size_t slot, const 
     }
 }
 
-template <typename TM>
+// TM: element-block type; TO: off-diagonal output type (float for the refined
+// forms, which keep only an FP32 matrix: the set-up scales it in place).
+template <typename TM, typename TO>
 __global__ void __launch_bounds__(1024) k_assemble(int N, const int *slot_ptr, const int *row_ptr,
-                                                   const int *sell_ptr, const TM *EM, double *diag, double *offd,
+                                                   const int *sell_ptr, const TM *EM, double *diag, TO *offd,
                                                    double *R, const double *u, const double *v, double *sv) {
     __shared__ double smax[16][3][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -534,7 +536,7 @@ __global__ void __launch_bounds__(1024) k_assemble(int N, const int *slot_ptr, c
                 sum_run(slot_ptr, (size_t)N + q, EM, a);
 #pragma unroll
                 for (int k = 0; k < 9; k++) {
-                    offd[sell_vidx(q, k)] = a[k];
+                    offd[sell_vidx(q, k)] = (TO)a[k];
                     const double x = fabs(a[k]);
                     if (x > s[k / 3]) s[k / 3] = x;
                 }
@@ -559,14 +561,20 @@ __global__ void __launch_bounds__(1024) k_assemble(int N, const int *slot_ptr, c
 
 void launch_assemble(Ctx *c) {
     const int nw = 1 + std::min(std::max(c->kmax_row, 1), 15);   // waves per 64-row slice
-    if (c->em32)   // element blocks stored in FP32 by the refined forms, summed in FP64
-        hipLaunchKernelGGL(k_assemble<float>, dim3(nblocks_for(c->N, 64)), dim3(64 * nw), 0, c->stream, c->N,
-                           c->slot_ptr, c->row_ptr, c->sell_ptr, (const float *)c->EM, c->diag, c->offd, c->R, c->u,
-                           c->v, c->scale);
+    // refined forms with block Jacobi: element blocks stored in FP32, summed in
+    // FP64, off-diagonal blocks written in FP32 (unscaled) straight into the
+    // matrix the FP32 Krylov reads; otherwise FP64 throughout
+    c->mat32 = c->em32 && c->opts.preconditioner != 1;
+    dim3 g(nblocks_for(c->N, 64)), b(64 * nw);
+    if (c->mat32)
+        hipLaunchKernelGGL((k_assemble<float, float>), g, b, 0, c->stream, c->N, c->slot_ptr, c->row_ptr, c->sell_ptr,
+                           (const float *)c->EM, c->diag, c->offd32, c->R, c->u, c->v, c->scale);
+    else if (c->em32)
+        hipLaunchKernelGGL((k_assemble<float, double>), g, b, 0, c->stream, c->N, c->slot_ptr, c->row_ptr,
+                           c->sell_ptr, (const float *)c->EM, c->diag, c->offd, c->R, c->u, c->v, c->scale);
     else
-        hipLaunchKernelGGL(k_assemble<double>, dim3(nblocks_for(c->N, 64)), dim3(64 * nw), 0, c->stream, c->N,
-                           c->slot_ptr, c->row_ptr, c->sell_ptr, (const double *)c->EM, c->diag, c->offd, c->R, c->u,
-                           c->v, c->scale);
+        hipLaunchKernelGGL((k_assemble<double, double>), g, b, 0, c->stream, c->N, c->slot_ptr, c->row_ptr,
+                           c->sell_ptr, (const double *)c->EM, c->diag, c->offd, c->R, c->u, c->v, c->scale);
     c->scale_valid = 1;
 }
 

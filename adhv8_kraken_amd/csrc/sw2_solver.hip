@@ -94,7 +94,9 @@ __device__ __forceinline__ void inv3(const double (&d)[9], double (&inv)[9]) {
 // preconditioner): no inverses.
 // K > 0: every slice has K slots (padding slots hold zeros and a valid
 // column), run as a fully unrolled loop with all column scales gathered first.
-template <bool F32, bool BJ, int K>
+// SRC32: the unscaled off-diagonal blocks are already in offd32 (the refined
+// forms' assembly writes them there) and are scaled in place.
+template <bool F32, bool BJ, int K, bool SRC32 = false>
 __global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double *offd, const int *row_ptr,
                                                      const int *sell_ptr, const int *sell_col, const double *sv,
                                                      double *R, double *sol, double *dinv, float *diag32,
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(256) k_scale_apply(int N, double *diag, double
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const size_t i = sell_vidx(q, 3 * j + k);
-                const double v = offd[i] * (rf[j] * sc[k]);
+                const double v = (SRC32 ? (double)offd32[i] : offd[i]) * (rf[j] * sc[k]);
                 if constexpr (F32) offd32[i] = (float)v;
                 else offd[i] = v;
             }
@@ -162,10 +164,17 @@ __global__ void k_to_f32(size_t n, const double *a, float *b) {
 }
 
 // k_scale_apply dispatched on the uniform slice width (0 = per-row lengths)
-#define SW2_SCALE_LAUNCH(F32, BJ, K)                                                                          \
-    hipLaunchKernelGGL((k_scale_apply<F32, BJ, K>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, \
-                       c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv, c->diag32,       \
-                       c->offd32, c->dinv32)
+#define SW2_SCALE_LAUNCH(F32, BJ, K)                                                                               \
+    do {                                                                                                            \
+        if (F32 && c->mat32)                                                                                        \
+            hipLaunchKernelGGL((k_scale_apply<F32, BJ, K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, \
+                               c->offd, c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol, c->dinv,      \
+                               c->diag32, c->offd32, c->dinv32);                                                    \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_scale_apply<F32, BJ, K, false>), dim3(nb), dim3(256), 0, c->stream, c->N,         \
+                               c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col, c->scale, c->R, c->sol,      \
+                               c->dinv, c->diag32, c->offd32, c->dinv32);                                           \
+    } while (0)
 #define SW2_SCALE_SWITCH(F32, BJ)                           \
     switch (c->kuni) {                                      \
         case 5: SW2_SCALE_LAUNCH(F32, BJ, 5); break;        \

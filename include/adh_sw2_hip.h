@@ -149,7 +149,8 @@ int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag);
 int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual);
 /* fe_sw2_load (src/fe/sw2/fe_sw2_load.c:34-269) incl. the diagonal floor;
  * diagonal[9N], offdiag[9*nnz] and residual[3N] (after the floor's residual
- * tweak) optional. */
+ * tweak) optional.  The refined forms (precision >= 1, block Jacobi) keep the
+ * off-diagonal blocks in FP32: offdiag then holds their FP32 values. */
 int adh_sw2_hip_load(adh_sw2_hip *h, double *diagonal, double *offdiag, double *residual);
 /* Element Jacobians as assembled (after Dirichlet edits): [81*E] in block order
  * (row node i, col node j) -> 9 values; for parity tests.  The refined forms
