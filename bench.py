@@ -313,8 +313,9 @@ def main():
     if args.steps is None:
         args.steps = 10
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described in "
-              f"DESIGN.md but not run here", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path is built and measured on one "
+              f"GPU; sharding across GPUs (domain decomposition, or instances for ensembles) is not built",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload != "channel":
         ensemble_mode(args)
