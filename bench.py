@@ -7,7 +7,7 @@ one Newton iteration (FD Jacobian + assembly + scaling + block-Jacobi
 BiCGSTAB + update + residual), iterated to 1e-8 relative residual within each
 time step.  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line (the driver's contract) with the FP32-refined form as
+Prints one JSON line with the FP32-refined form as
 `value`, the FP64 form beside it, the SpMV roofline (HIP-event timed inside
 the library on its stream) and the CPU baseline (the oracle restatement,
 timed on this host on a bounded sample).
