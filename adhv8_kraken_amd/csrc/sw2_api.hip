@@ -1029,8 +1029,7 @@ static int newton(Ctx &c, adh_sw2_report *rep, int *verdict) {
         // FP32-refined forms: an unconverged inner solve is still a
         // correction; the FP64 residual tests and the line search decide.
         solv_flag = (o.precision >= 1) ? ADH_SW2_YES : vd;
-        launch_inc(&c, 1.0);
-        launch_scale_sol(&c, -1.0);
+        launch_inc_flip(&c);   // state += sol; sol = -sol for the line search
         double old_l2 = nm.l2;
         launch_resid(&c);
         launch_norms(&c);
@@ -1206,8 +1205,7 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
         for (int i = 0; i < n; i++)
             if (active[i]) solv[i] = (o.precision >= 1) ? ADH_SW2_YES : vd[i];
         if ((rc = set_mask(c, active))) return rc;
-        launch_inc(&c, 1.0);
-        launch_scale_sol(&c, -1.0);
+        launch_inc_flip(&c);   // state += sol; sol = -sol for the line search
         std::vector<double> old_l2(n);
         for (int i = 0; i < n; i++) old_l2[i] = nm[i].l2;
         launch_resid(&c);

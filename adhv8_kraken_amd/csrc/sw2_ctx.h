@@ -207,6 +207,7 @@ void launch_init(Ctx *c, int *bc_mask_dev);
 void launch_shift(Ctx *c);
 void launch_inc(Ctx *c, double factor_on_sol);   // state += sol
 void launch_scale_sol(Ctx *c, double f);          // sol *= f
+void launch_inc_flip(Ctx *c);                     // state += sol, then sol = -sol
 void launch_norms(Ctx *c);
 void launch_setup(Ctx *c);
 void launch_krylov_prologue(Ctx *c);

@@ -707,6 +707,28 @@ void launch_inc(Ctx *c, double) {
                        c->v, c->node_inst, c->ninst ? c->inst_mask : nullptr);
 }
 
+// fe_sw2_inc, then sol = -sol (what the line search steps back along), in one
+// pass over the nodes: the same operations as k_inc followed by k_scale_vec(-1)
+__global__ void k_inc_flip(int N, double *sol, double *h, double *u, double *v, const int *node_inst,
+                           const int *mask) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    if (mask && (node_inst[n] < 0 || !mask[node_inst[n]])) return;  // gap nodes (-1) never change
+    const double s0 = sol[vidx3(n, 0)], s1 = sol[vidx3(n, 1)], s2 = sol[vidx3(n, 2)];
+    u[n] += 1. * s0;
+    v[n] += 1. * s1;
+    h[n] += 1. * s2;
+    sol[vidx3(n, 0)] = s0 * -1.0;
+    sol[vidx3(n, 1)] = s1 * -1.0;
+    sol[vidx3(n, 2)] = s2 * -1.0;
+}
+
+void launch_inc_flip(Ctx *c) {
+    c->split_valid = 0;
+    hipLaunchKernelGGL(k_inc_flip, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->sol, c->head, c->u,
+                       c->v, c->node_inst, c->ninst ? c->inst_mask : nullptr);
+}
+
 // x *= f over a sliced node vector; entry i belongs to node 64 (i / 192) + i mod 64
 __global__ void k_scale_vec(size_t n, double *x, double f, const int *node_inst, const int *mask) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
