@@ -555,6 +555,24 @@ extern "C" int adh_sw2_hip_set_wind(adh_sw2_hip *h, const double *stress_xy) {
     return 0;
 }
 
+// Series values as the reference averages them over [t, t + dt] before each
+// attempt (sseries_setall_ts_valuesAVG, src/main/adh_func.c:542) and reads them
+// in the residuals (sseries_get_value: fe_sw2_boundary_resid.c:212-214, 372;
+// fe_sw2_body_resid.c:728-754).  Both kernels read them on every call, so the
+// upload is all that changes.
+static int upload_series(Ctx &c, int e0, int ne, int b0, int nb, const double *e1d_val, const double *e2d_src) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (e1d_val && nb) HIPCHK(hipMemcpy(c.val1d + b0, e1d_val, sizeof(double) * nb, hipMemcpyHostToDevice));
+    if (e2d_src && ne) HIPCHK(hipMemcpy(c.src + e0, e2d_src, sizeof(double) * ne, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int adh_sw2_hip_set_series(adh_sw2_hip *h, const double *e1d_val, const double *e2d_src) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    return upload_series(h->c, 0, h->c.E, 0, h->c.B, e1d_val, e2d_src);
+}
+
 extern "C" int adh_sw2_hip_geometry(adh_sw2_hip *h, double *djac, double *grad, double *djac1d, double *nrml1d) {
     if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
@@ -765,6 +783,12 @@ extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) 
     if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
+    // The set-up scales A, b and x in place (as spv_jacobi does).  The
+    // refined forms keep the scaled off-diagonal blocks only in FP32, so a
+    // second scaling of the same system would start from stale FP64 blocks:
+    // refuse it; load or set_system gives the set-up a fresh system.
+    if (c.sys_scaled)
+        return fail(ADH_SW2_EINVAL, "linear_sys_setup: the system is already scaled (call load or set_system first)");
     int rc0 = ensure_band(c);
     if (rc0) return rc0;
     launch_setup(&c);
@@ -887,6 +911,7 @@ extern "C" int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, co
     if ((rc = nodes_from_host(c, c.diag, diagonal, 9))) return rc;
     if (c.nnz && (rc = offd_from_host(c, offdiag))) return rc;
     c.scale_valid = 0;
+    c.sys_scaled = 0;
     c.mat32 = 0;   // the system is now the FP64 one given here
     if ((rc = nodes_from_host(c, c.R, rhs, 3))) return rc;
     HIPCHK(hipMemset(c.sol, 0, sizeof(double) * 3 * c.Np));
@@ -1399,6 +1424,14 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
     }
     c.h_inst_node0 = node0;
     c.h_inst_N = inst_nodes;
+    c.h_inst_e0 = E0;
+    c.h_inst_b0 = B0;
+    c.h_inst_E.resize(n);
+    c.h_inst_B.resize(n);
+    for (int i = 0; i < n; i++) {
+        c.h_inst_E[i] = meshes[i].nelems2d;
+        c.h_inst_B[i] = meshes[i].nelems1d;
+    }
     c.h_inst_dt = dts;
     c.h_inst_tol.assign(n, std::nan(""));
     c.ninst = n;
@@ -1487,6 +1520,13 @@ extern "C" int adh_sw2_hip_batch_set_wind(adh_sw2_hip *h, int i, const double *s
     HIPCHK(hipStreamSynchronize(c.stream));
     HIPCHK(hipMemcpy(c.wind + 2 * (size_t)a, stress_xy, sizeof(double) * 2 * (size_t)nn, hipMemcpyHostToDevice));
     return 0;
+}
+
+extern "C" int adh_sw2_hip_batch_set_series(adh_sw2_hip *h, int i, const double *e1d_val, const double *e2d_src) {
+    int a, nn, rc;
+    if ((rc = model_range(h, i, &a, &nn))) return rc;
+    Ctx &c = h->c;
+    return upload_series(c, c.h_inst_e0[i], c.h_inst_E[i], c.h_inst_b0[i], c.h_inst_B[i], e1d_val, e2d_src);
 }
 
 extern "C" int adh_sw2_hip_batch_set_tol(adh_sw2_hip *h, int i, double tol_nonlin) {

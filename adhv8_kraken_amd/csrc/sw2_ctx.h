@@ -74,6 +74,7 @@ struct Ctx {
     int scale_valid;           // the assembly computed `scale` for the current matrix
     int em32;                  // EM holds FP32 blocks (the last Jacobian ran in a refined form)
     int mat32;                 // the assembled off-diagonal blocks are in offd32 (unscaled until set-up)
+    int sys_scaled;            // the set-up has scaled the current system in place (cleared by load/set_system)
     int *blkpos;               // element block (9e+3i+j, then 9E+4b+2i+j) -> contributor position
     std::vector<int> h_blkpos;
     std::vector<int> h_row_ptr, h_col;
@@ -142,6 +143,7 @@ struct Ctx {
     double *dt_inst;
     int *loc_bc_inst;
     std::vector<int> h_inst_node0, h_inst_N;
+    std::vector<int> h_inst_e0, h_inst_E, h_inst_b0, h_inst_B;   // per-model triangle / edge ranges
     std::vector<double> h_inst_dt;
     std::vector<double> h_inst_tol;   // per-model nonlinear tolerance (NaN: opts.tol_nonlin)
     // captured BiCGSTAB iteration chunks (k iterations each), replayed with

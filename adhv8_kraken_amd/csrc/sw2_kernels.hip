@@ -576,6 +576,7 @@ void launch_assemble(Ctx *c) {
         hipLaunchKernelGGL((k_assemble<double, double>), g, b, 0, c->stream, c->N, c->slot_ptr, c->row_ptr,
                            c->sell_ptr, (const double *)c->EM, c->diag, c->offd, c->R, c->u, c->v, c->scale);
     c->scale_valid = 1;
+    c->sys_scaled = 0;
 }
 
 void launch_jac2d_only(Ctx *c) {

@@ -190,6 +190,7 @@ void launch_setup(Ctx *c) {
         hipLaunchKernelGGL(k_scale_rows, dim3(nb), dim3(256), 0, c->stream, c->N, c->diag, c->offd, c->row_ptr,
                            c->sell_ptr, c->scale);
     c->scale_valid = 0;    // the scaling below consumes it (and rescales the FP64 matrix in place)
+    c->sys_scaled = 1;
     if (c->opts.preconditioner == 1) {  // profile LU of the node blocks (FP64)
         SW2_SCALE_SWITCH(false, false)
         launch_band_setup(c);

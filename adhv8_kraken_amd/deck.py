@@ -30,7 +30,7 @@ class _XY:
         self.x, self.y = xyz[:, 0], xyz[:, 1]
 
 
-NOT_QUITE_SMALL = 1.0e-6  # src/include/define.h (IOH zero depths become -NOT_QUITE_SMALL, ssw.c:221-224)
+NOT_QUITE_SMALL = 1.1920928955078125e-07  # FLT_EPSILON, src/include/define.h:28 (IOH zero depths become -NOT_QUITE_SMALL, ssw.c:221-224)
 
 
 @dataclass

@@ -116,6 +116,8 @@ def lib():
         L.adh_sw2_hip_batch_get_dt.argtypes = [C.c_void_p, C.c_int, PD]
         L.adh_sw2_hip_set_wind.argtypes = [C.c_void_p, PD]
         L.adh_sw2_hip_batch_set_wind.argtypes = [C.c_void_p, C.c_int, PD]
+        L.adh_sw2_hip_set_series.argtypes = [C.c_void_p, PD, PD]
+        L.adh_sw2_hip_batch_set_series.argtypes = [C.c_void_p, C.c_int, PD, PD]
         L.adh_sw2_hip_batch_set_tol.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.adh_sw2_hip_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.adh_sw2_hip_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
@@ -244,6 +246,13 @@ class SW2Model(_Tunable):
 
     def set_dt(self, dt):
         _check(self.L.adh_sw2_hip_set_dt(self.h, dt))
+
+    def set_series(self, e1d_val=None, e2d_src=None):
+        """Series values averaged over the coming step (sseries_setall_ts_valuesAVG,
+        src/main/adh_func.c:542): edge values [B] and rain sources [E]; None keeps them."""
+        a = None if e1d_val is None else np.ascontiguousarray(e1d_val, dtype=np.float64)
+        b = None if e2d_src is None else np.ascontiguousarray(e2d_src, dtype=np.float64)
+        _check(self.L.adh_sw2_hip_set_series(self.h, None if a is None else _pd(a), None if b is None else _pd(b)))
 
     def set_wind(self, stress_xy):
         """Node wind values (x, y per node) for the next time step."""
@@ -400,6 +409,12 @@ class SW2Ensemble(_Tunable):
 
     def set_dt(self, i, dt):
         _check(self.L.adh_sw2_hip_batch_set_dt(self.h, i, dt))
+
+    def set_series(self, i, e1d_val=None, e2d_src=None):
+        a = None if e1d_val is None else np.ascontiguousarray(e1d_val, dtype=np.float64)
+        b = None if e2d_src is None else np.ascontiguousarray(e2d_src, dtype=np.float64)
+        _check(self.L.adh_sw2_hip_batch_set_series(self.h, i, None if a is None else _pd(a),
+                                                   None if b is None else _pd(b)))
 
     def set_wind(self, i, stress_xy):
         _check(self.L.adh_sw2_hip_batch_set_wind(self.h, i, _pd(np.ascontiguousarray(stress_xy, dtype=np.float64))))

@@ -160,7 +160,9 @@ int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *em1d);
 /* fe_sw2_inc (src/fe/sw2/fe_sw2_inc.c:24-57); sol NULL = the device solution. */
 int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol);
 /* solv_linear_sys_setup (src/solver/solv_linear_sys_setup.c:15-74): Jacobi
- * scaling in place + preconditioner set-up; scale_vect[3N] optional. */
+ * scaling in place + preconditioner set-up; scale_vect[3N] optional.  Once per
+ * system: a second call before the next load or set_system returns
+ * ADH_SW2_EINVAL (the system is already scaled). */
 int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect);
 /* solv_linear_sys_solve (src/solver/solv_linear_sys_solve.c:16-69):
  * BiCGSTAB (src/solver/solv_bcgstab.c:39-301) on the scaled system;
@@ -183,6 +185,16 @@ int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *verdict, int 
  * station series (src/meteor/meteor_update.c:36-43, called at
  * src/main/adh_func.c:545).  Only for a model created with wind = 1. */
 int adh_sw2_hip_set_wind(adh_sw2_hip *h, const double *stress_xy);
+
+/* Time-series values the residuals read, re-averaged by the caller over
+ * [t, t + dt] before every time-step attempt, as the reference does with
+ * sseries_setall_ts_valuesAVG (src/main/adh_func.c:542, src/structs/sseries.c:618)
+ * before fe_sw2_boundary_resid reads them through sseries_get_value
+ * (src/fe/sw2/fe_sw2_boundary_resid.c:212-214, 372) and fe_sw2_body_resid its
+ * rain source (src/fe/sw2/fe_sw2_body_resid.c:728-754): e1d_val[B] as in
+ * adh_sw2_mesh.e1d_val (NB VEL flux / NB OTW tail elevation), e2d_src[E] as
+ * in adh_sw2_mesh.e2d_src.  Either pointer may be NULL (left as it is). */
+int adh_sw2_hip_set_series(adh_sw2_hip *h, const double *e1d_val, const double *e2d_src);
 
 /* ---- Ensembles: n independent models advanced together -------------------
  * No counterpart in the reference, whose fe_newton runs one model at a time
@@ -211,6 +223,9 @@ int adh_sw2_hip_batch_set_dt(adh_sw2_hip *h, int i, double dt);
 int adh_sw2_hip_batch_get_dt(adh_sw2_hip *h, int i, double *dt);
 /* node wind values of model i (as adh_sw2_hip_set_wind over its own nodes) */
 int adh_sw2_hip_batch_set_wind(adh_sw2_hip *h, int i, const double *stress_xy);
+/* series values of model i (as adh_sw2_hip_set_series over its own edges and
+ * triangles) */
+int adh_sw2_hip_batch_set_series(adh_sw2_hip *h, int i, const double *e1d_val, const double *e2d_src);
 /* nonlinear tolerance of model i (the reference's NTL is one absolute value
  * per model); models without one use opts.tol_nonlin */
 int adh_sw2_hip_batch_set_tol(adh_sw2_hip *h, int i, double tol_nonlin);

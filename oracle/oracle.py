@@ -74,6 +74,7 @@ def _setup(L):
     L.o_get_state.argtypes = [C.c_void_p] + [PD] * 6
     L.o_set_dt.argtypes = [C.c_void_p, C.c_double]
     L.o_set_wind.argtypes = [C.c_void_p, PD]
+    L.o_set_series.argtypes = [C.c_void_p, PD, PD]
     L.o_wdflag.argtypes = [C.c_void_p, PI]
     L.o_init.argtypes = [C.c_void_p, PI]
     L.o_body_resid.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, PD]
@@ -189,6 +190,11 @@ class Oracle:
 
     def set_wind(self, stress_xy):
         self.L.o_set_wind(self.h, _pd(np.ascontiguousarray(stress_xy, dtype=np.float64)))
+
+    def set_series(self, e1d_val=None, e2d_src=None):
+        a = None if e1d_val is None else np.ascontiguousarray(e1d_val, dtype=np.float64)
+        b = None if e2d_src is None else np.ascontiguousarray(e2d_src, dtype=np.float64)
+        self.L.o_set_series(self.h, None if a is None else _pd(a), None if b is None else _pd(b))
 
     def geometry(self):
         p = self.p

@@ -103,6 +103,9 @@ void o_set_dt(o_model *m, double dt);
 /* node wind values SWIND.stress (x, y per node) as meteor_update sets them
    (src/meteor/meteor_update.c:36-43) */
 void o_set_wind(o_model *m, const double *stress_xy);
+/* series values averaged over [t, t+dt] (sseries_setall_ts_valuesAVG,
+   src/main/adh_func.c:542): edge values [B], rain sources [E]; NULL keeps */
+void o_set_series(o_model *m, const double *e1d_val, const double *e2d_src);
 
 void o_wdflag(o_model *m, int *wd_flag_out);
 void o_init(o_model *m, int *bc_mask_out);

@@ -304,3 +304,40 @@ def test_newton_converges_quadratically(cfg1):
     # each of the first steps at least squares the relative residual (within a constant)
     rel = hm / hm[0]
     assert rel[2] < 10.0 * rel[1] ** 1.5
+
+
+def test_block_jacobi_is_prec1_with_one_node_per_block(cfg1):
+    """The HIP path's 3x3 block-Jacobi preconditioner is the reference's own
+    OP PRE 1 with OP BLK = nnodes: node_order's partition then gives every
+    node a block of its own (node_order.c:64-104), and the skyline LU of a
+    3x3 block (solv_blk_load_profile.c:16-233, no pivoting) is the block's
+    inverse up to rounding.  BiCGSTAB therefore takes the same iterations
+    and Newton the same steps: the solutions agree to rounding, the Newton
+    histories to 1e-10 of R0 (measured 3.3e-11: the inexact inner solves,
+    rtol 1e-5, carry the rounding differences of the two inverses into the
+    next iterate).  A preconditioner that dropped or transposed a
+    diagonal block would change the iteration counts."""
+    N = cfg1.nnodes
+    out = {}
+    for key, kw in (("bj", dict(prec=2)), ("blk", dict(prec=1, nblock=N))):
+        o = _loaded(cfg1)
+        o.linear_setup(opts(**kw))
+        out[key] = o.linear_solve(opts(lin_rtol=1e-10, lin_atol=0.0, **kw))
+    (ok1, x1, it1), (ok2, x2, it2) = out["bj"], out["blk"]
+    assert ok1 == ok2 == 1 and it1 == it2
+    assert np.abs(x1 - x2).max() <= 1e-12 * np.abs(x1).max()
+
+    hist = {}
+    for key, kw in (("bj", dict(prec=2)), ("blk", dict(prec=1, nblock=N))):
+        o = Oracle(cfg1)
+        o.shift()
+        o.init()
+        R0 = np.abs(o.resid()).max()
+        res = o.newton(opts(tol_nonlin=1e-8 * R0, max_nonlin_it=30, **kw))
+        assert res["converged"]
+        hist[key] = (R0, res["iterations"], res["hist_max"], o.get_state()[0])
+    R0, n1, h1, s1 = hist["bj"]
+    _, n2, h2, s2 = hist["blk"]
+    assert n1 == n2
+    assert np.abs(h1 - h2).max() <= 1e-10 * R0
+    assert np.abs(s1 - s2).max() <= 1e-10
