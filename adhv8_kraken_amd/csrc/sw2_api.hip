@@ -55,8 +55,10 @@ extern "C" void adh_sw2_hip_default_opts(adh_sw2_opts *o) {
 
 static int check_opts(const adh_sw2_opts *o) {
     if (o->precision < 0 || o->precision > 2) return fail(ADH_SW2_EINVAL, "precision must be 0, 1 or 2");
-    if (o->preconditioner != 1 && o->preconditioner != 2)
-        return fail(ADH_SW2_EUNSUPPORTED, "preconditioner must be 1 (profile LU) or 2 (block Jacobi)");
+    if (o->preconditioner < 1 || o->preconditioner > 3)
+        return fail(ADH_SW2_EUNSUPPORTED, "preconditioner must be 1 (profile LU), 2 (block Jacobi) or 3 (block ILU(0))");
+    if (o->preconditioner == 3 && o->precision == 2)
+        return fail(ADH_SW2_EUNSUPPORTED, "the ILU(0) preconditioner runs with FP64 Krylov vectors (precision 0 or 1)");
     if (o->preconditioner == 1 && o->precision != 0)
         return fail(ADH_SW2_EUNSUPPORTED, "the profile-LU preconditioner runs in FP64 (precision 0)");
     if (o->preconditioner == 1 && o->nblock < 1) return fail(ADH_SW2_EINVAL, "nblock must be >= 1");
@@ -519,6 +521,7 @@ extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     drop_graphs(c);
     band_free(&c);
+    ilu_free(&c);
     for (void *p : h->allocs) (void)hipFree(p);
     if (c.h_ksc) (void)hipHostFree(c.h_ksc);
     if (c.h_norms) (void)hipHostFree(c.h_norms);
@@ -772,6 +775,12 @@ extern "C" int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol) {
 
 // builds the profile-LU block structure when the options ask for it
 static int ensure_band(Ctx &c) {
+    if (c.opts.preconditioner == 3) {   // block ILU(0): the structure once per pattern and factor precision
+        if (c.ilu_ready && c.ilu_f32 == (c.opts.precision >= 1)) return 0;
+        std::string err = ilu_structure(&c);
+        if (!err.empty()) return fail(ADH_SW2_ENOMEM, err);
+        return 0;
+    }
     if (c.opts.preconditioner != 1) return 0;
     if (c.band_nblock == std::min(c.opts.nblock, c.N)) return 0;
     std::string err = band_structure(&c, c.opts.nblock);
@@ -779,7 +788,7 @@ static int ensure_band(Ctx &c) {
     return 0;
 }
 
-extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) {
+extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, const double *x, double *scale_vect) {
     if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
     if (!h) return fail(ADH_SW2_EINVAL, "null handle");
     Ctx &c = h->c;
@@ -791,6 +800,14 @@ extern "C" int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect) 
         return fail(ADH_SW2_EINVAL, "linear_sys_setup: the system is already scaled (call load or set_system first)");
     int rc0 = ensure_band(c);
     if (rc0) return rc0;
+    // x is the solve's initial shift (solv_bcgstab.c:96 keeps it as u0 and
+    // solv_return_old.c:23-25 adds it back); fe_newton.c:470 zeroes it before
+    // every solve, which a null x stands for.
+    if (x) {
+        if ((rc0 = nodes_from_host(c, c.sol, x, 3))) return rc0;
+    } else {
+        launch_zero(&c, c.sol, 3 * (size_t)c.Np);
+    }
     launch_setup(&c);
     int rc = sync_check(c);
     if (rc) return rc;
@@ -821,11 +838,13 @@ static int run_iters(Ctx &c, int k) {
         HIPCHK(hipGetLastError());
         return 0;
     }
-    if (c.kgraph_prec != c.opts.precision || c.kgraph_pc != c.opts.preconditioner || c.kgraph_band != c.band) {
+    // the captured kernels' arguments: the precision, the preconditioner and its storage
+    const double *pstore = c.opts.preconditioner == 3 ? (const double *)c.ilu_L : c.band;
+    if (c.kgraph_prec != c.opts.precision || c.kgraph_pc != c.opts.preconditioner || c.kgraph_band != pstore) {
         drop_graphs(c);
         c.kgraph_prec = c.opts.precision;
         c.kgraph_pc = c.opts.preconditioner;
-        c.kgraph_band = c.band;
+        c.kgraph_band = pstore;
     }
     if (!c.kgraph[k]) {
         hipGraph_t g = nullptr;
@@ -887,6 +906,32 @@ static int krylov(Ctx &c, int *lin_it, int *verdict, bool zero_guess) {
     HIPCHK(hipGetLastError());
     if (lin_it) *lin_it = c.h_ksc->it;
     *verdict = (c.h_ksc->it == lin_it_cap(c)) ? ADH_SW2_NO : ADH_SW2_YES;
+    return 0;
+}
+
+// y = M^-1 v with the preconditioner the last set-up built (profile LU or
+// ILU(0)); for tests of the preconditioner on its own.
+extern "C" int adh_sw2_hip_precond_apply(adh_sw2_hip *h, const double *v, double *y) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
+    if (!h || !v || !y) return fail(ADH_SW2_EINVAL, "null argument");
+    Ctx &c = h->c;
+    if (!c.sys_scaled) return fail(ADH_SW2_EINVAL, "precond_apply: no preconditioner set up (call linear_sys_setup)");
+    if (c.opts.preconditioner == 2) return fail(ADH_SW2_EUNSUPPORTED, "precond_apply: profile LU or ILU(0) only");
+    int rc;
+    if ((rc = nodes_from_host(c, c.kAs, v, 3))) return rc;
+    if (c.opts.preconditioner == 3) launch_ilu_solve(&c, c.kAs, nullptr);
+    else launch_band_solve(&c, c.kAs, nullptr);
+    if ((rc = sync_check(c))) return rc;
+    return nodes_to_host(c, c.kAs, y, 3);
+}
+
+// The ILU(0) node colouring: color[N], *ncolor.
+extern "C" int adh_sw2_hip_ilu_colors(adh_sw2_hip *h, int *color, int *ncolor) {
+    if (!h || !color || !ncolor) return fail(ADH_SW2_EINVAL, "null argument");
+    Ctx &c = h->c;
+    if (!c.ilu_ready) return fail(ADH_SW2_EINVAL, "ilu_colors: no ILU(0) structure (preconditioner 3, after a set-up)");
+    std::copy(c.h_ilu_color.begin(), c.h_ilu_color.end(), color);
+    *ncolor = c.ilu_ncolor;
     return 0;
 }
 
@@ -1598,7 +1643,7 @@ extern "C" int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out) {
     for (int k = 0; k < 4; k++) {
         out->block[k] = h->c.tune.bs[k];
         out->grid[k] = h->c.tune.grid[k];
-        out->waves[k] = h->c.tune.occ[k];
+        out->layout[k] = h->c.tune.layout[k];
         out->ms[k] = 0.0;
     }
     return 0;
@@ -1610,13 +1655,13 @@ extern "C" int adh_sw2_hip_set_tuning(adh_sw2_hip *h, const adh_sw2_tuning *t) {
         if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
             return fail(ADH_SW2_EINVAL, "list kernel block size must be 64, 128 or 256");
         if (t->grid[k] < 1 || t->grid[k] > (1 << 20)) return fail(ADH_SW2_EINVAL, "list kernel grid out of range");
-        if (t->waves[k] != 0 && !(k == 0 && (t->waves[k] == 3 || t->waves[k] == 4)))
-            return fail(ADH_SW2_EINVAL, "waves must be 0, or 3 or 4 for the Jacobian wet list");
+        if (t->layout[k] != 0 && !(k < 2 && t->layout[k] == 1))
+            return fail(ADH_SW2_EINVAL, "layout must be 0, or 1 for the Jacobian lists");
     }
     for (int k = 0; k < 4; k++) {
         h->c.tune.bs[k] = t->block[k];
         h->c.tune.grid[k] = t->grid[k];
-        h->c.tune.occ[k] = t->waves[k];
+        h->c.tune.layout[k] = t->layout[k];
     }
     return 0;
 }
@@ -1642,16 +1687,16 @@ extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *ou
         const int items = (phase < 2 ? 18 : 1) * cnt[phase & 1];
         if (items == 0) continue;
         double tbest = 1e30;
-        for (int occ : {0, 3, 4}) {
-            if (occ && phase != 0) continue;
+        for (int layout : {0, 1}) {
+            if (layout && phase >= 2) continue;
             for (int b : BS) {
                 const int full = nblocks_for(items, b);
                 for (int gi = 0; gi < 8; gi++) {
                     const int g = std::min(GRID[gi], full);
                     if (gi > 0 && std::min(GRID[gi - 1], full) == g) continue;   // same shape as the previous one
-                    launch_list_phase(&c, phase, b, g, occ);
+                    launch_list_phase(&c, phase, b, g, layout);
                     (void)hipEventRecord(c.ev[0], c.stream);
-                    for (int r = 0; r < reps; r++) launch_list_phase(&c, phase, b, g, occ);
+                    for (int r = 0; r < reps; r++) launch_list_phase(&c, phase, b, g, layout);
                     (void)hipEventRecord(c.ev[1], c.stream);
                     HIPCHK(hipEventSynchronize(c.ev[1]));
                     HIPCHK(hipGetLastError());
@@ -1660,7 +1705,7 @@ extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *ou
                         tbest = ms;
                         best.block[phase] = b;
                         best.grid[phase] = g;
-                        best.waves[phase] = occ;
+                        best.layout[phase] = layout;
                         best.ms[phase] = ms;
                     }
                 }
@@ -1670,7 +1715,7 @@ extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *ou
     for (int k = 0; k < 4; k++) {
         c.tune.bs[k] = best.block[k];
         c.tune.grid[k] = best.grid[k];
-        c.tune.occ[k] = best.waves[k];
+        c.tune.layout[k] = best.layout[k];
     }
     if (out) *out = best;
     return sync_check(c);

@@ -101,7 +101,7 @@ struct Ctx {
     struct Tuning {
         int bs[4] = {128, 128, 128, 128};
         int grid[4] = {4096, 1024, 2048, 512};
-        int occ[4] = {0, 0, 0, 0};   // minimum waves per SIMD asked of the compiler (0: its choice)
+        int layout[4] = {0, 0, 0, 0};   // Jacobian lists: 0 a lane per (triangle, column, sign), 1 column-uniform waves
     } tune;
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state
@@ -160,7 +160,45 @@ struct Ctx {
     int *blk_n0, *blk_w, *node_blk;
     size_t *blk_off;
     double *band;
+    // block ILU(0) in multicolour order (preconditioner 3), built on demand
+    int ilu_ready, ilu_ncolor, ilu_Rp, ilu_f32;
+    size_t ilu_nl, ilu_nu;                 // lower / upper slots (64 per slice slot)
+    std::vector<int> h_ilu_cstart;         // [ncolor + 1] first row of each colour (padded to 64)
+    std::vector<int> h_ilu_color;          // [N] colour of each node
+    std::vector<void *> ilu_allocs;
+    int *ilu_rnode, *ilu_lptr, *ilu_uptr, *ilu_lcol, *ilu_ucol, *ilu_lsrc, *ilu_usrc, *ilu_tptr;
+    int4 *ilu_upd;
+    double *ilu_Lw, *ilu_Uw, *ilu_Dw, *ilu_Dinv, *ilu_w;   // FP64 factor work, colour-ordered vector
+    void *ilu_L, *ilu_U, *ilu_D;           // the apply's factors (FP64 or FP32) and U_ii^-1
 };
+
+// 3x3 block inverse by Gauss-Jordan with partial pivoting (block-Jacobi set-up)
+__device__ __forceinline__ void inv3(const double (&d)[9], double (&inv)[9]) {
+    double a[3][6];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            a[r][c] = d[3 * r + c];
+            a[r][3 + c] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int k = 0; k < 3; k++) {
+        int p = k;
+        for (int r = k + 1; r < 3; r++) if (fabs(a[r][k]) > fabs(a[p][k])) p = r;
+        if (p != k)
+            for (int c = 0; c < 6; c++) { double t = a[k][c]; a[k][c] = a[p][c]; a[p][c] = t; }
+        double piv = a[k][k];
+        if (fabs(piv) < SMALL) piv = (piv < 0) ? -SMALL : SMALL;
+        double ip = 1.0 / piv;
+        for (int c = 0; c < 6; c++) a[k][c] *= ip;
+        for (int r = 0; r < 3; r++) {
+            if (r == k) continue;
+            double f = a[r][k];
+            for (int c = 0; c < 6; c++) a[r][c] -= f * a[k][c];
+        }
+    }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) inv[3 * r + c] = a[r][3 + c];
+}
+
 
 
 // value index of entry e (0..8) of SELL slot q
@@ -223,7 +261,11 @@ void launch_jac2d_only(Ctx *c);
 void launch_resid2d_only(Ctx *c);
 void launch_assemble_only(Ctx *c);
 void ensure_lists(Ctx *c);
-void launch_list_phase(Ctx *c, int phase, int bs, int grid, int occ);
+void launch_list_phase(Ctx *c, int phase, int bs, int grid, int layout);
+std::string ilu_structure(Ctx *c);
+void ilu_free(Ctx *c);
+void launch_ilu_setup(Ctx *c);
+void launch_ilu_solve(Ctx *c, double *x, const KScal *sc);
 std::string band_structure(Ctx *c, int nblock);
 void band_free(Ctx *c);
 void launch_band_setup(Ctx *c);

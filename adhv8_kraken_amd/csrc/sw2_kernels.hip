@@ -205,6 +205,67 @@ __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1) k_jac2d_list(ElemData D
     }
 }
 
+// The same Jacobian columns in a column-uniform layout: wave w takes column
+// q = w % 9 (node j = q / 3, variable q % 3) of the 32 triangles of group
+// w / 9, lanes 0-31 evaluating the +eps residuals and lanes 32-63 the -eps
+// ones of the same triangles.  The perturbed node and variable are then the
+// same across the wave (scalar registers), so the residual's branches on them
+// are taken or skipped by whole waves -- the u and v columns skip the
+// pressure block (fe_sw2_body_resid.c:761-862 runs for depth perturbations
+// only) instead of running it under a lane mask -- and each triangle's data
+// is loaded by 64 lanes of 32 different triangles rather than broadcast to 18
+// lanes of one.  Every (triangle, column, sign) evaluates exactly what the
+// lane-per-column kernel evaluates, so the blocks are bit-identical.
+template <bool WET, int BS>
+__global__ void __launch_bounds__(BS) k_jac2d_cols(ElemData D, const int *list, const int *count, int b0, int b1,
+                                                   int b2, const int *blkpos, double *EM, float *EM32) {
+    constexpr int WPB = BS / 64;
+    const int n = *count;
+    const int nunits = 9 * ((n + 31) >> 5);
+    const int lane = threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool minus = lane >= 32;
+    for (int w = blockIdx.x * WPB + wib; w < nunits; w += gridDim.x * WPB) {
+        const int q = w % 9, grp = w / 9;
+        const int j = q / 3, var = q % 3;
+        const int li = grp * 32 + (lane & 31);
+        const bool act = li < n;
+        // idle lanes of a group's last wave recompute its first triangle and store nothing
+        const int ie = list[act ? li : grp * 32];
+        const int nodes[3] = {D.e2d[3 * ie], D.e2d[3 * ie + 1], D.e2d[3 * ie + 2]};
+        const int nd = pick3(nodes, j);
+        const double val = var == 0 ? D.u[nd] : (var == 1 ? D.v[nd] : D.head[nd]);
+        const double eps = num_diff_eps(val), eps2 = 2.0 * eps;
+        const int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
+        d3 r[3];
+        body_resid<WET>(D, D.pre + PRE_STRIDE * (size_t)ie, ie, eps, j, pv, minus ? -1 : +1, false, r, nodes);
+        double col[9];
+        const double inv = 1.0 / eps2;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            // lanes l and l ^ 32 hold the +eps and -eps residuals of one triangle
+            double mx = __shfl_xor(r[i].x_eq, 32, 64), my = __shfl_xor(r[i].y_eq, 32, 64),
+                   mc = __shfl_xor(r[i].c_eq, 32, 64);
+            col[3 * i + 0] = div_rn(r[i].x_eq - mx, eps2, inv);
+            col[3 * i + 1] = div_rn(r[i].y_eq - my, eps2, inv);
+            col[3 * i + 2] = div_rn(r[i].c_eq - mc, eps2, inv);
+        }
+        if (minus || !act) continue;
+        const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.elem_inst[ie] : nullptr;
+        const int lb[3] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1, lbi ? lbi[2] : b2};
+        jac_column_db<3>(D, nodes, lb, j, var, col);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const size_t o = 9 * (size_t)blkpos[9 * ie + 3 * i + j];
+#pragma unroll
+            for (int rr = 0; rr < 3; rr++) {
+                if (EM32) EM32[o + 3 * rr + var] = (float)col[3 * i + rr];
+                else EM[o + 3 * rr + var] = col[3 * i + rr];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, const int *blkpos1, double *EM,
                                                float *EM32) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,20 +443,18 @@ static void launch_split(Ctx *c) {
 // Grid-stride list launches.  Block size and grid size change only the
 // schedule (every triangle's arithmetic and output slot are fixed), so they
 // are free parameters: defaults below, or what adh_sw2_hip_autotune measured.
-template <bool WET, int BS, int OCC>
-static void jac_list_occ(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
-    hipLaunchKernelGGL((k_jac2d_list<WET, BS, OCC>), dim3(std::min(grid, nblocks_for(18 * c->E, BS))), dim3(BS), 0,
+template <bool WET, int BS>
+static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid, int layout) {
+    if (layout == 1) {   // column-uniform waves: 9 waves per 32 triangles
+        const int units = 9 * ((c->E + 31) / 32);
+        hipLaunchKernelGGL((k_jac2d_cols<WET, BS>), dim3(std::min(grid, nblocks_for(units, BS / 64))), dim3(BS), 0,
+                           c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
+                           em32_of(c));
+        return;
+    }
+    hipLaunchKernelGGL((k_jac2d_list<WET, BS, 0>), dim3(std::min(grid, nblocks_for(18 * c->E, BS))), dim3(BS), 0,
                        c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
                        em32_of(c));
-}
-// the wet list carries almost all triangles: it also gets register-budget variants
-template <bool WET, int BS>
-static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid, int occ) {
-    if constexpr (WET) {
-        if (occ == 3) return jac_list_occ<WET, BS, 3>(c, D, list, count, grid);
-        if (occ == 4) return jac_list_occ<WET, BS, 4>(c, D, list, count, grid);
-    }
-    jac_list_occ<WET, BS, 0>(c, D, list, count, grid);
 }
 template <bool WET, int BS>
 static void res_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid) {
@@ -404,14 +463,14 @@ static void res_list_bs(Ctx *c, const ElemData &D, const int *list, const int *c
 }
 
 // phase: 0 Jacobian wet list, 1 Jacobian general list, 2 residual wet list, 3 residual general list
-void launch_list(Ctx *c, const ElemData &D, int phase, int bs, int grid, int occ) {
+void launch_list(Ctx *c, const ElemData &D, int phase, int bs, int grid, int layout) {
     const bool wet = phase == 0 || phase == 2;
     const int *list = wet ? c->elist_wet : c->elist_gen;
     const int *count = wet ? c->ecount : c->ecount + 1;
 #define SW2_LIST_CASE(B)                                                                      \
     case B:                                                                                    \
-        if (phase == 0) jac_list_bs<true, B>(c, D, list, count, grid, occ);                    \
-        else if (phase == 1) jac_list_bs<false, B>(c, D, list, count, grid, occ);              \
+        if (phase == 0) jac_list_bs<true, B>(c, D, list, count, grid, layout);                    \
+        else if (phase == 1) jac_list_bs<false, B>(c, D, list, count, grid, layout);              \
         else if (phase == 2) res_list_bs<true, B>(c, D, list, count, grid);                    \
         else res_list_bs<false, B>(c, D, list, count, grid);                                   \
         break;
@@ -430,24 +489,24 @@ void ensure_lists(Ctx *c) {
 }
 
 // one list kernel with the element data its phase uses (autotuning)
-void launch_list_phase(Ctx *c, int phase, int bs, int grid, int occ) {
+void launch_list_phase(Ctx *c, int phase, int bs, int grid, int layout) {
     ElemData D = c->ed;
     D.first_elem = phase >= 2 ? c->first_elem : nullptr;
-    launch_list(c, D, phase, bs, grid, occ);
+    launch_list(c, D, phase, bs, grid, layout);
 }
 
 static void launch_jac2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
     if (!c->split_valid) launch_split(c);
-    launch_list(c, D, 0, c->tune.bs[0], c->tune.grid[0], c->tune.occ[0]);
-    launch_list(c, D, 1, c->tune.bs[1], c->tune.grid[1], c->tune.occ[1]);
+    launch_list(c, D, 0, c->tune.bs[0], c->tune.grid[0], c->tune.layout[0]);
+    launch_list(c, D, 1, c->tune.bs[1], c->tune.grid[1], c->tune.layout[1]);
 }
 
 static void launch_resid2d_split(Ctx *c, const ElemData &D) {
     if (!c->E) return;
     if (!c->split_valid) launch_split(c);
-    launch_list(c, D, 2, c->tune.bs[2], c->tune.grid[2], c->tune.occ[2]);
-    launch_list(c, D, 3, c->tune.bs[3], c->tune.grid[3], c->tune.occ[3]);
+    launch_list(c, D, 2, c->tune.bs[2], c->tune.grid[2], c->tune.layout[2]);
+    launch_list(c, D, 3, c->tune.bs[3], c->tune.grid[3], c->tune.layout[3]);
 }
 
 void launch_jacobian(Ctx *c) {

@@ -20,6 +20,8 @@
 // didx9(n, e), off-diagonal blocks in SELL-64 (sell_vidx), node vectors at
 // vidx3(n, k); every per-node load of a wavefront is one 64-wide contiguous
 // access.  SpMV reads 72 B (FP64) per block.
+#include <type_traits>
+
 #include "sw2_ctx.h"
 
 namespace sw2 {
@@ -58,33 +60,6 @@ __global__ void k_scale_rows(int N, const double *diag, const double *offd, cons
     }
     const double small_factor = 1.0 / sqrt(SOLV_TOL);
     for (int j = 0; j < 3; j++) sv[vidx3(n, j)] = s[j] < SOLV_TOL ? small_factor : 1.0 / sqrt(fabs(s[j]));
-}
-
-// 3x3 block inverse by Gauss-Jordan with partial pivoting (block-Jacobi set-up)
-__device__ __forceinline__ void inv3(const double (&d)[9], double (&inv)[9]) {
-    double a[3][6];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) {
-            a[r][c] = d[3 * r + c];
-            a[r][3 + c] = (r == c) ? 1.0 : 0.0;
-        }
-    for (int k = 0; k < 3; k++) {
-        int p = k;
-        for (int r = k + 1; r < 3; r++) if (fabs(a[r][k]) > fabs(a[p][k])) p = r;
-        if (p != k)
-            for (int c = 0; c < 6; c++) { double t = a[k][c]; a[k][c] = a[p][c]; a[p][c] = t; }
-        double piv = a[k][k];
-        if (fabs(piv) < SMALL) piv = (piv < 0) ? -SMALL : SMALL;
-        double ip = 1.0 / piv;
-        for (int c = 0; c < 6; c++) a[k][c] *= ip;
-        for (int r = 0; r < 3; r++) {
-            if (r == k) continue;
-            double f = a[r][k];
-            for (int c = 0; c < 6; c++) a[r][c] -= f * a[k][c];
-        }
-    }
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) inv[3 * r + c] = a[r][3 + c];
 }
 
 // Applies the scaling (A <- S A S, b <- S b, x <- x / S) and sets up the
@@ -194,6 +169,12 @@ void launch_setup(Ctx *c) {
     if (c->opts.preconditioner == 1) {  // profile LU of the node blocks (FP64)
         SW2_SCALE_SWITCH(false, false)
         launch_band_setup(c);
+        return;
+    }
+    if (c->opts.preconditioner == 3) {  // block ILU(0): scaled matrix without block inverses, then the factors
+        if (c->opts.precision >= 1) { SW2_SCALE_SWITCH(true, false) }
+        else { SW2_SCALE_SWITCH(false, false) }
+        launch_ilu_setup(c);
         return;
     }
     if (c->opts.precision >= 1) { SW2_SCALE_SWITCH(true, true) }
@@ -565,14 +546,16 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part,
 }
 
 // ---------------------------------------------------------------------------
-// BiCGSTAB with the profile-LU preconditioner (FP64).  Same recurrences as
-// K1..K3 above; the product M^-1 A x is split into a plain SpMV and the block
-// triangular solves (sw2_blk.hip), so each half-iteration is SpMV, solve,
-// then a reduction kernel.  K3 is shared.
+// BiCGSTAB with a preconditioner applied after the SpMV: the profile LU
+// (FP64, sw2_blk.hip) or the block ILU(0) (FP64 or FP32 factors,
+// sw2_ilu.hip).  Same recurrences as K1..K3 above; the product M^-1 A x is
+// split into a plain SpMV (the matrix in the form's storage, FP64 vectors)
+// and the preconditioner's triangular solves, so each half-iteration is
+// SpMV, solve, then a reduction kernel.  K3 is shared.
 
 // y = A x with x = p (S = false) or x = s = r - alpha Ap (S = true, s stored)
-template <int K, bool S>
-__global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<PrecF64> V, const int *sell_ptr, const int *sell_col,
+template <typename P, int K, bool S>
+__global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col,
                                                  const double *x, double *y, const KScal *sc, int check_done) {
     if (check_done && sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -580,11 +563,11 @@ __global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<PrecF64> V, const 
     double a[3];
     if constexpr (S) {
         VecS<double> sf{V.r, V.Ap, -sc->alpha};
-        amult_prec_row<PrecF64, K, VecS<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr, sf, a);
+        amult_prec_row<P, K, VecS<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr, sf, a);
         for (int j = 0; j < 3; j++) V.s[vidx3(n, j)] = sf(vidx3(n, j));
     } else {
-        amult_prec_row<PrecF64, K, VecX<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr,
-                                                       VecX<double>{x}, a);
+        amult_prec_row<P, K, VecX<double>, false>(n, V.diag, V.offd, sell_ptr, sell_col, nullptr,
+                                                 VecX<double>{x}, a);
     }
     for (int j = 0; j < 3; j++) y[vidx3(n, j)] = a[j];
 }
@@ -596,7 +579,8 @@ __global__ void k_pl_copy(size_t n, const double *src, double *dst) {
 }
 
 // u = b' (= M^-1 b), ||b'||inf
-__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+template <typename P>
+__global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
     const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
@@ -614,7 +598,8 @@ __global__ void __launch_bounds__(256) k_pl_p1(int N, KVecs<PrecF64> V, double *
 }
 
 // r = b' - y (y = M^-1 A u in As), q = p = r, rho, ||r||inf, scalars
-__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+template <typename P>
+__global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
     const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
@@ -635,7 +620,8 @@ __global__ void __launch_bounds__(256) k_pl_p2(int N, KVecs<PrecF64> V, double *
 }
 
 // gamma = (q, Ap), alpha = rho / gamma
-__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+template <typename P>
+__global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
     const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
@@ -649,7 +635,8 @@ __global__ void __launch_bounds__(256) k_pl_gamma(int N, KVecs<PrecF64> V, doubl
 }
 
 // omega = (As, s) / (As, As), rho' = -omega (As, q)
-__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<PrecF64> V, double *part, const int *blk_inst, KScal *sc) {
+template <typename P>
+__global__ void __launch_bounds__(256) k_pl_omega(int N, KVecs<P> V, double *part, const int *blk_inst, KScal *sc) {
     int nend = N;  // ensembles: the block's model and that model's last node + 1
     const int lb = lblock(blk_inst);   // XCD-aware block order
     if (blk_inst) { sc += blk_inst[2 * lb]; nend = blk_inst[2 * lb + 1]; }
@@ -747,55 +734,72 @@ static void k1_only_k(Ctx *c) {
         default: FN<T, 0>(__VA_ARGS__); break;           \
     }
 
-template <typename T, int K>
+// the preconditioner applied after an SpMV: profile LU or ILU(0), in place
+static void prec_solve(Ctx *c, double *x, const KScal *sc) {
+    if (c->opts.preconditioner == 3) launch_ilu_solve(c, x, sc);
+    else launch_band_solve(c, x, sc);
+}
+
+template <typename P, int K>
 static void pl_prologue_k(Ctx *c) {
     const int nb = nblocks_for(c->N, 256);
-    KVecs<PrecF64> V = vecs<PrecF64>(c);
+    KVecs<P> V = vecs<P>(c);
     const size_t n3 = 3 * (size_t)c->Np;
     hipLaunchKernelGGL(k_pl_copy, dim3(nblocks_for((int)n3, 256)), dim3(256), 0, c->stream, n3, c->R, V.b);
-    launch_band_solve(c, V.b, nullptr);
-    hipLaunchKernelGGL(k_pl_p1, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
+    prec_solve(c, V.b, nullptr);
+    hipLaunchKernelGGL(k_pl_p1<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
     hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
-    hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr, c->sell_col,
-                       V.u, V.As, c->ksc, 0);
-    launch_band_solve(c, V.As, nullptr);
-    hipLaunchKernelGGL(k_pl_p2, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
+    hipLaunchKernelGGL((k_pl_spmv<P, K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
+                       c->sell_col, V.u, V.As, c->ksc, 0);
+    prec_solve(c, V.As, nullptr);
+    hipLaunchKernelGGL(k_pl_p2<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
     hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
 }
 
-template <typename T, int K>
+template <typename P, int K>
 static void pl_iters_k(Ctx *c, int k) {
     const int nb = nblocks_for(c->N, 256);
-    KVecs<PrecF64> V = vecs<PrecF64>(c);
+    KVecs<P> V = vecs<P>(c);
     for (int i = 0; i < k; i++) {
-        hipLaunchKernelGGL((k_pl_spmv<K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
+        hipLaunchKernelGGL((k_pl_spmv<P, K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
                            c->sell_col, V.p, V.Ap, c->ksc, 1);
-        launch_band_solve(c, V.Ap, c->ksc);
-        hipLaunchKernelGGL(k_pl_gamma, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
+        prec_solve(c, V.Ap, c->ksc);
+        hipLaunchKernelGGL(k_pl_gamma<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
-        hipLaunchKernelGGL((k_pl_spmv<K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
+        hipLaunchKernelGGL((k_pl_spmv<P, K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
                            c->sell_col, nullptr, V.As, c->ksc, 1);
-        launch_band_solve(c, V.As, c->ksc);
-        hipLaunchKernelGGL(k_pl_omega, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
+        prec_solve(c, V.As, c->ksc);
+        hipLaunchKernelGGL(k_pl_omega<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
-        hipLaunchKernelGGL(k_bcg_k3<PrecF64>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part,
-                           BI(c), c->ksc);
+        hipLaunchKernelGGL(k_bcg_k3<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K3>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     }
 }
 
+// the post-SpMV preconditioners run with FP64 Krylov vectors only (the
+// options refuse them with precision 2)
 template <typename P>
 static void prologue(Ctx *c) {
-    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_prologue_k, P, c) }
-    else { SW2_KSWITCH(prologue_k, P, c) }
+    if constexpr (std::is_same<typename P::V, double>::value) {
+        if (c->opts.preconditioner != 2) {
+            SW2_KSWITCH(pl_prologue_k, P, c)
+            return;
+        }
+    }
+    SW2_KSWITCH(prologue_k, P, c)
 }
 
 template <typename P>
 static void iters(Ctx *c, int k) {
-    if (c->opts.preconditioner == 1) { SW2_KSWITCH(pl_iters_k, P, c, k) }
-    else { SW2_KSWITCH(iters_k, P, c, k) }
+    if constexpr (std::is_same<typename P::V, double>::value) {
+        if (c->opts.preconditioner != 2) {
+            SW2_KSWITCH(pl_iters_k, P, c, k)
+            return;
+        }
+    }
+    SW2_KSWITCH(iters_k, P, c, k)
 }
 
 void launch_k1_only(Ctx *c) {

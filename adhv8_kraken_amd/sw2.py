@@ -63,11 +63,11 @@ class Report(C.Structure):
 class Tuning(C.Structure):
     """adh_sw2_tuning: launch shapes of the element list kernels
     (0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general)."""
-    _fields_ = [("block", C.c_int * 4), ("grid", C.c_int * 4), ("waves", C.c_int * 4), ("ms", C.c_double * 4)]
+    _fields_ = [("block", C.c_int * 4), ("grid", C.c_int * 4), ("layout", C.c_int * 4), ("ms", C.c_double * 4)]
 
     def as_dict(self):
         names = ("jac_wet", "jac_gen", "res_wet", "res_gen")
-        return {n: dict(block=self.block[k], grid=self.grid[k], waves=self.waves[k], ms=round(self.ms[k], 4))
+        return {n: dict(block=self.block[k], grid=self.grid[k], layout=self.layout[k], ms=round(self.ms[k], 4))
                 for k, n in enumerate(names)}
 
 
@@ -102,8 +102,10 @@ def lib():
         L.adh_sw2_hip_load.argtypes = [C.c_void_p, PD, PD, PD]
         L.adh_sw2_hip_elem_jacobians.argtypes = [C.c_void_p, PD, PD]
         L.adh_sw2_hip_inc.argtypes = [C.c_void_p, PD]
-        L.adh_sw2_hip_linear_sys_setup.argtypes = [C.c_void_p, PD]
+        L.adh_sw2_hip_linear_sys_setup.argtypes = [C.c_void_p, PD, PD]
         L.adh_sw2_hip_linear_sys_solve.argtypes = [C.c_void_p, PD, PI, PI]
+        L.adh_sw2_hip_precond_apply.argtypes = [C.c_void_p, PD, PD]
+        L.adh_sw2_hip_ilu_colors.argtypes = [C.c_void_p, PI, PI]
         L.adh_sw2_hip_set_system.argtypes = [C.c_void_p, PD, PD, PD]
         L.adh_sw2_hip_newton.argtypes = [C.c_void_p, C.POINTER(Report), PI]
         L.adh_sw2_hip_timestep.argtypes = [C.c_void_p, C.POINTER(Report), PI, C.c_int]
@@ -312,9 +314,12 @@ class SW2Model(_Tunable):
         s = None if sol is None else np.ascontiguousarray(sol, dtype=np.float64)
         _check(self.L.adh_sw2_hip_inc(self.h, _pd(s)))
 
-    def linear_sys_setup(self):
+    def linear_sys_setup(self, x=None):
+        """x: the solution array on entry (the shift the solve adds back);
+        None = zero, as fe_newton sets it before every solve."""
+        xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
         s = np.empty(3 * self.N)
-        _check(self.L.adh_sw2_hip_linear_sys_setup(self.h, _pd(s)))
+        _check(self.L.adh_sw2_hip_linear_sys_setup(self.h, _pd(xs), _pd(s)))
         return s
 
     def linear_sys_solve(self):
@@ -322,6 +327,19 @@ class SW2Model(_Tunable):
         it, vd = C.c_int(0), C.c_int(0)
         _check(self.L.adh_sw2_hip_linear_sys_solve(self.h, _pd(x), C.byref(it), C.byref(vd)))
         return vd.value, x, it.value
+
+    def precond_apply(self, v):
+        """M^-1 v with the preconditioner of the last set-up (profile LU or ILU(0))."""
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        y = np.empty(3 * self.N)
+        _check(self.L.adh_sw2_hip_precond_apply(self.h, _pd(vv), _pd(y)))
+        return y
+
+    def ilu_colors(self):
+        color = np.empty(self.N, dtype=np.int32)
+        nc = C.c_int(0)
+        _check(self.L.adh_sw2_hip_ilu_colors(self.h, _pi(color), C.byref(nc)))
+        return color, nc.value
 
     def set_system(self, diag, offd, rhs):
         d, o, r = (np.ascontiguousarray(a, dtype=np.float64) for a in (diag, offd, rhs))

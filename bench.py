@@ -30,15 +30,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(p, precision, steps, warmup, rtol, graphs=1, autotune=True):
+def run_form(p, precision, steps, warmup, rtol, graphs=1, autotune=True, preconditioner=2):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts, YES
-    g = SW2Model(p, default_opts(precision=precision, graphs=graphs))
+    g = SW2Model(p, default_opts(precision=precision, graphs=graphs, preconditioner=preconditioner))
     # tolerance: 1e-8 of the initial residual of the first step (the reference's NTL is absolute)
     g.shift()
     g.init()
     r0 = float(np.abs(g.resid()).max())
     g.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
-    g.set_opts(default_opts(precision=precision, tol_nonlin=rtol * r0, max_nonlin_it=40, graphs=graphs))
+    g.set_opts(default_opts(precision=precision, tol_nonlin=rtol * r0, max_nonlin_it=40, graphs=graphs,
+                            preconditioner=preconditioner))
     for i in range(warmup):
         g.timestep()
     tuning = g.autotune(reps=3) if autotune else None
@@ -49,21 +50,26 @@ def run_form(p, precision, steps, warmup, rtol, graphs=1, autotune=True):
     newton_its = 0
     lin_its = 0
     phases = dict(load=0.0, setup=0.0, solve=0.0, resid=0.0)
-    t0 = time.perf_counter()
+    first = None
+    elapsed = 0.0
     for i in range(steps):
+        t0 = time.perf_counter()
         r = g.timestep()
+        elapsed += time.perf_counter() - t0
         if r["verdict"] != YES:
             raise RuntimeError(f"time step {i} did not converge: {r}")
         newton_its += r["iterations"]
         lin_its += r["lin_its"]
         for k in phases:
             phases[k] += r["ms_" + k]
-        log(f"  precision={precision} step {i}: newton {r['iterations']} lin {r['lin_its']} "
+        if i == 0:   # the state after the first step (outside the timed region), for the oracle comparison
+            first = g.get_state()[:2]
+        log(f"  precision={precision} pc={preconditioner} step {i}: newton {r['iterations']} lin {r['lin_its']} "
             f"final max|R| {r['hist_max'][-1]:.3e}")
-    elapsed = time.perf_counter() - t0
     h, vel, *_ = g.get_state()
     return g, dict(newton_its=newton_its, lin_its=lin_its, seconds=elapsed, r0=r0,
-                   newton_per_s=newton_its / elapsed, phases_ms=phases, head=h, vel=vel, tuning=tuning)
+                   newton_per_s=newton_its / elapsed, phases_ms=phases, head=h, vel=vel, tuning=tuning,
+                   first=first)
 
 
 def measured_traffic(precision):
@@ -78,6 +84,18 @@ def measured_traffic(precision):
         return None
     m = re.search(r"^traffic_bytes " + re.escape(key) + r" (\d+)$", txt, re.M)
     return int(m.group(1)) if m else None
+
+
+def measured_traffic_kernel(name):
+    """HBM bytes (read + write) per launch of a kernel from the committed
+    FETCH_SIZE / WRITE_SIZE table (copy-kernel calibrated), or None."""
+    import re
+    try:
+        txt = open(os.path.join(ROOT, "profiles", "traffic_k_bcg_k1_channel1000x500.md")).read()
+    except OSError:
+        return None
+    m = re.search(r"^\| `[^`]*" + re.escape(name) + r"[^`]*` \| \d+ \| \d+ \| ([0-9.]+) \| ([0-9.]+) \|", txt, re.M)
+    return int(round((float(m.group(1)) + float(m.group(2))) * 1e6)) if m else None
 
 
 def spmv_roofline(g, p, precision):
@@ -112,43 +130,86 @@ def jacobian_roofline(g, p):
             flops = float(m.group(1)) * 1e9 if m else None
         except OSError:
             pass
-    out = dict(bound="fp64-valu", kernel="k_jac2d_list", avg_us=round(ms * 1e3, 1), peak=78.6, unit="TFLOP/s")
+    out = dict(bound="fp64-vector", kernel="k_jac2d_list", avg_us=round(ms * 1e3, 1), peak=78.6, unit="TFLOP/s")
     if flops:
         ach = flops / (ms * 1e-3) / 1e12
         out.update(achieved=round(ach, 2), frac=round(ach / 78.6, 4), flops_per_launch=flops)
     return out
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """The oracle (C restatement of the reference's serial path, skyline-LU
-    preconditioned BiCGSTAB) on a bounded sample: one time step of the same
-    channel family at 200 x 100 cells (40,000 triangles), scaled to the
-    metric's 1M-triangle mesh by triangle count."""
-    from adhv8_kraken_amd import mesh
+def profile_lu_cost(p, nblock):
+    """Storage (bytes) and flops of the reference's skyline LU of `nblock`
+    node blocks on mesh p: contiguous node ranges of N // nblock nodes, the
+    last taking the rest (src/node/node_order.c:64-91); a DOF row's profile
+    runs from its leftmost same-block neighbour (solv_blk_profile.c:5-47), and
+    the factorisation (solv_blk_load_profile.c:51-79) costs about
+    2 * sum(row length^2) flops."""
+    N = p.nnodes
+    e = np.asarray(p.e2d).reshape(-1, 3)
+    inc = N // nblock
+    blk = np.minimum(np.arange(N) // inc, nblock - 1)
+    mn = np.arange(N)
+    for a in range(3):
+        for b in range(3):
+            same = blk[e[:, a]] == blk[e[:, b]]
+            np.minimum.at(mn, e[same, a], e[same, b])
+    rowlen = np.repeat(3 * (np.arange(N) - mn), 3) + np.tile(np.arange(3), N)
+    return 8 * (2 * int(rowlen.sum()) + 3 * N), 2.0 * float((rowlen.astype(np.float64) ** 2).sum())
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(p):
+    """The oracle (C restatement of the reference's serial path: FD Jacobian,
+    assembly, Jacobi scaling, skyline-LU preconditioned BiCGSTAB, Newton) on
+    the metric's own mesh, one core.  OP BLK candidates: the reference's
+    default 1 (one LU of the whole matrix) and blocks of 256-4096 nodes plus
+    one mesh column; the cost model `profile_lu_cost` ranks them and the
+    cheapest is run for one full time step (Newton to 1e-8 of R0)."""
     from oracle.oracle import Oracle, opts
-    p = mesh.bumpy_channel(200, 100, seed=2)
+    N = p.nnodes
+    npy = p.meta["npy"]
+    sizes = {"BLK 1": 1}
+    for nodes in (256, 512, 1024, 2048, 4096):
+        sizes[f"{nodes}-node blocks"] = N // nodes
+    sizes[f"one mesh column ({npy} nodes)"] = N // npy
+    costs = {k: profile_lu_cost(p, nb) for k, nb in sizes.items()}
+    best = min(costs, key=lambda k: costs[k][1])
+    nblock = sizes[best]
     o = Oracle(p)
     o.shift()
     o.init()
     r0 = float(np.abs(o.resid()).max())
-    best = None
-    for nblock in (1, 4, 16):
-        o.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
-        o.shift()
-        t0 = time.perf_counter()
-        res = o.newton(opts(tol_nonlin=1e-8 * r0, prec=1, nblock=nblock, max_nonlin_it=40))
-        dt = time.perf_counter() - t0
-        rate = res["iterations"] / dt
-        log(f"  cpu oracle nblock={nblock}: {res['iterations']} newton its in {dt:.2f}s ok={res['ok']}")
-        if res["ok"] == 1 and (best is None or rate > best[0]):
-            best = (rate, nblock, res["iterations"], dt)
-        if dt > seconds_budget:
-            break
-    rate, nblock, its, dt = best
-    scale = p.nelems2d / 1.0e6
-    return dict(value=round(rate * scale, 4), unit="Newton steps/s", cores=1, kind="port",
-                sample=f"oracle on the 200x100-cell channel (40k triangles), 1 time step, OP BLK {nblock}, "
-                       f"{its} Newton its in {dt:.2f}s; rate scaled by 40k/1M triangles")
+    o.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+    o.shift()
+    t0 = time.perf_counter()
+    res = o.newton(opts(tol_nonlin=1e-8 * r0, prec=1, nblock=nblock, max_nonlin_it=40))
+    dt = time.perf_counter() - t0
+    its = res["iterations"]
+    log(f"  cpu oracle OP BLK {nblock} ({best}): {its} newton its in {dt:.2f}s ok={res['ok']} "
+        f"(load {res['time_load']:.1f}s, setup {res['time_setup']:.1f}s, solve {res['time_solve']:.1f}s)")
+    o_state = o.get_state()[:2]
+    model, nproc = host_cpu()
+    others = "; ".join(f"{k}: OP BLK {sizes[k]}, {costs[k][0] / 1e9:.2f} GB, {costs[k][1]:.2e} flops"
+                       for k in sizes if k != best)
+    return dict(value=round(its / dt, 4) if res["ok"] == 1 else None, unit="Newton steps/s", cores=1, kind="port",
+                converged=res["ok"] == 1, op_blk=nblock, cpu_model=model, nproc=nproc,
+                sample=f"oracle on the {p.meta['npx'] - 1}x{npy - 1}-cell channel itself ({p.nelems2d} triangles), "
+                       f"one time step from the initial state, OP BLK {nblock} ({best}: skyline LU of "
+                       f"{costs[best][0] / 1e9:.2f} GB, {costs[best][1]:.2e} flops per factorisation), "
+                       f"{its} Newton its in {dt:.1f} s (load {res['time_load']:.1f} s, set-up "
+                       f"{res['time_setup']:.1f} s, solve {res['time_solve']:.1f} s); not run, by the same "
+                       f"cost model: {others}"), o_state
 
 
 def run_ensemble(ps, precision, steps, warmup, rtol, max_retries=2, autotune=True):
@@ -329,31 +390,49 @@ def main():
         return
     res = {}
     roof = {}
-    for prec in (0, 1, 2):
-        log(f"form precision={prec}")
-        g, r = run_form(p, prec, args.steps, args.warmup, 1e-8, args.graphs, autotune=not args.no_autotune)
-        res[prec] = r
-        roof[prec] = spmv_roofline(g, p, prec)
-        roof[prec]["jac_us"] = round(g.time_kernel(1, reps=5) * 1e3, 1)
-        if prec == 1:
+    # forms: (precision, preconditioner) -- FP64, FP32-refined, all-FP32 with
+    # block Jacobi, and the FP32-refined form with the block ILU(0)
+    forms = [(0, 2), (1, 2), (2, 2), (1, 3)]
+    for prec, pc in forms:
+        log(f"form precision={prec} preconditioner={pc}")
+        g, r = run_form(p, prec, args.steps, args.warmup, 1e-8, args.graphs, autotune=not args.no_autotune,
+                        preconditioner=pc)
+        res[(prec, pc)] = r
+        if pc == 2:
+            roof[prec] = spmv_roofline(g, p, prec)
+            roof[prec]["jac_us"] = round(g.time_kernel(1, reps=5) * 1e3, 1)
+        if (prec, pc) == (1, 2):
             jroof = jacobian_roofline(g, p)
         g.close()
-    cpu = None if args.no_cpu else cpu_baseline()
-    r32, r64 = res[1], res[0]
-    # both forms ran the same steps from the same state: error of the FP32-refined
-    # state against the FP64 run, relative to the FP64 field's largest magnitude
-    def rel_err(r):
-        return dict(head=float(np.abs(r["head"] - r64["head"]).max() / np.abs(r64["head"]).max()),
-                    velocity=float(np.abs(r["vel"] - r64["vel"]).max() / np.abs(r64["vel"]).max()))
-    err, err32 = rel_err(r32), rel_err(res[2])
+    cpu, o_state = (None, None) if args.no_cpu else cpu_baseline(p)
+    r64 = res[(0, 2)]
+    # the refined forms' headline is the faster preconditioner
+    best_pc = 3 if res[(1, 3)]["newton_per_s"] > res[(1, 2)]["newton_per_s"] else 2
+    r32 = res[(1, best_pc)]
+
+    # every form ran the same time steps from the same state: error against the FP64 run, relative to
+    # the FP64 field's largest magnitude
+    def rel_err(a, b):
+        return dict(head=float(np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()),
+                    velocity=float(np.abs(a[1] - b[1]).max() / np.abs(b[1]).max()))
+
+    def form_out(r, form):
+        out = {"value": round(r["newton_per_s"], 3), "newton_its": r["newton_its"], "lin_its": r["lin_its"],
+               "phases_ms": r["phases_ms"], "form": form,
+               "max_rel_error_vs_fp64_gpu": rel_err((r["head"], r["vel"]), (r64["head"], r64["vel"]))}
+        if o_state is not None:   # the oracle ran the first time step in FP64 from the same state
+            out["first_step_max_rel_error_vs_fp64_oracle"] = rel_err(r["first"], o_state)
+        return out
+
+    jroof["traffic"] = measured_traffic_kernel("k_jac2d_list<true")
     out = {
         "metric": "SW2 Newton steps/sec @1M tris, FP64 vs FP32-refined, 1e-8 rel residual",
         "value": round(r32["newton_per_s"], 3),
         "unit": "Newton steps/s",
         "n_gpus": 1,
-        "steps": r32["newton_its"],
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * r32["seconds"] / r32["newton_its"], 3),
+        "ms_per_step": round(1e3 * r32["seconds"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -362,22 +441,21 @@ def main():
         "config": {"workload": f"channel {args.ncx}x{args.ncy} cells, {p.nelems2d} triangles, dt 5 s, "
                                f"{args.steps} time steps, Newton to 1e-8 rel",
                    "nodes": p.nnodes, "triangles": p.nelems2d},
-        "fp64": {"value": round(r64["newton_per_s"], 3), "newton_its": r64["newton_its"],
-                 "lin_its": r64["lin_its"], "phases_ms": r64["phases_ms"], "roofline": roof[0]},
-        "fp32_refined": {"newton_its": r32["newton_its"], "lin_its": r32["lin_its"], "phases_ms": r32["phases_ms"],
-                         "list_kernel_shapes": r32["tuning"],
-                         "max_rel_error_vs_fp64": err,
-                         "form": "FP32 Jacobian storage, preconditioner and SpMV; FP64 Krylov vectors, residual, update"},
-        "fp32_all": {"value": round(res[2]["newton_per_s"], 3), "newton_its": res[2]["newton_its"],
-                     "lin_its": res[2]["lin_its"], "phases_ms": res[2]["phases_ms"],
-                     "max_rel_error_vs_fp64": err32, "roofline": roof[2],
-                     "form": "FP32 matrix and Krylov vectors; FP64 residual, update"},
-        "roofline": roof[1],
-        "jacobian_roofline": jroof,
+        "time_steps": args.steps,
+        "newton_its": r32["newton_its"],
+        "ms_per_newton_it": round(1e3 * r32["seconds"] / r32["newton_its"], 3),
+        "form": ("FP32 Jacobian storage, preconditioner and SpMV; FP64 Krylov vectors, residual, update; "
+                 + ("block ILU(0)" if best_pc == 3 else "3x3 block Jacobi")),
+        "fp64": form_out(r64, "FP64, 3x3 block Jacobi"),
+        "fp32_refined_block_jacobi": form_out(res[(1, 2)], "FP32 matrix and block-Jacobi inverses, FP64 vectors"),
+        "fp32_refined_ilu0": form_out(res[(1, 3)], "FP32 matrix and ILU(0) factors, FP64 vectors"),
+        "fp32_all": form_out(res[(2, 2)], "FP32 matrix and Krylov vectors; FP64 residual, update"),
+        "list_kernel_shapes": r32["tuning"],
+        "roofline": jroof,
+        "spmv_roofline": {"fp32_matrix_fp64_vectors": roof[1], "fp64": roof[0], "fp32": roof[2]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-
 
 if __name__ == "__main__":
     main()

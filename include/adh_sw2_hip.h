@@ -96,7 +96,9 @@ typedef struct {
                             2: FP32 matrix and Krylov; 1 and 2 form the FD Jacobian in FP64 and keep the
                             residual, update and Newton tests in FP64 */
     int preconditioner;  /* 2: 3x3 block Jacobi; 1: profile LU of node blocks (OP PRE 1 with OP BLK
-                            nblock, src/solver/solv_blk_*.c), FP64 only */
+                            nblock, src/solver/solv_blk_*.c), FP64 only; 3: block ILU(0) in multicolour
+                            order (no counterpart in the reference), FP64 or FP32 factors (precision
+                            0 or 1) */
     int check_every;     /* Krylov iterations before the first host convergence check of a solve that has
                             no previous count (later solves start from the previous count, then check every
                             2; with graph replay both are rounded up to multiples of 4) */
@@ -160,14 +162,24 @@ int adh_sw2_hip_elem_jacobians(adh_sw2_hip *h, double *em2d, double *em1d);
 /* fe_sw2_inc (src/fe/sw2/fe_sw2_inc.c:24-57); sol NULL = the device solution. */
 int adh_sw2_hip_inc(adh_sw2_hip *h, const double *sol);
 /* solv_linear_sys_setup (src/solver/solv_linear_sys_setup.c:15-74): Jacobi
- * scaling in place + preconditioner set-up; scale_vect[3N] optional.  Once per
- * system: a second call before the next load or set_system returns
- * ADH_SW2_EINVAL (the system is already scaled). */
-int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, double *scale_vect);
+ * scaling in place + preconditioner set-up; scale_vect[3N] optional.  x[3N]
+ * is the caller's solution array on entry, the shift the solve adds back to
+ * its result (solv_return_old.c:23-25); NULL = zero, what fe_newton passes
+ * (fe_newton.c:470).  Once per system: a second call before the next load or
+ * set_system returns ADH_SW2_EINVAL (the system is already scaled). */
+int adh_sw2_hip_linear_sys_setup(adh_sw2_hip *h, const double *x, double *scale_vect);
 /* solv_linear_sys_solve (src/solver/solv_linear_sys_solve.c:16-69):
  * BiCGSTAB (src/solver/solv_bcgstab.c:39-301) on the scaled system;
  * *verdict = YES/NO, sol[3N] optional, *lin_it optional. */
 int adh_sw2_hip_linear_sys_solve(adh_sw2_hip *h, double *sol, int *lin_it, int *verdict);
+/* y[3N] = M^-1 v[3N] with the preconditioner of the last set-up, as
+ * solv_prec (src/solver/solv_prec.c:11-66) applies it inside BiCGSTAB:
+ * the profile LU (solv_blk_solve.c:5-62) or the block ILU(0); for tests. */
+int adh_sw2_hip_precond_apply(adh_sw2_hip *h, const double *v, double *y);
+/* The node colouring of the block ILU(0) (preconditioner 3; no counterpart
+ * in the reference): color[N] in 0..*ncolor-1, no two nodes that share a
+ * matrix block alike.  Available after a set-up with preconditioner 3. */
+int adh_sw2_hip_ilu_colors(adh_sw2_hip *h, int *color, int *ncolor);
 /* Loads an explicit (unscaled) system: diagonal[9N], offdiag[9*nnz] in the
  * pattern above, rhs[3N]; sol is zeroed.  For solver-only use and tests. */
 int adh_sw2_hip_set_system(adh_sw2_hip *h, const double *diagonal, const double *offdiag, const double *rhs);
@@ -266,8 +278,8 @@ int adh_sw2_hip_time_kernel(adh_sw2_hip *h, int which, int reps, double *avg_ms)
 typedef struct {
     int block[4];      /* 0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general */
     int grid[4];
-    int waves[4];      /* register budget: 0 = the compiler's, 3 or 4 = at least that many waves
-                          per SIMD (Jacobian wet list only; 0 elsewhere) */
+    int layout[4];     /* Jacobian lists: 0 = a lane per (triangle, column, sign), 1 = waves that
+                          each take one column of 32 triangles (both signs); 0 for the residual lists */
     double ms[4];
 } adh_sw2_tuning;
 int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out);

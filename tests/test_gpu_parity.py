@@ -151,7 +151,8 @@ def test_setup_twice_is_refused_and_load_rearms_it(precision, stepped):
     """The set-up scales the system in place; a second set-up of the same
     system is refused with an error (the refined forms keep the scaled
     blocks in FP32 only), and a new load gives it a fresh system that solves
-    exactly as the first did."""
+    exactly as the first did: the set-up's x (None = zero, as fe_newton
+    passes it) replaces the previous solution as the solve's shift."""
     from adhv8_kraken_amd.sw2 import SW2Error
     name = "bumpy-60x30"
     p = CASES[name]
@@ -166,9 +167,14 @@ def test_setup_twice_is_refused_and_load_rearms_it(precision, stepped):
     g.init(); g.resid(); g.load()
     s2 = g.linear_sys_setup()
     vd2, x2, it2 = g.linear_sys_solve()
+    # a caller's x is the shift the solve adds back (solv_return_old.c:23-25)
+    g.init(); g.resid(); g.load()
+    g.linear_sys_setup(x=x1)
+    vd3, x3, it3 = g.linear_sys_solve()
     g.close()
-    assert vd1 == vd2 == YES and it1 == it2
+    assert vd1 == vd2 == vd3 == YES and it1 == it2 == it3
     assert bits_equal(s1, s2) and bits_equal(x1, x2)
+    assert np.abs(x3 - 2.0 * x1).max() <= 1e-14 * np.abs(x1).max()
 
 
 @pytest.mark.parametrize("nblock", [1, 3])
@@ -475,7 +481,8 @@ def test_autotuned_launch_shapes_change_nothing(name):
             for k in range(4):
                 t.block[k] = (64, 256, 64, 256)[k]
                 t.grid[k] = (3, 1, 7, 2)[k]
-            t.waves[0] = 4   # the tightest register budget (spills to scratch)
+            t.layout[0] = 1 - g.get_tuning().layout[0]   # the other Jacobian layout than the default, both lists
+            t.layout[1] = 1 - g.get_tuning().layout[1]
             g.set_tuning(t)
         r = g.timestep(max_retries=2)
         h, v, *_ = g.get_state()
