@@ -9,6 +9,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,7 +36,39 @@ static int fail(int code, const std::string &msg) {
 struct adh_sw2_hip {
     Ctx c;
     std::vector<void *> allocs;
+    int *scratch;   // this handle's device's copy of `scratch` (adh_sw2_hip_init_noalloc), or null
 };
+
+// ------------------------------------------- staging area of init_noalloc
+// adh_sw2_hip_init_noalloc stages the DOF boundary mask in this file-scope
+// array instead of a per-call device allocation.  Every GPU the library runs
+// on holds its own copy (24 MiB) until the process ends.  Host code never
+// uses the array's name as an address: create_impl looks up the current
+// device's copy once per device with hipGetSymbolAddress and the handle
+// keeps that address.  Calls on one GPU share the copy, so overlapping calls
+// on one GPU race; each call writes all 3N entries it reads back.
+constexpr size_t NOALLOC_MAX_NODES = (size_t)1 << 21;   // 2,097,152 nodes
+static __device__ __align__(256) int scratch[3 * NOALLOC_MAX_NODES];
+static std::mutex g_scratch_mu;
+static std::map<int, int *> g_scratch_addr;   // device ordinal -> address of its `scratch`
+
+// Binds h to the current device's staging area (looked up on the first
+// handle created on that device).
+static int bind_scratch(adh_sw2_hip *h) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(dev);
+    if (it == g_scratch_addr.end()) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ADH_SW2_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        it = g_scratch_addr.emplace(dev, (int *)p).first;
+    }
+    h->scratch = it->second;
+    return 0;
+}
 
 extern "C" const char *adh_sw2_hip_last_error(void) { return g_err.c_str(); }
 
@@ -507,6 +541,10 @@ static int create_impl(const adh_sw2_mesh *m, const adh_sw2_opts *opts, adh_sw2_
     D.mat_windatt = c.mat_windatt;
     D.density = m->density;
     HIPCHK(hipDeviceSynchronize());
+    if ((rc = bind_scratch(h))) {
+        adh_sw2_hip_destroy(h);
+        return rc;
+    }
     *out = h;
     return ADH_SW2_OK;
 }
@@ -671,6 +709,25 @@ extern "C" int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag) {
     if (dmask) (void)hipFree(dmask);
     if (!rc && wd_flag) HIPCHK(hipMemcpy(wd_flag, c.wd_flag, sizeof(int) * c.E, hipMemcpyDeviceToHost));
     return rc;
+}
+
+// adh_sw2_hip_init with the boundary mask staged in the device's `scratch`
+// array: no device allocation and no free.  The kernel writes all 3N mask
+// entries, so nothing left by an earlier call is read back.
+extern "C" int adh_sw2_hip_init_noalloc(adh_sw2_hip *h, int *bc_mask, int *wd_flag) {
+    if (h && h->c.ninst) return fail(ADH_SW2_EINVAL, "an ensemble handle takes the adh_sw2_hip_batch_* calls");
+    if (!h) return fail(ADH_SW2_EINVAL, "null handle");
+    Ctx &c = h->c;
+    if ((size_t)c.N > NOALLOC_MAX_NODES)
+        return fail(ADH_SW2_EINVAL, "init_noalloc: the model has " + std::to_string(c.N) +
+                                        " nodes, more than the 2097152 its staging area holds (use adh_sw2_hip_init)");
+    if (!h->scratch) return fail(ADH_SW2_EHIP, "init_noalloc: no staging area on this handle's device");
+    launch_init(&c, bc_mask ? h->scratch : nullptr);
+    int rc = sync_check(c);
+    if (rc) return rc;
+    if (bc_mask) HIPCHK(hipMemcpy(bc_mask, h->scratch, sizeof(int) * 3 * c.N, hipMemcpyDeviceToHost));
+    if (wd_flag) HIPCHK(hipMemcpy(wd_flag, c.wd_flag, sizeof(int) * c.E, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 // node vectors [3N] / diagonal blocks [9N] between node-major (host) and the

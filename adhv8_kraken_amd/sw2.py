@@ -98,6 +98,7 @@ def lib():
         L.adh_sw2_hip_get_state.argtypes = [C.c_void_p] + [PD] * 6
         L.adh_sw2_hip_shift.argtypes = [C.c_void_p]
         L.adh_sw2_hip_init.argtypes = [C.c_void_p, PI, PI]
+        L.adh_sw2_hip_init_noalloc.argtypes = [C.c_void_p, PI, PI]
         L.adh_sw2_hip_resid.argtypes = [C.c_void_p, PD]
         L.adh_sw2_hip_load.argtypes = [C.c_void_p, PD, PD, PD]
         L.adh_sw2_hip_elem_jacobians.argtypes = [C.c_void_p, PD, PD]
@@ -291,6 +292,13 @@ class SW2Model(_Tunable):
         bc = np.empty(3 * self.N, dtype=np.int32)
         wd = np.empty(self.p.nelems2d, dtype=np.int32)
         _check(self.L.adh_sw2_hip_init(self.h, _pi(bc), _pi(wd)))
+        return bc, wd
+
+    def init_noalloc(self):
+        """init through the entry point that allocates no device memory."""
+        bc = np.empty(3 * self.N, dtype=np.int32)
+        wd = np.empty(self.p.nelems2d, dtype=np.int32)
+        _check(self.L.adh_sw2_hip_init_noalloc(self.h, _pi(bc), _pi(wd)))
         return bc, wd
 
     def resid(self):

@@ -147,6 +147,13 @@ int adh_sw2_hip_get_state(adh_sw2_hip *h, double *head, double *vel, double *old
 int adh_sw2_hip_shift(adh_sw2_hip *h);
 /* fe_sw2_init (src/fe/sw2/fe_sw2_init.c:24-92) incl. fe_sw2_wdflag; bc_mask[3N] optional. */
 int adh_sw2_hip_init(adh_sw2_hip *h, int *bc_mask, int *wd_flag);
+/* adh_sw2_hip_init without any device allocation or free: the bc_mask is
+ * staged in a fixed per-GPU area of the library (24 MiB, models of up to
+ * 2,097,152 nodes; larger ones return ADH_SW2_EINVAL).  No counterpart in the
+ * reference: for callers that cannot allocate device memory, e.g. inside HIP
+ * graph capture or a serving loop that runs fe_sw2_init once per Newton
+ * attempt.  Calls that overlap on one GPU race on the staging area. */
+int adh_sw2_hip_init_noalloc(adh_sw2_hip *h, int *bc_mask, int *wd_flag);
 /* fe_sw2_resid (src/fe/sw2/fe_sw2_resid.c:29-216); residual[3N] optional. */
 int adh_sw2_hip_resid(adh_sw2_hip *h, double *residual);
 /* fe_sw2_load (src/fe/sw2/fe_sw2_load.c:34-269) incl. the diagonal floor;
