@@ -85,12 +85,16 @@ extern "C" void adh_sw2_hip_default_opts(adh_sw2_opts *o) {
     o->check_every = 8;
     o->nblock = 1;
     o->graphs = 0;  // measured: no gain over eager launches on MI355X for single models
+    o->krylov = 0;  // BiCGSTAB, as the reference
 }
 
 static int check_opts(const adh_sw2_opts *o) {
     if (o->precision < 0 || o->precision > 2) return fail(ADH_SW2_EINVAL, "precision must be 0, 1 or 2");
     if (o->preconditioner < 1 || o->preconditioner > 3)
         return fail(ADH_SW2_EUNSUPPORTED, "preconditioner must be 1 (profile LU), 2 (block Jacobi) or 3 (block ILU(0))");
+    if (o->krylov != 0 && o->krylov != 1) return fail(ADH_SW2_EINVAL, "krylov must be 0 (BiCGSTAB) or 1 (GMRES)");
+    if (o->krylov == 1 && o->precision == 2)
+        return fail(ADH_SW2_EUNSUPPORTED, "GMRES runs with FP64 Krylov vectors (precision 0 or 1)");
     if (o->preconditioner == 3 && o->precision == 2)
         return fail(ADH_SW2_EUNSUPPORTED, "the ILU(0) preconditioner runs with FP64 Krylov vectors (precision 0 or 1)");
     if (o->preconditioner == 1 && o->precision != 0)
@@ -560,6 +564,7 @@ extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     drop_graphs(c);
     band_free(&c);
     ilu_free(&c);
+    gmres_free(&c);
     for (void *p : h->allocs) (void)hipFree(p);
     if (c.h_ksc) (void)hipHostFree(c.h_ksc);
     if (c.h_norms) (void)hipHostFree(c.h_norms);
@@ -934,6 +939,18 @@ static void chunk_plan(const Ctx &c, int &first, int &more) {
 // the epilogue is idempotent and is queued behind every chunk before the
 // convergence flag is read; otherwise it runs once, after the last chunk.
 static int krylov(Ctx &c, int *lin_it, int *verdict, bool zero_guess) {
+    if (c.opts.krylov == 1) {   // restarted GMRES: host-driven cycles
+        std::string err = gmres_alloc(&c);
+        if (!err.empty()) return fail(ADH_SW2_ENOMEM, err);
+        bool conv = false;
+        const int it = launch_gmres(&c, lin_it_cap(c), &conv);
+        HIPCHK(hipGetLastError());
+        launch_krylov_epilogue(&c, zero_guess);
+        c.last_lin_it = it;
+        if (lin_it) *lin_it = it;
+        *verdict = conv ? ADH_SW2_YES : ADH_SW2_NO;
+        return 0;
+    }
     KScal init{};
     init.atol = c.opts.lin_atol;
     init.rtol = c.opts.lin_rtol;
@@ -1401,6 +1418,7 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
     if (!meshes || n < 1 || !out) return fail(ADH_SW2_EINVAL, "null argument or empty ensemble");
     if (opts && opts->preconditioner != 2)
         return fail(ADH_SW2_EUNSUPPORTED, "ensembles use the 3x3 block-Jacobi preconditioner");
+    if (opts && opts->krylov != 0) return fail(ADH_SW2_EUNSUPPORTED, "ensembles use BiCGSTAB");
     int rc;
     for (int i = 0; i < n; i++)
         if ((rc = validate(meshes + i))) return rc;

@@ -170,6 +170,11 @@ struct Ctx {
     int4 *ilu_upd;
     double *ilu_Lw, *ilu_Uw, *ilu_Dw, *ilu_Dinv, *ilu_w;   // FP64 factor work, colour-ordered vector
     void *ilu_L, *ilu_U, *ilu_D;           // the apply's factors (FP64 or FP32) and U_ii^-1
+    // restarted GMRES (opts.krylov = 1), allocated on first use
+    double *gm_V;          // basis, GMRES restart + 1 vectors of 3 Np
+    double *gm_part;       // block partials of the multi-dots
+    double *gm_dev;        // coefficients of the two Gram-Schmidt passes, ||w||^2, the update's y
+    double *gm_host;       // pinned copy of gm_dev
 };
 
 // 3x3 block inverse by Gauss-Jordan with partial pivoting (block-Jacobi set-up)
@@ -266,6 +271,9 @@ std::string ilu_structure(Ctx *c);
 void ilu_free(Ctx *c);
 void launch_ilu_setup(Ctx *c);
 void launch_ilu_solve(Ctx *c, double *x, const KScal *sc);
+std::string gmres_alloc(Ctx *c);
+void gmres_free(Ctx *c);
+int launch_gmres(Ctx *c, int max_it, bool *converged);
 std::string band_structure(Ctx *c, int nblock);
 void band_free(Ctx *c);
 void launch_band_setup(Ctx *c);

@@ -20,6 +20,8 @@
 // didx9(n, e), off-diagonal blocks in SELL-64 (sell_vidx), node vectors at
 // vidx3(n, k); every per-node load of a wavefront is one 64-wide contiguous
 // access.  SpMV reads 72 B (FP64) per block.
+#include <cmath>
+#include <string>
 #include <type_traits>
 
 #include "sw2_ctx.h"
@@ -830,6 +832,282 @@ void launch_krylov_epilogue(Ctx *c, bool zero_guess) {
         if (zero_guess) hipLaunchKernelGGL((k_bcg_epi<double, true>), g, dim3(256), 0, c->stream, n3, c->ku, c->scale, c->sol);
         else hipLaunchKernelGGL((k_bcg_epi<double, false>), g, dim3(256), 0, c->stream, n3, c->ku, c->scale, c->sol);
     }
+}
+
+}  // namespace sw2
+
+// ===========================================================================
+// Restarted GMRES(m), left-preconditioned (opts.krylov = 1).  No counterpart
+// in the reference's serial path, which solves with BiCGSTAB only
+// (solv_linear_sys_solve.c -> solv_bcgstab.c); it is the north star's
+// alternative Krylov method and uses the same SpMV, preconditioners and
+// scaled system.  Arnoldi with classical Gram-Schmidt applied twice (CGS2):
+// each pass is one multi-dot kernel (all (v_i, w) of the basis at once, block
+// partials reduced in block order by a one-block finisher) and one update
+// kernel, so an iteration reads the basis twice instead of once per vector.
+// The small Hessenberg least-squares problem (Givens rotations) is solved on
+// the host from the coefficients copied back once per iteration.  Stop: the
+// 2-norm of the preconditioned residual |g_{j+1}| <= lin_atol +
+// lin_rtol * ||M^-1 b||_2 (BiCGSTAB's rule with the Arnoldi residual in place
+// of its max-norm), or lin_it_cap iterations in total.
+namespace sw2 {
+
+constexpr int GM_RESTART = 30;              // Krylov vectors per cycle
+constexpr int GM_MAX = GM_RESTART + 1;      // basis vectors stored
+
+// (v_i, w) for i < nv: per-block partials, i at part[GM_MAX * block + i]
+__global__ void __launch_bounds__(256) k_gm_mdot(int N, const double *V, size_t ld, int nv, const double *w,
+                                                 double *part) {
+    __shared__ double sh[GM_MAX][4];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+    if (n < N) { w0 = w[vidx3(n, 0)]; w1 = w[vidx3(n, 1)]; w2 = w[vidx3(n, 2)]; }
+    for (int i = 0; i < nv; i++) {
+        double x = 0.0;
+        if (n < N) {
+            const double *v = V + ld * i;
+            x = v[vidx3(n, 0)] * w0;
+            x += v[vidx3(n, 1)] * w1;
+            x += v[vidx3(n, 2)] * w2;
+        }
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) sh[i][wid] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv) {
+        double s = sh[threadIdx.x][0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) s += sh[threadIdx.x][k];
+        part[GM_MAX * (size_t)blockIdx.x + threadIdx.x] = s;
+    }
+}
+
+// out[i] = sum of the block partials of value i, in block order (one block)
+__global__ void __launch_bounds__(256) k_gm_fin(int nparts, const double *part, int nv, double *out) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = 0; i < nv; i++) {
+        double x = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) x += part[GM_MAX * (size_t)b + i];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) sh[wid] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = sh[0];
+            for (int k = 1; k < (int)(blockDim.x >> 6); k++) s += sh[k];
+            out[i] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// w -= sum_{i < nv} h_i v_i; with `norm`, also the block partials of (w, w) at value 0
+__global__ void __launch_bounds__(256) k_gm_update(int N, const double *V, size_t ld, int nv, const double *h,
+                                                   double *w, double *part, int norm) {
+    __shared__ double sh[4];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0.0;
+    if (n < N) {
+        for (int k = 0; k < 3; k++) {
+            const size_t q = vidx3(n, k);
+            double a = w[q];
+            for (int i = 0; i < nv; i++) a -= h[i] * V[ld * i + q];
+            w[q] = a;
+            x += a * a;
+        }
+    }
+    if (!norm) return;
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = sh[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) s += sh[k];
+        part[GM_MAX * (size_t)blockIdx.x] = s;
+    }
+}
+
+// v = w / sqrt(nrm2[0])  (the next basis vector)
+__global__ void k_gm_scale(size_t n3, const double *w, const double *nrm2, double *v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) v[i] = w[i] / sqrt(nrm2[0]);
+}
+
+// r = b - y (restart residual), y in place
+__global__ void k_gm_sub(size_t n3, const double *b, double *y) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) y[i] = b[i] - y[i];
+}
+
+// x += sum_{i < nv} y_i v_i
+__global__ void k_gm_xupd(size_t n3, const double *V, size_t ld, int nv, const double *y, double *x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double a = x[i];
+    for (int k = 0; k < nv; k++) a += y[k] * V[ld * k + i];
+    x[i] = a;
+}
+
+// x <- D^-1 x with the block-Jacobi inverses (storage type MT)
+template <typename MT>
+__global__ void k_bj_apply(int N, const MT *dinv, double *x) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double a0 = x[vidx3(n, 0)], a1 = x[vidx3(n, 1)], a2 = x[vidx3(n, 2)];
+    const MT *M = dinv + didx9(n, 0);
+    x[vidx3(n, 0)] = M[0] * a0 + M[64] * a1 + M[128] * a2;
+    x[vidx3(n, 1)] = M[192] * a0 + M[256] * a1 + M[320] * a2;
+    x[vidx3(n, 2)] = M[384] * a0 + M[448] * a1 + M[512] * a2;
+}
+
+static void gm_prec(Ctx *c, double *x) {
+    if (c->opts.preconditioner == 2) {
+        if (c->opts.precision >= 1)
+            hipLaunchKernelGGL(k_bj_apply<float>, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N,
+                               c->dinv32, x);
+        else
+            hipLaunchKernelGGL(k_bj_apply<double>, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N,
+                               c->dinv, x);
+    } else {
+        prec_solve(c, x, nullptr);
+    }
+}
+
+template <typename P, int K>
+static void gm_spmv_k(Ctx *c, const double *x, double *y) {
+    hipLaunchKernelGGL((k_pl_spmv<P, K, false>), dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N,
+                       vecs<P>(c), c->sell_ptr, c->sell_col, x, y, c->ksc, 0);
+}
+
+// y = M^-1 A x
+static void gm_op(Ctx *c, const double *x, double *y) {
+    if (c->opts.precision >= 1) { SW2_KSWITCH(gm_spmv_k, PrecMix, c, x, y) }
+    else { SW2_KSWITCH(gm_spmv_k, PrecF64, c, x, y) }
+    gm_prec(c, y);
+}
+
+// ||w||_2^2 of a vector into dev[0] (via the update kernel with no basis vectors)
+static void gm_norm2(Ctx *c, double *w, double *dev) {
+    const int nb = nblocks_for(c->N, 256);
+    hipLaunchKernelGGL(k_gm_update, dim3(nb), dim3(256), 0, c->stream, c->N, (const double *)nullptr, (size_t)0, 0,
+                       (const double *)nullptr, w, c->gm_part, 1);
+    hipLaunchKernelGGL(k_gm_fin, dim3(1), dim3(256), 0, c->stream, nb, c->gm_part, 1, dev);
+}
+
+std::string gmres_alloc(Ctx *c) {
+    if (c->gm_V) return "";
+    const size_t ld = 3 * (size_t)c->Np;
+    const int nb = nblocks_for(c->N, 256);
+    if (hipMalloc((void **)&c->gm_V, sizeof(double) * ld * GM_MAX) != hipSuccess ||
+        hipMalloc((void **)&c->gm_part, sizeof(double) * GM_MAX * (size_t)(nb + 8)) != hipSuccess ||
+        hipMalloc((void **)&c->gm_dev, sizeof(double) * 4 * GM_MAX) != hipSuccess ||
+        hipHostMalloc((void **)&c->gm_host, sizeof(double) * 4 * GM_MAX) != hipSuccess) {
+        gmres_free(c);
+        return "hipMalloc failed for the GMRES basis";
+    }
+    (void)hipMemset(c->gm_V, 0, sizeof(double) * ld * GM_MAX);
+    return "";
+}
+
+void gmres_free(Ctx *c) {
+    if (c->gm_V) (void)hipFree(c->gm_V);
+    if (c->gm_part) (void)hipFree(c->gm_part);
+    if (c->gm_dev) (void)hipFree(c->gm_dev);
+    if (c->gm_host) (void)hipHostFree(c->gm_host);
+    c->gm_V = c->gm_part = c->gm_dev = c->gm_host = nullptr;
+}
+
+// Solves the scaled system into ku (x, zero initial guess); returns the
+// iteration count and whether the stop rule was met.  hip errors surface
+// through the caller's hipGetLastError / synchronisation.
+int launch_gmres(Ctx *c, int max_it, bool *converged) {
+    const size_t ld = 3 * (size_t)c->Np, n3 = ld;
+    const int nb = nblocks_for(c->N, 256), nbv = nblocks_for((int)n3, 256);
+    double *V = c->gm_V, *w = c->kAp, *bp = c->kb, *x = c->ku;
+    double *c1 = c->gm_dev, *c2 = c->gm_dev + GM_MAX, *nrm = c->gm_dev + 2 * GM_MAX, *ydev = c->gm_dev + 3 * GM_MAX;
+    double *hh = c->gm_host;
+    // b' = M^-1 b, x = 0
+    hipLaunchKernelGGL(k_pl_copy, dim3(nbv), dim3(256), 0, c->stream, n3, c->R, bp);
+    gm_prec(c, bp);
+    launch_zero(c, x, n3);
+    gm_norm2(c, bp, nrm);
+    (void)hipMemcpyAsync(hh, nrm, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    const double bnorm = std::sqrt(hh[0]);
+    const double tol = c->opts.lin_atol + c->opts.lin_rtol * bnorm;
+    *converged = bnorm <= tol;
+    int it = 0;
+    double H[GM_MAX][GM_RESTART], cs[GM_RESTART], sn[GM_RESTART], g[GM_MAX];
+    bool first = true;
+    while (!*converged && it < max_it) {
+        // r = b' - M^-1 A x (the first cycle: r = b'), v0 = r / ||r||
+        if (first) {
+            hipLaunchKernelGGL(k_pl_copy, dim3(nbv), dim3(256), 0, c->stream, n3, bp, w);
+        } else {
+            gm_op(c, x, w);
+            hipLaunchKernelGGL(k_gm_sub, dim3(nbv), dim3(256), 0, c->stream, n3, bp, w);
+        }
+        gm_norm2(c, w, nrm);
+        hipLaunchKernelGGL(k_gm_scale, dim3(nbv), dim3(256), 0, c->stream, n3, w, nrm, V);
+        (void)hipMemcpyAsync(hh, nrm, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        const double beta = first ? bnorm : std::sqrt(hh[0]);
+        first = false;
+        if (beta <= tol) { *converged = true; break; }
+        for (int i = 0; i < GM_MAX; i++) g[i] = 0.0;
+        g[0] = beta;
+        int j = 0;
+        for (; j < GM_RESTART && it < max_it; j++) {
+            const int nv = j + 1;
+            gm_op(c, V + ld * j, w);
+            hipLaunchKernelGGL(k_gm_mdot, dim3(nb), dim3(256), 0, c->stream, c->N, V, ld, nv, w, c->gm_part);
+            hipLaunchKernelGGL(k_gm_fin, dim3(1), dim3(256), 0, c->stream, nb, c->gm_part, nv, c1);
+            hipLaunchKernelGGL(k_gm_update, dim3(nb), dim3(256), 0, c->stream, c->N, V, ld, nv, c1, w, c->gm_part, 0);
+            hipLaunchKernelGGL(k_gm_mdot, dim3(nb), dim3(256), 0, c->stream, c->N, V, ld, nv, w, c->gm_part);
+            hipLaunchKernelGGL(k_gm_fin, dim3(1), dim3(256), 0, c->stream, nb, c->gm_part, nv, c2);
+            hipLaunchKernelGGL(k_gm_update, dim3(nb), dim3(256), 0, c->stream, c->N, V, ld, nv, c2, w, c->gm_part, 1);
+            hipLaunchKernelGGL(k_gm_fin, dim3(1), dim3(256), 0, c->stream, nb, c->gm_part, 1, nrm);
+            hipLaunchKernelGGL(k_gm_scale, dim3(nbv), dim3(256), 0, c->stream, n3, w, nrm, V + ld * (j + 1));
+            (void)hipMemcpyAsync(hh, c1, sizeof(double) * (2 * GM_MAX + 1), hipMemcpyDeviceToHost, c->stream);
+            (void)hipStreamSynchronize(c->stream);
+            it++;
+            for (int i = 0; i < nv; i++) H[i][j] = hh[i] + hh[GM_MAX + i];
+            const double hn = std::sqrt(hh[2 * GM_MAX]);   // ||w|| after orthogonalisation
+            H[nv][j] = hn;
+            for (int i = 0; i < j; i++) {   // previous rotations
+                const double a = H[i][j], b = H[i + 1][j];
+                H[i][j] = cs[i] * a + sn[i] * b;
+                H[i + 1][j] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = H[j][j], b = H[j + 1][j];
+            const double r = std::hypot(a, b);
+            cs[j] = r > 0.0 ? a / r : 1.0;
+            sn[j] = r > 0.0 ? b / r : 0.0;
+            H[j][j] = r;
+            H[j + 1][j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            // stop on the tolerance, or on a breakdown (w in the span: the
+            // subspace holds the solution)
+            const bool ok = std::fabs(g[j + 1]) <= tol;
+            if (ok || hn == 0.0) {
+                *converged = ok || hn == 0.0;
+                j++;
+                break;
+            }
+        }
+        // x += V y, H y = g (upper triangular, j columns)
+        double y[GM_RESTART];
+        for (int i = j - 1; i >= 0; i--) {
+            double s = g[i];
+            for (int k = i + 1; k < j; k++) s -= H[i][k] * y[k];
+            y[i] = s / H[i][i];
+        }
+        (void)hipMemcpyAsync(ydev, y, sizeof(double) * j, hipMemcpyHostToDevice, c->stream);
+        hipLaunchKernelGGL(k_gm_xupd, dim3(nbv), dim3(256), 0, c->stream, n3, V, ld, j, ydev, x);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    return it;
 }
 
 }  // namespace sw2

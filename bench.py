@@ -424,7 +424,8 @@ def main():
             out["first_step_max_rel_error_vs_fp64_oracle"] = rel_err(r["first"], o_state)
         return out
 
-    jroof["traffic"] = measured_traffic_kernel("k_jac2d_list<true")
+    # the counter table was taken on the 1000x500-cell channel
+    jroof["traffic"] = measured_traffic_kernel("k_jac2d_list<true") if (p.nnodes, p.nelems2d) == (501501, 1000000) else None
     out = {
         "metric": "SW2 Newton steps/sec @1M tris, FP64 vs FP32-refined, 1e-8 rel residual",
         "value": round(r32["newton_per_s"], 3),

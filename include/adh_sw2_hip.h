@@ -104,6 +104,9 @@ typedef struct {
                             2; with graph replay both are rounded up to multiples of 4) */
     int nblock;          /* OP BLK: node blocks of the profile-LU preconditioner (default 1) */
     int graphs;          /* 1: replay captured HIP graphs of Krylov iteration chunks; 0 (default): launch eagerly */
+    int krylov;          /* 0 (default): BiCGSTAB (solv_bcgstab.c); 1: restarted GMRES(30), left-preconditioned,
+                            stopping on the Arnoldi 2-norm of the preconditioned residual (no counterpart in the
+                            reference; single models, precision 0 or 1) */
 } adh_sw2_opts;
 
 typedef struct {
