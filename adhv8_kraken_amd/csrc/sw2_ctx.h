@@ -99,7 +99,7 @@ struct Ctx {
     // launch shape of the four list kernels (Jacobian wet/general, residual
     // wet/general): block size and grid-stride grid size
     struct Tuning {
-        int bs[4] = {128, 128, 128, 128};
+        int bs[4] = {256, 128, 128, 128};
         int grid[4] = {4096, 1024, 2048, 512};
         int layout[4] = {0, 0, 0, 0};   // Jacobian lists: 0 a lane per (triangle, column, sign), 1 column-uniform waves
     } tune;
