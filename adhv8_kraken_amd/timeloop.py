@@ -76,3 +76,52 @@ def run(model, t0, tf, dt_times, dt_values, on_step=None, max_attempts=30):
         if dt_new < dt:
             dt = dt_new
         dt = _land_on_final(t, dt, tf)
+
+
+def run_ensemble(ens, tf, dt_max, wall_s=None, max_retries=3, on_call=None):
+    """The same time control for every model of an ensemble (`SW2Ensemble`),
+    each with its own time and dt: the models not yet at tf take one step
+    together per call (up to max_retries + 1 attempts, each failure cutting
+    that model's dt by DT_REDUCE_FACTOR in the library, src/tools/tc_scale.c);
+    a model whose attempts all failed stays where it was with the cut dt,
+    which is how the reference retries until fe_main succeeds
+    (src/main/adh_func.c:535-549); after a success its time advances and dt
+    doubles, capped at dt_max and landed on tf (tc_end.c:5-50).  Stops when
+    every model reached tf or after wall_s seconds.  Returns per-model times,
+    attempt and failure counts, Newton iterations and the smallest dt used."""
+    import time
+    n = len(ens)
+    t = np.zeros(n)
+    dt = np.full(n, float(dt_max))
+    for i in range(n):
+        dt[i] = _land_on_final(0.0, dt[i], tf)
+        ens.set_dt(i, dt[i])
+    attempts = np.zeros(n, dtype=np.int64)
+    failed = np.zeros(n, dtype=np.int64)
+    newton = np.zeros(n, dtype=np.int64)
+    min_dt = float(dt_max)
+    calls = 0
+    t0 = time.perf_counter()
+    while (t < tf).any():
+        if wall_s is not None and time.perf_counter() - t0 > wall_s:
+            break
+        which = (t < tf).astype(np.int32)
+        rs = ens.timestep(max_retries=max_retries, which=which)
+        calls += 1
+        for i in np.nonzero(which)[0]:
+            r = rs[i]
+            newton[i] += r["iterations"]
+            attempts[i] += len(r["hist_max"]) - r["iterations"]
+            used = ens.get_dt(i)
+            min_dt = min(min_dt, used)
+            if r["verdict"] == 1:
+                t[i] = tf if t[i] + used >= tf else t[i] + used
+                if t[i] < tf:
+                    nd = min(used * DT_ENLARGE_FACTOR, dt_max)
+                    ens.set_dt(i, _land_on_final(t[i], nd, tf))
+            else:
+                failed[i] += 1
+        if on_call is not None:
+            on_call(calls, t, failed)
+    return dict(t=t, attempts=attempts, failed_calls=failed, newton=newton, min_dt=min_dt, calls=calls,
+                seconds=time.perf_counter() - t0)

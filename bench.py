@@ -295,6 +295,8 @@ def ensemble_mode(args):
                 "rain 1e-5 m/s, dt 2 s (cut by 0.25 per failed attempt, up to 2 retries)")
     ntri = sum(p.nelems2d for p in ps)
     log(f"{args.workload}: {len(ps)} instances, {ntri} triangles, {sum(p.nnodes for p in ps)} nodes")
+    if args.time_span > 0:
+        return time_control_mode(args, ps, forms[0], desc, ntri)
     res = {}
     for prec in forms:
         log(f"form precision={prec}")
@@ -344,6 +346,39 @@ def ensemble_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def time_control_mode(args, ps, precision, desc, ntri):
+    """Every instance integrated to t = --time-span with the reference's time
+    control, capped at --wall seconds: reports how far the instances got."""
+    from adhv8_kraken_amd.sw2 import SW2Ensemble, default_opts
+    from adhv8_kraken_amd import timeloop
+    ens = SW2Ensemble(ps, default_opts(precision=precision, max_nonlin_it=40))
+    r0 = ens.residual_norm()
+    for i, p in enumerate(ps):
+        ens.set_state(i, p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+        ens.set_tol(i, 1e-8 * r0[i])
+    dt_max = ps[0].dt
+
+    def progress(calls, t, failed):
+        log(f"  call {calls}: t min {t.min():.4g} s, max {t.max():.4g} s, at the end "
+            f"{int((t >= args.time_span).sum())}/{len(t)}, calls with every attempt failed {int(failed.sum())}")
+    res = timeloop.run_ensemble(ens, args.time_span, dt_max, wall_s=args.wall, on_call=progress)
+    ens.close()
+    tris = np.array([p.nelems2d for p in ps], dtype=np.float64)
+    out = {
+        "metric": "SW2 ensemble under the reference's time control: instances reaching the end time",
+        "value": int((res["t"] >= args.time_span).sum()), "unit": "instances", "n_gpus": 1,
+        "higher_is_better": True, "data": "synthetic", "dtype": "f64" if precision == 0 else "f32+f64",
+        "config": {"workload": desc, "instances": len(ps), "triangles": ntri, "end_time_s": args.time_span,
+                   "dt_max_s": dt_max, "wall_cap_s": args.wall},
+        "seconds": round(res["seconds"], 1), "calls": res["calls"],
+        "t_reached_min_s": float(res["t"].min()), "t_reached_median_s": float(np.median(res["t"])),
+        "t_reached_max_s": float(res["t"].max()), "smallest_dt_s": res["min_dt"],
+        "attempts": int(res["attempts"].sum()), "calls_with_all_attempts_failed": int(res["failed_calls"].sum()),
+        "tri_newton_per_s": float((res["newton"] * tris).sum() / max(res["seconds"], 1e-9)),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def kernel_mode(p):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts
     out = {}
@@ -375,6 +410,10 @@ def main():
                     help="channel: the 1M-triangle headline; the ensembles print their own JSON line")
     ap.add_argument("--compare", type=int, default=0,
                     help="ensembles: also run the first K instances one by one through the single-model path")
+    ap.add_argument("--time-span", type=float, default=0.0,
+                    help="ensembles: run the reference's time control (retry with dt/4 until a step succeeds, "
+                         "regrow dt x2 to the deck's dt) for every instance up to this simulated time")
+    ap.add_argument("--wall", type=float, default=300.0, help="wall-clock cap of --time-span runs, seconds")
     ap.add_argument("--kernels", action="store_true",
                     help="profiling mode: one time step per form, then the K1 kernel, the FD Jacobian "
                          "and the calibration copy launched back to back (for rocprofv3 counter passes)")
