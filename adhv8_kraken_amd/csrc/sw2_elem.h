@@ -22,9 +22,11 @@
 #ifdef __HIPCC__
 #define HD __host__ __device__ inline
 #define HDF __host__ __device__ __forceinline__
+#define HDM __host__ __device__ __forceinline__
 #else
 #define HD static inline
 #define HDF static inline
+#define HDM inline
 #endif
 
 namespace sw2 {
@@ -558,21 +560,58 @@ HD void elem_old_terms(const ElemData &D, int ie, double *pre) {
 // for the lowest-index element that touches it.  Old/older-state terms come
 // from `pre` (elem_old_terms); the reference's wet/dry average of the old
 // state (:248) feeds only debug output and is not formed.
+// Where body_resid reads one triangle's inputs: the model arrays themselves
+// (GlobalSrc), or a copy a kernel has staged in LDS (sw2_kernels.hip).  Node
+// fields are addressed by the triangle's local node 0..2.
+struct GlobalSrc {
+    const ElemData &D;
+    const int *nd;
+    int ie;
+    const double *pre_;
+    HDM double x(int i) const { return D.x[nd[i]]; }
+    HDM double y(int i) const { return D.y[nd[i]]; }
+    HDM double z(int i) const { return D.z[nd[i]]; }
+    HDM double head(int i) const { return D.head[nd[i]]; }
+    HDM double u(int i) const { return D.u[nd[i]]; }
+    HDM double v(int i) const { return D.v[nd[i]]; }
+    HDM int node_bc(int i) const { return D.node_bc[nd[i]]; }
+    HDM double grad(int k) const { return D.grad[6 * ie + k]; }
+    HDM double djac() const { return D.djac[ie]; }
+    HDM int mat() const { return D.mat[ie]; }
+    HDM int wd_flag() const { return D.wd_flag[ie]; }
+    HDM double ks() const { return D.ks[ie]; }
+    HDM int src_flag() const { return D.src_flag[ie]; }
+    HDM double src() const { return D.src[ie]; }
+    HDM double pre(int k) const { return pre_[k]; }
+};
+
+template <bool WET = false, typename SRC>
+HD void body_resid_src(const ElemData &D, const SRC &S, int ie, double pert, int pnode, int pvar, int psign,
+                       bool dry_rule, d3 *elem_rhs, const int *nd);
+
 template <bool WET = false>
 HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, int pnode, int pvar, int psign,
                    bool dry_rule, d3 *elem_rhs, const int *ndp = nullptr) {
-    int PRESSURE_FLAG = 1;
     // ndp: the triangle's node ids when the caller already holds them
     const int nd[3] = {ndp ? ndp[0] : D.e2d[3 * ie], ndp ? ndp[1] : D.e2d[3 * ie + 1],
                        ndp ? ndp[2] : D.e2d[3 * ie + 2]};
+    body_resid_src<WET>(D, GlobalSrc{D, nd, ie, pre}, ie, pert, pnode, pvar, psign, dry_rule, elem_rhs, nd);
+}
+
+// The body residual with its inputs read through S (GlobalSrc or a staged
+// copy); nd is only used by the dry-node rule and the winds.
+template <bool WET, typename SRC>
+HD void body_resid_src(const ElemData &D, const SRC &S, int ie, double pert, int pnode, int pvar, int psign,
+                       bool dry_rule, d3 *elem_rhs, const int *nd) {
+    int PRESSURE_FLAG = 1;
     v2 gs[3];
     for (int i = 0; i < 3; i++) {
-        gs[i].x = D.grad[6 * ie + 2 * i];
-        gs[i].y = D.grad[6 * ie + 2 * i + 1];
+        gs[i].x = S.grad(2 * i);
+        gs[i].y = S.grad(2 * i + 1);
     }
-    const double alpha = D.alpha, dt = elem_dt(D, ie), djac = D.djac[ie], g = D.g, dll = D.dll, dul = D.dul;
-    const int imat = D.mat[ie];
-    const int wd_flag = D.wd_flag[ie];
+    const double alpha = D.alpha, dt = elem_dt(D, ie), djac = S.djac(), g = D.g, dll = D.dll, dul = D.dul;
+    const int imat = S.mat();
+    const int wd_flag = S.wd_flag();
     double t1;
 
     double node_z[3];
@@ -581,12 +620,11 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     v2 vel[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        int n = nd[i];
-        nds[i].x = D.x[n]; nds[i].y = D.y[n]; nds[i].z = D.z[n];
+        nds[i].x = S.x(i); nds[i].y = S.y(i); nds[i].z = S.z(i);
         node_z[i] = nds[i].z;
-        h[i] = D.head[n];
-        double un = D.u[n], vn = D.v[n];
-        if (dry_rule && D.old_head[n] <= 0. && D.first_elem[n] != ie) { un = 0.; vn = 0.; }
+        h[i] = S.head(i);
+        double un = S.u(i), vn = S.v(i);
+        if (dry_rule && D.old_head[nd[i]] <= 0. && D.first_elem[nd[i]] != ie) { un = 0.; vn = 0.; }
         vel[i].x = un; vel[i].y = vn;
     }
     if (pvar == PERTURB_U || pvar == PERTURB_V) PRESSURE_FLAG = 0;
@@ -622,7 +660,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     const double vst_j = elem_v_avg / dmax(vmag, 1.E-8);
 
     double factor;
-    const double factor_old = pre[18];
+    const double factor_old = S.pre(18);
     double h_wd = elem_h_avg;
     v2 vwd;
     wd_average<WET>(nds, h, vel, djac, &h_wd, &vwd);
@@ -630,7 +668,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     const double vwd_mag = sqrt(vwd.x * vwd.x + vwd.y * vwd.y);
 
     const double h_fric = dmax(h_wd, SMALL);
-    const double ks = D.ks[ie];
+    const double ks = S.ks();
     const double roughness = ks > SMALL ? drag_coef(h_fric, ks, D.logt) : 0.0;
 
     double ev_st = 0., ev_tr = 0.;
@@ -705,12 +743,11 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     sw2_temporal<WET>(nds, djac, dll, h, vel, factor, (1. + alpha / 2.), elem_rhs, -3);
 #pragma unroll
     for (int k = 0; k < 2; k++) {   // t(n) and t(n-1) terms, fe_sw2_temporal with the element's wd_flag
-        const double *t = pre + 9 * k;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            if (wd_flag != 2) elem_rhs[i].c_eq += t[i];
-            elem_rhs[i].x_eq += t[3 + i];
-            elem_rhs[i].y_eq += t[6 + i];
+            if (wd_flag != 2) elem_rhs[i].c_eq += S.pre(9 * k + i);
+            elem_rhs[i].x_eq += S.pre(9 * k + 3 + i);
+            elem_rhs[i].y_eq += S.pre(9 * k + 6 + i);
         }
     }
 
@@ -803,8 +840,8 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
 
     // rainfall :728-754
     double src = 0.;
-    if (D.src_flag[ie]) {
-        src = D.src[ie];
+    if (S.src_flag()) {
+        src = S.src();
         for (int i = 0; i < 3; i++) rcq[i] = 0.;
         t1 = src * djac * one_3;   // integrate_triangle_phi :159-164
         for (int i = 0; i < 3; i++) rcq[i] += t1;
@@ -849,7 +886,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
         }
         d3 dfdt[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
         d3 adv[3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-        double dudt = pre[19], dvdt = pre[20], dhdt = pre[21];   // -(old-state integrals), :1012-1013
+        double dudt = S.pre(19), dvdt = S.pre(20), dhdt = S.pre(21);   // -(old-state integrals), :1012-1013
         factor = wd_wrapper<WD_INTEGRATE_F, false, true, true, false, WET>(dfdt, nds, h, nullptr, v2{0., 0.}, vel, h, djac, 1, 1);
         dudt += dfdt[0].x_eq; dvdt += dfdt[0].y_eq; dhdt += dfdt[0].c_eq;
         factor = wd_wrapper<WD_GLS_CONVECTION, true, true, true, false, WET>(adv, nds, h, gs, v2{0., 0.}, vel, h, djac, dt, 1);
@@ -876,7 +913,7 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
 
     // Dirichlet :1239-1256
     for (int i = 0; i < 3; i++) {
-        int bc = D.node_bc[nd[i]];
+        int bc = S.node_bc(i);
         if (bc == BCT_PRS_DIR) {
             elem_rhs[i].c_eq = 0.0;
         } else if (bc == BCT_VEL_DIR) {

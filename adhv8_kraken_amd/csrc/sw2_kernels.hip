@@ -266,6 +266,167 @@ __global__ void __launch_bounds__(BS) k_jac2d_cols(ElemData D, const int *list, 
     }
 }
 
+// The column-uniform layout with every triangle's inputs staged in LDS
+// first.  Each wave copies the inputs of its 32 triangles -- node
+// coordinates, depths and velocities, P1 gradients, the 24 old-state terms,
+// djac, roughness, rain, material, wet/dry flag and the nodes' boundary flags
+// -- with 29 loads per lane issued back to back (lanes l and l + 32 share a
+// triangle and take alternate fields), waits once, and then evaluates the
+// residuals reading those inputs from LDS.  In the global-memory layouts
+// many of these loads sit deep in the element code, each waited for on its
+// own; here they cost one round trip per triangle group.  Same arithmetic,
+// same results.
+constexpr int LREC = 57;   // doubles per staged triangle (integers held as exact doubles)
+enum : int { LR_NODE = 0, LR_GRAD = 18, LR_PRE = 24, LR_DJAC = 48, LR_KS = 49, LR_SRC = 50, LR_MAT = 51,
+             LR_WD = 52, LR_SRCF = 53, LR_NBC = 54 };
+
+struct LdsSrc {
+    const double *r;   // the triangle's staged record (LDS)
+    __device__ __forceinline__ double x(int i) const { return r[6 * i]; }
+    __device__ __forceinline__ double y(int i) const { return r[6 * i + 1]; }
+    __device__ __forceinline__ double z(int i) const { return r[6 * i + 2]; }
+    __device__ __forceinline__ double head(int i) const { return r[6 * i + 3]; }
+    __device__ __forceinline__ double u(int i) const { return r[6 * i + 4]; }
+    __device__ __forceinline__ double v(int i) const { return r[6 * i + 5]; }
+    __device__ __forceinline__ int node_bc(int i) const { return (int)r[LR_NBC + i]; }
+    __device__ __forceinline__ double grad(int k) const { return r[LR_GRAD + k]; }
+    __device__ __forceinline__ double djac() const { return r[LR_DJAC]; }
+    __device__ __forceinline__ int mat() const { return (int)r[LR_MAT]; }
+    __device__ __forceinline__ int wd_flag() const { return (int)r[LR_WD]; }
+    __device__ __forceinline__ double ks() const { return r[LR_KS]; }
+    __device__ __forceinline__ int src_flag() const { return (int)r[LR_SRCF]; }
+    __device__ __forceinline__ double src() const { return r[LR_SRC]; }
+    __device__ __forceinline__ double pre(int k) const { return r[LR_PRE + k]; }
+};
+
+// The triangle's 57 inputs into rec: lane half 0 takes one field of each
+// pair, half 1 the other (both fields of a step come from one array where
+// possible, so a step is one load instruction for the whole wave).
+__device__ __forceinline__ void stage_triangle(const ElemData &D, int ie, const int *nd, int half, double *rec) {
+    double val[29];
+    int fld[29];
+    const int na = half ? nd[1] : nd[0];
+    // node fields of nodes 0 and 1
+    const double *narr[6] = {D.x, D.y, D.z, D.head, D.u, D.v};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        val[c] = narr[c][na];
+        fld[c] = (half ? 6 : 0) + c;
+    }
+    // node 2: (x, y), (z, h), (u, v)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double *a = half ? narr[2 * c + 1] : narr[2 * c];
+        val[6 + c] = a[nd[2]];
+        fld[6 + c] = 12 + 2 * c + half;
+    }
+    // gradients and old-state terms
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        val[9 + m] = D.grad[6 * (size_t)ie + 2 * m + half];
+        fld[9 + m] = LR_GRAD + 2 * m + half;
+    }
+    const double *pre = D.pre + PRE_STRIDE * (size_t)ie;
+#pragma unroll
+    for (int m = 0; m < 12; m++) {
+        val[12 + m] = pre[2 * m + half];
+        fld[12 + m] = LR_PRE + 2 * m + half;
+    }
+    val[24] = half ? D.ks[ie] : D.djac[ie];
+    fld[24] = half ? LR_KS : LR_DJAC;
+    val[25] = D.src[ie];
+    fld[25] = LR_SRC;   // both halves write the same value
+    val[26] = (double)(half ? D.wd_flag[ie] : D.mat[ie]);
+    fld[26] = half ? LR_WD : LR_MAT;
+    val[27] = (double)(half ? D.node_bc[nd[0]] : D.src_flag[ie]);
+    fld[27] = half ? LR_NBC : LR_SRCF;
+    val[28] = (double)D.node_bc[half ? nd[2] : nd[1]];
+    fld[28] = LR_NBC + 1 + half;
+#pragma unroll
+    for (int k = 0; k < 29; k++) rec[fld[k]] = val[k];
+}
+
+template <bool WET, int BS>
+__global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list, const int *count, int b0, int b1,
+                                                  int b2, const int *blkpos, double *EM, float *EM32) {
+    constexpr int WPB = BS / 64;
+    __shared__ double srec[WPB][32][LREC];
+    // the friction law's log table (4 KB) copied to LDS: its lookup is the
+    // one data-dependent load of the element code
+    __shared__ crlog::tables slog;
+    {
+        const double *src = (const double *)D.logt;
+        double *dst = (double *)&slog;
+        for (int i = threadIdx.x; i < (int)(sizeof(crlog::tables) / sizeof(double)); i += BS) dst[i] = src[i];
+        __syncthreads();
+    }
+    ElemData DL = D;
+    DL.logt = &slog;
+    const int n = *count;
+    const int nunits = 9 * ((n + 31) >> 5);
+    const int lane = threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool minus = lane >= 32;
+    double *rec = &srec[wib][lane & 31][0];
+    const int wstride = gridDim.x * WPB;
+    // software pipeline: the triangle id two units ahead and its node ids one
+    // unit ahead are loaded while the current unit is computed
+    auto tri_of = [&](int w) { const int g = w / 9, l = g * 32 + (lane & 31); return list[l < n ? l : g * 32]; };
+    int w = blockIdx.x * WPB + wib;
+    int ie_a = w < nunits ? tri_of(w) : 0;
+    int ie_b = w + wstride < nunits ? tri_of(w + wstride) : 0;
+    int nd_a[3] = {0, 0, 0};
+    if (w < nunits)
+        for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
+    for (; w < nunits; w += wstride) {
+        const int q = w % 9, grp = w / 9;
+        const int j = q / 3, var = q % 3;
+        const int li = grp * 32 + (lane & 31);
+        const bool act = li < n;
+        const int ie = ie_a;
+        const int nodes[3] = {nd_a[0], nd_a[1], nd_a[2]};
+        ie_a = ie_b;
+        ie_b = w + 2 * wstride < nunits ? tri_of(w + 2 * wstride) : 0;
+        if (w + wstride < nunits)
+            for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
+        // the previous group's reads are done before the records are rewritten
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        stage_triangle(D, ie, nodes, minus ? 1 : 0, rec);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const LdsSrc S{rec};
+        const double val = var == 0 ? S.u(j) : (var == 1 ? S.v(j) : S.head(j));
+        const double eps = num_diff_eps(val), eps2 = 2.0 * eps;
+        const int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
+        d3 r[3];
+        body_resid_src<WET>(DL, S, ie, eps, j, pv, minus ? -1 : +1, false, r, nodes);
+        double col[9];
+        const double inv = 1.0 / eps2;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double mx = __shfl_xor(r[i].x_eq, 32, 64), my = __shfl_xor(r[i].y_eq, 32, 64),
+                   mc = __shfl_xor(r[i].c_eq, 32, 64);
+            col[3 * i + 0] = div_rn(r[i].x_eq - mx, eps2, inv);
+            col[3 * i + 1] = div_rn(r[i].y_eq - my, eps2, inv);
+            col[3 * i + 2] = div_rn(r[i].c_eq - mc, eps2, inv);
+        }
+        if (minus || !act) continue;
+        const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.elem_inst[ie] : nullptr;
+        const int lb[3] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1, lbi ? lbi[2] : b2};
+        jac_column_db<3>(D, nodes, lb, j, var, col);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const size_t o = 9 * (size_t)blkpos[9 * ie + 3 * i + j];
+#pragma unroll
+            for (int rr = 0; rr < 3; rr++) {
+                if (EM32) EM32[o + 3 * rr + var] = (float)col[3 * i + rr];
+                else EM[o + 3 * rr + var] = col[3 * i + rr];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(128) k_jac1d(ElemData D, int B, int b0, int b1, const int *blkpos1, double *EM,
                                                float *EM32) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -445,6 +606,13 @@ static void launch_split(Ctx *c) {
 // are free parameters: defaults below, or what adh_sw2_hip_autotune measured.
 template <bool WET, int BS>
 static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid, int layout) {
+    if (layout == 2) {   // column-uniform waves, inputs staged in LDS
+        const int units = 9 * ((c->E + 31) / 32);
+        hipLaunchKernelGGL((k_jac2d_lds<WET, BS>), dim3(std::min(grid, nblocks_for(units, BS / 64))), dim3(BS), 0,
+                           c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
+                           em32_of(c));
+        return;
+    }
     if (layout == 1) {   // column-uniform waves: 9 waves per 32 triangles
         const int units = 9 * ((c->E + 31) / 32);
         hipLaunchKernelGGL((k_jac2d_cols<WET, BS>), dim3(std::min(grid, nblocks_for(units, BS / 64))), dim3(BS), 0,

@@ -469,7 +469,7 @@ def test_autotuned_launch_shapes_change_nothing(name):
     p = CASES[name]
     tol = 1e-5 if name.startswith("flood") else 5e-8
     runs = []
-    for mode in ("default", "auto", "odd"):
+    for mode in ("default", "auto", "odd", "staged"):
         g = SW2Model(p, default_opts(tol_nonlin=tol, max_nonlin_it=30))
         if mode == "auto":
             chosen = g.autotune(reps=2)
@@ -483,6 +483,10 @@ def test_autotuned_launch_shapes_change_nothing(name):
                 t.grid[k] = (3, 1, 7, 2)[k]
             t.layout[0] = 1 - g.get_tuning().layout[0]   # the other Jacobian layout than the default, both lists
             t.layout[1] = 1 - g.get_tuning().layout[1]
+            g.set_tuning(t)
+        elif mode == "staged":
+            t = g.get_tuning()
+            t.layout[0] = t.layout[1] = 2                # inputs staged in LDS
             g.set_tuning(t)
         r = g.timestep(max_retries=2)
         h, v, *_ = g.get_state()
