@@ -1730,8 +1730,8 @@ extern "C" int adh_sw2_hip_set_tuning(adh_sw2_hip *h, const adh_sw2_tuning *t) {
         if (t->block[k] != 64 && t->block[k] != 128 && t->block[k] != 256)
             return fail(ADH_SW2_EINVAL, "list kernel block size must be 64, 128 or 256");
         if (t->grid[k] < 1 || t->grid[k] > (1 << 20)) return fail(ADH_SW2_EINVAL, "list kernel grid out of range");
-        if (t->layout[k] != 0 && !(k < 2 && (t->layout[k] == 1 || t->layout[k] == 2)))
-            return fail(ADH_SW2_EINVAL, "layout must be 0, or 1 or 2 for the Jacobian lists");
+        if (t->layout[k] < 0 || t->layout[k] > (k < 2 ? 3 : 0))
+            return fail(ADH_SW2_EINVAL, "layout must be 0 to 3 for the Jacobian lists, 0 for the residual lists");
     }
     for (int k = 0; k < 4; k++) {
         h->c.tune.bs[k] = t->block[k];
@@ -1762,7 +1762,7 @@ extern "C" int adh_sw2_hip_autotune(adh_sw2_hip *h, int reps, adh_sw2_tuning *ou
         const int items = (phase < 2 ? 18 : 1) * cnt[phase & 1];
         if (items == 0) continue;
         double tbest = 1e30;
-        for (int layout : {0, 1, 2}) {
+        for (int layout : {0, 1, 2, 3}) {
             if (layout && phase >= 2) continue;
             for (int b : BS) {
                 const int full = nblocks_for(items, b);

@@ -99,9 +99,12 @@ struct Ctx {
     // launch shape of the four list kernels (Jacobian wet/general, residual
     // wet/general): block size and grid-stride grid size
     struct Tuning {
-        int bs[4] = {256, 128, 128, 128};
-        int grid[4] = {4096, 1024, 2048, 512};
-        int layout[4] = {0, 0, 0, 0};   // Jacobian lists: 0 a lane per (triangle, column, sign), 1 column-uniform waves
+        int bs[4] = {128, 256, 128, 128};
+        int grid[4] = {8192, 1024, 2048, 512};
+        // Jacobian lists: 0 a lane per (triangle, column, sign), 1 column-uniform
+        // waves, 2 column-uniform waves reading their triangles' inputs from LDS,
+        // 3 a wave per 32-triangle group doing its nine columns from one staging
+        int layout[4] = {3, 3, 0, 0};
     } tune;
     double *pre;           // [PRE_STRIDE * E] old/older-state element terms
     int pre_valid;         // pre matches the current old/older state

@@ -346,14 +346,61 @@ __device__ __forceinline__ void stage_triangle(const ElemData &D, int ie, const 
     for (int k = 0; k < 29; k++) rec[fld[k]] = val[k];
 }
 
-template <bool WET, int BS>
+// One Jacobian column (node j, variable var) of the staged triangle: the
+// +eps residual on lanes 0-31, the -eps one on lanes 32-63, the difference
+// quotient, the Dirichlet edits, and the stores of the +eps lanes.
+template <bool WET>
+__device__ __forceinline__ void lds_column(const ElemData &DL, const LdsSrc &S, int ie, const int *nodes, int j,
+                                           int var, bool minus, bool act, int b0, int b1, int b2,
+                                           const int *blkpos, double *EM, float *EM32) {
+    const double val = var == 0 ? S.u(j) : (var == 1 ? S.v(j) : S.head(j));
+    const double eps = num_diff_eps(val), eps2 = 2.0 * eps;
+    const int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
+    d3 r[3];
+    body_resid_src<WET>(DL, S, ie, eps, j, pv, minus ? -1 : +1, false, r, nodes);
+    double col[9];
+    const double inv = 1.0 / eps2;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double mx = __shfl_xor(r[i].x_eq, 32, 64), my = __shfl_xor(r[i].y_eq, 32, 64),
+               mc = __shfl_xor(r[i].c_eq, 32, 64);
+        col[3 * i + 0] = div_rn(r[i].x_eq - mx, eps2, inv);
+        col[3 * i + 1] = div_rn(r[i].y_eq - my, eps2, inv);
+        col[3 * i + 2] = div_rn(r[i].c_eq - mc, eps2, inv);
+    }
+    if (minus || !act) return;
+    const int *lbi = DL.loc_bc_inst ? DL.loc_bc_inst + 3 * DL.elem_inst[ie] : nullptr;
+    const int lb[3] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1, lbi ? lbi[2] : b2};
+    jac_column_db<3>(DL, nodes, lb, j, var, col);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const size_t o = 9 * (size_t)blkpos[9 * ie + 3 * i + j];
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++) {
+            if (EM32) EM32[o + 3 * rr + var] = (float)col[3 * i + rr];
+            else EM[o + 3 * rr + var] = col[3 * i + rr];
+        }
+    }
+}
+
+// GROUP = false: wave w takes column w % 9 of group w / 9 (as the
+// column-uniform layout) and stages its group's inputs for that one column.
+// GROUP = true: wave w takes all nine columns of group w, one after the
+// other, from one staging of the group's inputs; the nine column stores of a
+// triangle then come from one wave, so its element blocks are completed in
+// one L2 rather than in partial lines from several XCDs.
+template <bool WET, int BS, bool GROUP>
 __global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list, const int *count, int b0, int b1,
                                                   int b2, const int *blkpos, double *EM, float *EM32) {
     constexpr int WPB = BS / 64;
     __shared__ double srec[WPB][32][LREC];
+    __shared__ crlog::tables slog;
+    const int n = *count;
+    const int ngroups = (n + 31) >> 5;
+    const int nunits = GROUP ? ngroups : 9 * ngroups;
+    if ((int)blockIdx.x * WPB >= nunits) return;   // the whole block has no unit (uniform exit)
     // the friction law's log table (4 KB) copied to LDS: its lookup is the
     // one data-dependent load of the element code
-    __shared__ crlog::tables slog;
     {
         const double *src = (const double *)D.logt;
         double *dst = (double *)&slog;
@@ -362,8 +409,6 @@ __global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list
     }
     ElemData DL = D;
     DL.logt = &slog;
-    const int n = *count;
-    const int nunits = 9 * ((n + 31) >> 5);
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool minus = lane >= 32;
@@ -371,7 +416,10 @@ __global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list
     const int wstride = gridDim.x * WPB;
     // software pipeline: the triangle id two units ahead and its node ids one
     // unit ahead are loaded while the current unit is computed
-    auto tri_of = [&](int w) { const int g = w / 9, l = g * 32 + (lane & 31); return list[l < n ? l : g * 32]; };
+    auto tri_of = [&](int w) {
+        const int g = GROUP ? w : w / 9, l = g * 32 + (lane & 31);
+        return list[l < n ? l : g * 32];
+    };
     int w = blockIdx.x * WPB + wib;
     int ie_a = w < nunits ? tri_of(w) : 0;
     int ie_b = w + wstride < nunits ? tri_of(w + wstride) : 0;
@@ -379,10 +427,9 @@ __global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list
     if (w < nunits)
         for (int k = 0; k < 3; k++) nd_a[k] = D.e2d[3 * ie_a + k];
     for (; w < nunits; w += wstride) {
-        const int q = w % 9, grp = w / 9;
-        const int j = q / 3, var = q % 3;
+        const int grp = GROUP ? w : w / 9;
         const int li = grp * 32 + (lane & 31);
-        const bool act = li < n;
+        const bool act = li < n;   // idle lanes of a last group recompute its first triangle, store nothing
         const int ie = ie_a;
         const int nodes[3] = {nd_a[0], nd_a[1], nd_a[2]};
         ie_a = ie_b;
@@ -396,33 +443,18 @@ __global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const LdsSrc S{rec};
-        const double val = var == 0 ? S.u(j) : (var == 1 ? S.v(j) : S.head(j));
-        const double eps = num_diff_eps(val), eps2 = 2.0 * eps;
-        const int pv = var == 0 ? PERTURB_U : (var == 1 ? PERTURB_V : PERTURB_H);
-        d3 r[3];
-        body_resid_src<WET>(DL, S, ie, eps, j, pv, minus ? -1 : +1, false, r, nodes);
-        double col[9];
-        const double inv = 1.0 / eps2;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            double mx = __shfl_xor(r[i].x_eq, 32, 64), my = __shfl_xor(r[i].y_eq, 32, 64),
-                   mc = __shfl_xor(r[i].c_eq, 32, 64);
-            col[3 * i + 0] = div_rn(r[i].x_eq - mx, eps2, inv);
-            col[3 * i + 1] = div_rn(r[i].y_eq - my, eps2, inv);
-            col[3 * i + 2] = div_rn(r[i].c_eq - mc, eps2, inv);
-        }
-        if (minus || !act) continue;
-        const int *lbi = D.loc_bc_inst ? D.loc_bc_inst + 3 * D.elem_inst[ie] : nullptr;
-        const int lb[3] = {lbi ? lbi[0] : b0, lbi ? lbi[1] : b1, lbi ? lbi[2] : b2};
-        jac_column_db<3>(D, nodes, lb, j, var, col);
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const size_t o = 9 * (size_t)blkpos[9 * ie + 3 * i + j];
-#pragma unroll
-            for (int rr = 0; rr < 3; rr++) {
-                if (EM32) EM32[o + 3 * rr + var] = (float)col[3 * i + rr];
-                else EM[o + 3 * rr + var] = col[3 * i + rr];
+        if constexpr (GROUP) {
+#pragma unroll 1
+            for (int q = 0; q < 9; q++) {
+                // keeps the record's reads inside the column loop (hoisted
+                // out of it they would hold the whole record in registers)
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                lds_column<WET>(DL, S, ie, nodes, q / 3, q % 3, minus, act, b0, b1, b2, blkpos, EM, EM32);
             }
+        } else {
+            const int q = w % 9;
+            lds_column<WET>(DL, S, ie, nodes, q / 3, q % 3, minus, act, b0, b1, b2, blkpos, EM, EM32);
         }
     }
 }
@@ -608,8 +640,15 @@ template <bool WET, int BS>
 static void jac_list_bs(Ctx *c, const ElemData &D, const int *list, const int *count, int grid, int layout) {
     if (layout == 2) {   // column-uniform waves, inputs staged in LDS
         const int units = 9 * ((c->E + 31) / 32);
-        hipLaunchKernelGGL((k_jac2d_lds<WET, BS>), dim3(std::min(grid, nblocks_for(units, BS / 64))), dim3(BS), 0,
-                           c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
+        hipLaunchKernelGGL((k_jac2d_lds<WET, BS, false>), dim3(std::min(grid, nblocks_for(units, BS / 64))), dim3(BS),
+                           0, c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
+                           em32_of(c));
+        return;
+    }
+    if (layout == 3) {   // a wave per triangle group, its nine columns from one staging
+        const int units = (c->E + 31) / 32;
+        hipLaunchKernelGGL((k_jac2d_lds<WET, BS, true>), dim3(std::min(grid, nblocks_for(units, BS / 64))), dim3(BS),
+                           0, c->stream, D, list, count, c->loc_bc[0], c->loc_bc[1], c->loc_bc[2], c->blkpos, c->EM,
                            em32_of(c));
         return;
     }

@@ -86,12 +86,12 @@ def measured_traffic(precision):
     return int(m.group(1)) if m else None
 
 
-def measured_traffic_kernel(name):
-    """HBM bytes (read + write) per launch of a kernel from the committed
+def measured_traffic_kernel(name, summary="traffic_k_bcg_k1_channel1000x500.md"):
+    """HBM bytes (read + write) per launch of a kernel from a committed
     FETCH_SIZE / WRITE_SIZE table (copy-kernel calibrated), or None."""
     import re
     try:
-        txt = open(os.path.join(ROOT, "profiles", "traffic_k_bcg_k1_channel1000x500.md")).read()
+        txt = open(os.path.join(ROOT, "profiles", summary)).read()
     except OSError:
         return None
     m = re.search(r"^\| `[^`]*" + re.escape(name) + r"[^`]*` \| \d+ \| \d+ \| ([0-9.]+) \| ([0-9.]+) \|", txt, re.M)
@@ -116,30 +116,33 @@ def spmv_roofline(g, p, precision):
                 avg_us=round(ms * 1e3, 2), algo_bytes=int(algo))
 
 
+# Jacobian layout -> (summary name, kernel name prefix in the counter tables)
+JAC_KERNELS = {0: ("k_jac2d_list", "k_jac2d_list<true"), 1: ("k_jac2d_cols", "k_jac2d_cols<true"),
+               2: ("k_jac2d_lds_cols", "k_jac2d_lds<true"), 3: ("k_jac2d_lds_group", "k_jac2d_lds<true")}
+
+
 def jacobian_roofline(g, p):
     """FD Jacobian (wet list kernel, the dominant kernel): FP64 flops per
     launch, counted with rocprofv3 FP64 instruction counters on this workload
-    (committed summary), over its live HIP-event launch time, against the FP64
-    vector peak."""
+    for the kernel of the layout in use (committed summary), over its live
+    HIP-event launch time, against the FP64 vector peak.  The column-uniform
+    layouts skip the pressure block for whole waves, so they execute fewer
+    FP64 instructions than the lane-per-column layout: the count is the
+    executed one of the kernel that is timed."""
     import re
-    # timed in the layout the counters were taken on (a lane per (triangle,
-    # column, sign)); the column-uniform layout skips the pressure block for
-    # whole waves and so executes fewer FP64 instructions
-    tuned = g.get_tuning()
-    t = g.get_tuning()
-    t.layout[0] = 0
-    g.set_tuning(t)
+    kernel, kname = JAC_KERNELS[g.get_tuning().layout[0]]
     ms = g.time_kernel(1, reps=5)
-    g.set_tuning(tuned)
-    flops = None
+    flops, traffic = None, None
     if (p.nnodes, p.nelems2d) == (501501, 1000000):
         try:
-            txt = open(os.path.join(ROOT, "profiles", "counters_k_jac2d_list_channel1000x500.md")).read()
+            txt = open(os.path.join(ROOT, "profiles", f"counters_{kernel}_channel1000x500.md")).read()
             m = re.search(r"= \*\*([0-9.]+) GFLOP\*\* per launch", txt)
             flops = float(m.group(1)) * 1e9 if m else None
         except OSError:
             pass
-    out = dict(bound="fp64-vector", kernel="k_jac2d_list", avg_us=round(ms * 1e3, 1), peak=78.6, unit="TFLOP/s")
+        traffic = measured_traffic_kernel(kname, f"traffic_{kernel}_channel1000x500.md")
+    out = dict(bound="fp64-vector", kernel=kernel, avg_us=round(ms * 1e3, 1), peak=78.6, unit="TFLOP/s",
+               traffic=traffic)
     if flops:
         ach = flops / (ms * 1e-3) / 1e12
         out.update(achieved=round(ach, 2), frac=round(ach / 78.6, 4), flops_per_launch=flops)
@@ -379,16 +382,16 @@ def time_control_mode(args, ps, precision, desc, ntri):
     print(json.dumps(out), flush=True)
 
 
-def kernel_mode(p):
+def kernel_mode(p, forms):
     from adhv8_kraken_amd.sw2 import SW2Model, default_opts
     out = {}
-    for prec in (2, 1, 0):
+    for prec in forms:
         g = SW2Model(p, default_opts(precision=prec, max_nonlin_it=2))
         g.shift()
         g.timestep()
         out[f"k1_prec{prec}_ms"] = g.time_kernel(0, reps=20)
         out[f"jac_prec{prec}_ms"] = g.time_kernel(1, reps=2)
-        if prec == 0:
+        if prec == forms[-1]:
             out["copy_ms"] = g.time_kernel(4, reps=5)
             out["copy_bytes"] = 2 * 8 * 81 * p.nelems2d
         g.close()
@@ -417,6 +420,8 @@ def main():
     ap.add_argument("--kernels", action="store_true",
                     help="profiling mode: one time step per form, then the K1 kernel, the FD Jacobian "
                          "and the calibration copy launched back to back (for rocprofv3 counter passes)")
+    ap.add_argument("--kernel-forms", default="2,1,0",
+                    help="--kernels: the precisions to run (0 FP64, 1 FP32-refined, 2 all-FP32)")
     args = ap.parse_args()
     args.steps_set = args.steps is not None
     if args.steps is None:
@@ -434,7 +439,7 @@ def main():
     p = mesh.bumpy_channel(args.ncx, args.ncy, seed=2)
     log(f"  {p.nnodes} nodes, {p.nelems2d} triangles")
     if args.kernels:
-        kernel_mode(p)
+        kernel_mode(p, [int(f) for f in args.kernel_forms.split(",")])
         return
     res = {}
     roof = {}
@@ -472,8 +477,6 @@ def main():
             out["first_step_max_rel_error_vs_fp64_oracle"] = rel_err(r["first"], o_state)
         return out
 
-    # the counter table was taken on the 1000x500-cell channel
-    jroof["traffic"] = measured_traffic_kernel("k_jac2d_list<true") if (p.nnodes, p.nelems2d) == (501501, 1000000) else None
     out = {
         "metric": "SW2 Newton steps/sec @1M tris, FP64 vs FP32-refined, 1e-8 rel residual",
         "value": round(r32["newton_per_s"], 3),

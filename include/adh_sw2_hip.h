@@ -289,7 +289,9 @@ typedef struct {
     int block[4];      /* 0 Jacobian wet, 1 Jacobian general, 2 residual wet, 3 residual general */
     int grid[4];
     int layout[4];     /* Jacobian lists: 0 = a lane per (triangle, column, sign), 1 = waves that
-                          each take one column of 32 triangles (both signs); 0 for the residual lists */
+                          each take one column of 32 triangles (both signs), 2 = as 1 with the
+                          triangles' inputs staged in LDS, 3 = waves that each take all nine
+                          columns of 32 triangles from one LDS staging; 0 for the residual lists */
     double ms[4];
 } adh_sw2_tuning;
 int adh_sw2_hip_get_tuning(adh_sw2_hip *h, adh_sw2_tuning *out);

@@ -469,7 +469,7 @@ def test_autotuned_launch_shapes_change_nothing(name):
     p = CASES[name]
     tol = 1e-5 if name.startswith("flood") else 5e-8
     runs = []
-    for mode in ("default", "auto", "odd", "staged"):
+    for mode in ("default", "auto", "odd", "lanes", "cols", "staged", "group"):
         g = SW2Model(p, default_opts(tol_nonlin=tol, max_nonlin_it=30))
         if mode == "auto":
             chosen = g.autotune(reps=2)
@@ -481,12 +481,15 @@ def test_autotuned_launch_shapes_change_nothing(name):
             for k in range(4):
                 t.block[k] = (64, 256, 64, 256)[k]
                 t.grid[k] = (3, 1, 7, 2)[k]
-            t.layout[0] = 1 - g.get_tuning().layout[0]   # the other Jacobian layout than the default, both lists
-            t.layout[1] = 1 - g.get_tuning().layout[1]
+            t.layout[0] = (g.get_tuning().layout[0] + 1) % 4   # other Jacobian layouts than the default
+            t.layout[1] = (g.get_tuning().layout[1] + 3) % 4
             g.set_tuning(t)
-        elif mode == "staged":
+        elif mode in ("lanes", "cols", "staged", "group"):
+            # every Jacobian layout on both lists: a lane per (triangle, column,
+            # sign); column-uniform waves; column-uniform waves with the inputs in
+            # LDS; a wave per triangle group doing all nine columns from LDS
             t = g.get_tuning()
-            t.layout[0] = t.layout[1] = 2                # inputs staged in LDS
+            t.layout[0] = t.layout[1] = ("lanes", "cols", "staged", "group").index(mode)
             g.set_tuning(t)
         r = g.timestep(max_retries=2)
         h, v, *_ = g.get_state()
