@@ -86,6 +86,7 @@ extern "C" void adh_sw2_hip_default_opts(adh_sw2_opts *o) {
     o->nblock = 1;
     o->graphs = 0;  // measured: no gain over eager launches on MI355X for single models
     o->krylov = 0;  // BiCGSTAB, as the reference
+    o->lu_tiled = 0;
 }
 
 static int check_opts(const adh_sw2_opts *o) {
@@ -100,6 +101,7 @@ static int check_opts(const adh_sw2_opts *o) {
     if (o->preconditioner == 1 && o->precision != 0)
         return fail(ADH_SW2_EUNSUPPORTED, "the profile-LU preconditioner runs in FP64 (precision 0)");
     if (o->preconditioner == 1 && o->nblock < 1) return fail(ADH_SW2_EINVAL, "nblock must be >= 1");
+    if (o->lu_tiled != 0 && o->lu_tiled != 1) return fail(ADH_SW2_EINVAL, "lu_tiled must be 0 or 1");
     return 0;
 }
 
@@ -844,7 +846,8 @@ static int ensure_band(Ctx &c) {
         return 0;
     }
     if (c.opts.preconditioner != 1) return 0;
-    if (c.band_nblock == std::min(c.opts.nblock, c.N)) return 0;
+    if (c.band_nblock == std::min(c.opts.nblock, c.N) && c.band_lu_tiled_opt == c.opts.lu_tiled) return 0;
+    c.band_lu_tiled_opt = c.opts.lu_tiled;
     std::string err = band_structure(&c, c.opts.nblock);
     if (!err.empty()) return fail(ADH_SW2_EUNSUPPORTED, err);
     return 0;
@@ -895,7 +898,7 @@ static void drop_graphs(Ctx &c) {
 // k BiCGSTAB iterations: a captured graph of the 3k kernels and their
 // finishers, replayed (captured on first use for this k)
 static int run_iters(Ctx &c, int k) {
-    if (k > KGRAPH_MAX || !c.opts.graphs) {
+    if (k > KGRAPH_MAX || !c.opts.graphs || c.band_tiled) {   // the tiled LU's sweeps: a launch per tile, eager
         launch_krylov_iters(&c, k);
         HIPCHK(hipGetLastError());
         return 0;
@@ -964,7 +967,17 @@ static int krylov(Ctx &c, int *lin_it, int *verdict, bool zero_guess) {
     int first, more;
     chunk_plan(c, first, more);
     int left = lin_it_cap(c);
-    for (int chunk = 0;; chunk++) {
+    // with the tiled band LU an iteration is two sweeps of a launch per tile:
+    // the flag is read after the prologue and after every iteration instead
+    bool done_early = false;
+    if (c.opts.preconditioner == 1 && c.band_tiled) {
+        first = more = 1;
+        HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        done_early = c.h_ksc->done;
+        if (done_early && zero_guess) launch_krylov_epilogue(&c, true);
+    }
+    for (int chunk = 0; !done_early; chunk++) {
         int want = chunk == 0 ? first : more;
         int k = std::max(1, std::min(want, left));
         int rc = run_iters(c, k);

@@ -116,6 +116,8 @@ __global__ void __launch_bounds__(64) k_band_solve(BandView B, double *x, const 
 static BandView view(Ctx *c) { return BandView{c->blk_n0, c->blk_w, c->blk_off, c->band}; }
 
 void band_free(Ctx *c) {
+    tband_free(c);
+    c->band_tiled = 0;
     for (void *p : {(void *)c->blk_n0, (void *)c->blk_w, (void *)c->blk_off, (void *)c->node_blk, (void *)c->band})
         if (p) (void)hipFree(p);
     c->blk_n0 = nullptr; c->blk_w = nullptr; c->blk_off = nullptr; c->node_blk = nullptr; c->band = nullptr;
@@ -158,10 +160,25 @@ std::string band_structure(Ctx *c, int nblock) {
         max_n = std::max(max_n, nb);
         off[b + 1] = off[b] + (size_t)nb * (2 * w[b] + 1);
     }
-    if (max_n > BAND_MAX_DOF)
-        return "profile-LU blocks of more than " + std::to_string(BAND_MAX_DOF / 3) +
-               " nodes are not supported; raise nblock";
-    if (off[nblock] * sizeof(double) > BAND_MAX_BYTES) return "profile-LU band storage too large; raise nblock";
+    // blocks beyond the one-workgroup band LU (its solve keeps a block's
+    // right-hand side in LDS), or when asked for: the tiled band LU
+    if (max_n > BAND_MAX_DOF || off[nblock] * sizeof(double) > BAND_MAX_BYTES || c->opts.lu_tiled) {
+        if (hipMalloc((void **)&c->node_blk, sizeof(int) * N) != hipSuccess ||
+            hipMemcpy(c->node_blk, node_blk.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess) {
+            band_free(c);
+            return "hipMalloc failed for the profile-LU blocks";
+        }
+        std::string err = tband_structure(c, n0, w);
+        if (!err.empty()) {
+            band_free(c);
+            return err;
+        }
+        c->band_tiled = 1;
+        c->band_nblock = nblock;
+        c->band_nb = nblock;
+        c->band_max_n = max_n;
+        return "";
+    }
     auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
         return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -183,6 +200,10 @@ std::string band_structure(Ctx *c, int nblock) {
 
 // Loads the scaled system into the bands and factors every block.
 void launch_band_setup(Ctx *c) {
+    if (c->band_tiled) {
+        launch_tband_setup(c);
+        return;
+    }
     launch_zero(c, c->band, c->band_size);
     hipLaunchKernelGGL(k_band_load, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->node_blk,
                        view(c), c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col);
@@ -190,6 +211,10 @@ void launch_band_setup(Ctx *c) {
 }
 
 void launch_band_solve(Ctx *c, double *x, const KScal *sc) {
+    if (c->band_tiled) {
+        launch_tband_solve(c, x, sc);
+        return;
+    }
     hipLaunchKernelGGL(k_band_solve, dim3(c->band_nb), dim3(64), sizeof(double) * c->band_max_n, c->stream, view(c), x,
                        sc);
 }

@@ -40,6 +40,19 @@ struct Norms {
     double maxn, l2, incn, pad;
 };
 
+// One node block's band for the tiled band LU (sw2_lu.hip)
+struct TBand {
+    int n0;        // first node
+    int m;         // DOFs
+    int w;         // half bandwidth of the pattern (DOFs)
+    int wp;        // storage half bandwidth: w + 64
+    int nt;        // 64-DOF tiles (panels)
+    int pad;
+    size_t off;    // first double of the band
+    size_t toff;   // first tile in the per-tile inverse arrays
+    size_t xoff;   // first DOF in the band-ordered vectors
+};
+
 struct Ctx {
     int N, E, B, M, nnz;
     int Np;                // N rounded up to a multiple of 64 (node-slice padding)
@@ -163,6 +176,13 @@ struct Ctx {
     int *blk_n0, *blk_w, *node_blk;
     size_t *blk_off;
     double *band;
+    // the same blocks factored by the tiled band LU (blocks beyond the one-
+    // workgroup band LU's size, or opts.lu_tiled): band_tiled selects it
+    int band_tiled, band_lu_tiled_opt;
+    int tb_nband, tb_maxnt, tb_gt, tb_gf;
+    size_t tb_size;
+    TBand *tb_bands;
+    double *tb_AB, *tb_LI, *tb_UI, *tb_xb, *tb_yb, *tb_panel;
     // block ILU(0) in multicolour order (preconditioner 3), built on demand
     int ilu_ready, ilu_ncolor, ilu_Rp, ilu_f32;
     size_t ilu_nl, ilu_nu;                 // lower / upper slots (64 per slice slot)
@@ -281,6 +301,10 @@ std::string band_structure(Ctx *c, int nblock);
 void band_free(Ctx *c);
 void launch_band_setup(Ctx *c);
 void launch_band_solve(Ctx *c, double *x, const KScal *sc);
+std::string tband_structure(Ctx *c, const std::vector<int> &n0, const std::vector<int> &w);
+void tband_free(Ctx *c);
+void launch_tband_setup(Ctx *c);
+void launch_tband_solve(Ctx *c, double *x, const KScal *sc);
 void launch_probe(const double *x, int n, const crlog::tables *t, double *lg, double *sq, double *dv);
 
 int nblocks_for(int n, int bs);

@@ -1,0 +1,485 @@
+// Tiled band LU: the profile-LU preconditioner (OP BLK) for node blocks of
+// any size, up to the reference's default OP BLK 1 -- one LU of the whole
+// matrix (solv_blk_set.c:5-33, solv_blk_profile.c:51-79,
+// solv_blk_solve.c:5-62).  Nodes keep the reference's order and the LU has
+// no pivoting; a multiplier is its entry over the pivot, or over SMALL when
+// |pivot| <= SMALL (solv_blk_profile.c:66-71; formed here as the entry times
+// the reciprocal, and the updates as fused multiply-adds), and the backward
+// substitution divides by the pivot itself (solv_blk_solve.c:50-52).  Entries outside a
+// row's profile are zero and stay zero (no fill outside the envelope without
+// pivoting), so factoring the band that covers every row's profile gives the
+// skyline factors; only the rounding order of the updates differs.
+//
+// Storage: each band (one node block) in LAPACK band layout, column-major,
+// half bandwidth wp = w + TB (w: the pattern's), entry (i, j) at
+// off + (wp + i - j) + j (2 wp + 1).  The TB extra diagonals hold zeros and
+// let a panel's L block (rows up to TB + w - 1 below its first column) and U
+// block be addressed without bounds tests.  Seen with leading dimension 2 wp
+// the band is a column-major dense matrix, so tiles are strided blocks.
+//
+// Factorisation, per panel of TB columns (one launch each, all bands of a
+// model at once):
+//   k_tb_diag  one wave factors the TB x TB diagonal block in registers
+//              (row t in lane t, pivot rows broadcast with readlane);
+//   k_tb_trsm  one wave per 64-row tile of L (rows below the panel:
+//              L = A U~^-1 by substitution, U~ = U with the guarded pivots)
+//              and per 64-column tile of U (right of the panel: U = L^-1 A);
+//   k_tb_gemm  the trailing w x w window -= L U, 64 x 64 tiles, 4 x 4
+//              register blocks per thread from LDS.
+// Then k_tb_inv forms the inverses of every diagonal block's unit-lower and
+// upper factors (all blocks in parallel), so that each step of the
+// substitutions is matrix-vector work: the forward sweep is one launch per
+// tile (y_k = L_kk^-1 x_k, then the tiles below take -= L_ik y_k, one wave
+// each), the backward sweep likewise upwards.
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "sw2_ctx.h"
+
+namespace sw2 {
+
+constexpr int TB = 64;   // tile and panel width (DOFs)
+
+__device__ __forceinline__ size_t tb_idx(const TBand &B, int i, int j) {
+    return B.off + (size_t)(B.wp + i - j) + (size_t)j * (2 * B.wp + 1);
+}
+__device__ __forceinline__ double &tb_at(double *AB, const TBand &B, int i, int j) { return AB[tb_idx(B, i, j)]; }
+__device__ __forceinline__ double tb_get(const double *AB, const TBand &B, int i, int j) { return AB[tb_idx(B, i, j)]; }
+
+// lane l's value of v, for a lane index known at compile time or uniform
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double guarded(double piv) { return fabs(piv) > SMALL ? piv : SMALL; }
+
+// Scaled matrix (diagonal blocks and SELL off-diagonal blocks) into the bands;
+// the bands are zeroed before.
+__global__ void k_tb_load(int N, const int *node_blk, const TBand *bands, double *AB, const double *diag,
+                          const double *offd, const int *row_ptr, const int *sell_ptr, const int *sell_col) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    const TBand B = bands[node_blk[a]];
+    const int n1 = B.n0 + B.m / 3;
+    const int ia = 3 * (a - B.n0);
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++) tb_at(AB, B, ia + r, ia + cc) = diag[didx9(a, 3 * r + cc)];
+    const size_t q0 = 64 * (size_t)sell_ptr[a >> 6] + (a & 63);
+    const int len = row_ptr[a + 1] - row_ptr[a];
+    for (int s = 0; s < len; s++) {
+        const size_t q = q0 + 64 * (size_t)s;
+        const int cn = sell_col[q];
+        if (cn < B.n0 || cn >= n1) continue;
+        const int jc = 3 * (cn - B.n0);
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) tb_at(AB, B, ia + r, jc + cc) = offd[sell_vidx(q, 3 * r + cc)];
+    }
+}
+
+// Right-looking LU of panel k's diagonal block, one wave per band: lane t
+// holds row t; at pivot p every lane below takes its multiplier (entry times
+// the reciprocal guarded pivot) and updates its row with row p, read from
+// lane p (readlane).
+// It also writes the block for k_tb_trsm: U by rows with the reciprocal
+// guarded pivots, L by columns.
+constexpr int TB_PANEL = 2 * TB * TB + TB;   // doubles of one band's panel copy
+__global__ void __launch_bounds__(64) k_tb_diag(const TBand *bands, double *AB, double *panel, int k) {
+    const TBand B = bands[blockIdx.x];
+    if (k >= B.nt) return;
+    const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
+    double r[TB];
+    // loads at clamped (in-band) positions, then zeroed outside the block: no
+    // load waits on a branch, so all 64 are in flight together
+    const int tc = min(t, kb - 1);
+#pragma unroll
+    for (int j = 0; j < TB; j++) {
+        const double v = tb_get(AB, B, c0 + tc, c0 + min(j, kb - 1));
+        r[j] = (t < kb && j < kb) ? v : 0.0;
+    }
+    double myrden = 0.0;   // lane t: 1 / guarded pivot of row t
+#pragma unroll
+    for (int p = 0; p < TB; p++) {
+        if (p < kb) {   // uniform
+            const double rden = 1.0 / guarded(lane_bcast(r[p], p));
+            if (t == p) myrden = rden;
+            const double l = t > p ? r[p] * rden : 0.0;
+            if (t > p) r[p] = l;
+#pragma unroll
+            for (int q = p + 1; q < TB; q++) r[q] = fma(-l, lane_bcast(r[q], p), r[q]);
+        }
+    }
+    if (t < kb) {
+#pragma unroll
+        for (int j = 0; j < TB; j++)
+            if (j < kb) tb_at(AB, B, c0 + t, c0 + j) = r[j];
+    }
+    double *P = panel + (size_t)TB_PANEL * blockIdx.x;   // [U rows | L columns | 1 / guarded pivot]
+#pragma unroll
+    for (int j = 0; j < TB; j++) {
+        P[TB * t + j] = j >= t ? r[j] : 0.0;              // U, row t
+        P[TB * TB + TB * j + t] = j < t ? r[j] : 0.0;     // L, column j (unit diagonal not stored)
+    }
+    P[2 * TB * TB + t] = t < kb ? myrden : 0.0;
+}
+
+// Panel k's L tiles (blockIdx.x < gl: 64 rows each below the panel) and U
+// tiles (blockIdx.x >= gl: 64 columns each right of it), one wave per tile.
+// Lane t holds a row (L side) or a column (U side) of its tile; the factored
+// diagonal block's entries it meets are the same for every lane and are read
+// from the panel copy through restrict-qualified pointers (scalar loads).
+__global__ void __launch_bounds__(64) k_tb_trsm(const TBand *bands, double *AB, const double *__restrict__ panel,
+                                                int k, int gl) {
+    __shared__ double T[TB][TB + 1];   // U side: the tile, transposed through LDS
+    const TBand B = bands[blockIdx.y];
+    if (k >= B.nt) return;
+    const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
+    const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
+    const bool lside = (int)blockIdx.x < gl;
+    const int g = lside ? blockIdx.x : blockIdx.x - gl;
+    const int base = r0 + TB * g;
+    if (base >= r1) return;
+    const double *__restrict__ U = panel + (size_t)TB_PANEL * blockIdx.y;
+    const double *__restrict__ L = U + TB * TB;
+    const double *__restrict__ R = U + 2 * TB * TB;
+    double r[TB];
+    if (lside) {
+        const int i = base + t;
+        const bool act = i < r1;
+        const int ic = min(i, r1 - 1);
+#pragma unroll
+        for (int j = 0; j < TB; j++) {
+            const double v = tb_get(AB, B, ic, c0 + min(j, kb - 1));   // clamped: in flight together
+            r[j] = (act && j < kb) ? v : 0.0;
+        }
+#pragma unroll
+        for (int p = 0; p < TB; p++) {
+            if (p < kb) {
+                const double l = r[p] * R[p];
+                r[p] = l;
+#pragma unroll
+                for (int q = p + 1; q < TB; q++) r[q] = fma(-l, U[TB * p + q], r[q]);
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < TB; j++)
+                if (j < kb) tb_at(AB, B, i, c0 + j) = r[j];
+        }
+    } else {
+        const int ncol = min(TB, r1 - base), tc = min(t, kb - 1);
+        for (int jj = 0; jj < TB; jj++) {
+            const double v = tb_get(AB, B, c0 + tc, base + min(jj, ncol - 1));
+            T[t][jj] = (t < kb && jj < ncol) ? v : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < TB; p++) r[p] = T[p][t];
+#pragma unroll
+        for (int p = 0; p < TB; p++) {
+            if (p < kb) {
+#pragma unroll
+                for (int q = p + 1; q < TB; q++) r[q] = fma(-L[TB * p + q], r[p], r[q]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < TB; p++) T[p][t] = r[p];
+        __syncthreads();
+        for (int jj = 0; jj < ncol; jj++)
+            if (t < kb) tb_at(AB, B, c0 + t, base + jj) = T[t][jj];
+    }
+}
+
+// Trailing window of panel k: A[r0:r1, r0:r1] -= L[r0:r1, panel] U[panel, r0:r1],
+// one 64 x 64 tile per workgroup (256 threads, a 4 x 4 block each).
+__global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, int k, int gt) {
+    __shared__ double Ls[TB][TB];       // Ls[p][row]
+    __shared__ double Us[TB][TB + 1];   // Us[p][col]
+    const TBand B = bands[blockIdx.y];
+    if (k >= B.nt) return;
+    const int c0 = k * TB, kb = min(TB, B.m - c0);
+    const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
+    const int I = blockIdx.x % gt, J = blockIdx.x / gt;
+    const int i0 = r0 + TB * I, j0 = r0 + TB * J;
+    if (i0 >= r1 || j0 >= r1) return;
+    const int t = threadIdx.x;
+    const int tr = t & 15, tc = t >> 4;   // rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3
+    // this thread's 16 entries of the tile, fetched first (clamped positions,
+    // all in the window) so that their loads overlap the product
+    double cv[4][4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            cv[u][v] = tb_get(AB, B, min(i0 + 4 * tr + u, r1 - 1), min(j0 + 4 * tc + v, r1 - 1));
+    // both tiles read down their columns (consecutive addresses across threads)
+    for (int e = t; e < TB * TB; e += 256) {
+        const int lo = e & (TB - 1), hi = e >> 6;
+        const double lv = tb_get(AB, B, min(i0 + lo, r1 - 1), c0 + min(hi, kb - 1));
+        const double uv = tb_get(AB, B, c0 + min(lo, kb - 1), min(j0 + hi, r1 - 1));
+        Ls[hi][lo] = (hi < kb && i0 + lo < r1) ? lv : 0.0;   // row lo, panel column hi
+        Us[lo][hi] = (lo < kb && j0 + hi < r1) ? uv : 0.0;   // panel row lo, column hi
+    }
+    __syncthreads();
+    double acc[4][4] = {};
+#pragma unroll 8
+    for (int p = 0; p < kb; p++) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { a[u] = Ls[p][4 * tr + u]; b[u] = Us[p][4 * tc + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int j = j0 + 4 * tc + v;
+        if (j >= r1) continue;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + 4 * tr + u;
+            if (i < r1) tb_at(AB, B, i, j) = cv[u][v] - acc[u][v];
+        }
+    }
+}
+
+// Inverses of every diagonal block's factors for the substitutions: LI the
+// unit-lower one, UI the upper one with the pivots themselves; 64 x 64
+// column-major per tile (padding rows and columns of a short last tile are
+// identity).  One wave per tile: lane j forms column j.
+__global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double *AB, double *LI, double *UI) {
+    __shared__ double D[TB][TB + 1];
+    __shared__ double T[TB][TB + 1];
+    const TBand B = bands[blockIdx.y];
+    const int k = blockIdx.x;
+    if (k >= B.nt) return;
+    const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
+    for (int j = 0; j < TB; j++)
+        D[t][j] = (t < kb && j < kb) ? tb_get(AB, B, c0 + t, c0 + j) : (t == j ? 1.0 : 0.0);
+    __syncthreads();
+    double x[TB];
+    // unit lower: x_i = delta_ij - sum_{q<i} L_iq x_q
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        double s = i == t ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < i; q++) s = fma(-D[i][q], x[q], s);
+        x[i] = s;
+    }
+    const size_t o = 4096 * (B.toff + k);
+#pragma unroll
+    for (int i = 0; i < TB; i++) T[i][t] = x[i];
+    __syncthreads();
+    for (int j = 0; j < TB; j++) LI[o + 64 * j + t] = T[t][j];
+    __syncthreads();
+    // upper: x_i = (delta_ij - sum_{q>i} U_iq x_q) / U_ii
+#pragma unroll
+    for (int i = TB - 1; i >= 0; i--) {
+        double s = i == t ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = i + 1; q < TB; q++) s = fma(-D[i][q], x[q], s);
+        x[i] = s / D[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < TB; i++) T[i][t] = x[i];
+    __syncthreads();
+    for (int j = 0; j < TB; j++) UI[o + 64 * j + t] = T[t][j];
+}
+
+// sliced node vector <-> band-ordered vector
+__global__ void k_tb_gather(int N, const int *node_blk, const TBand *bands, const double *x, double *xb) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    const TBand B = bands[node_blk[a]];
+    for (int v = 0; v < 3; v++) xb[B.xoff + 3 * (a - B.n0) + v] = x[vidx3(a, v)];
+}
+__global__ void k_tb_scatter(int N, const int *node_blk, const TBand *bands, const double *xb, double *x,
+                             const KScal *sc) {
+    if (sc && sc->done) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    const TBand B = bands[node_blk[a]];
+    for (int v = 0; v < 3; v++) x[vidx3(a, v)] = xb[B.xoff + 3 * (a - B.n0) + v];
+}
+
+// Sweep kernels: four waves per workgroup split a tile's 64 columns (16
+// each); lane r accumulates row r over its wave's columns (the column value
+// broadcast from lane c) and the four partial sums are added in LDS in wave
+// order.
+__device__ __forceinline__ double tile_mv_part(const double *M, double xr, int wv) {
+    double s = 0.0;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int c = 16 * wv + u;
+        s = fma(M[64 * c + lane], lane_bcast(xr, c), s);
+    }
+    return s;
+}
+__device__ __forceinline__ double sum4(double (*part)[TB], int lane) {
+    return ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+// Forward step k: block 0 stores y_k = LI_k x_k in yb; block g >= 1 takes
+// x[tile k + g] -= L[tile k + g, tile k] y_k.
+__global__ void __launch_bounds__(256) k_tb_fwd(const TBand *bands, const double *AB, const double *LI, double *xb,
+                                                double *yb, int k, const KScal *sc) {
+    __shared__ double part[4][TB];
+    if (sc && sc->done) return;
+    const TBand B = bands[blockIdx.y];
+    if (k >= B.nt) return;
+    const int c0 = k * TB, kb = min(TB, B.m - c0), g = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rmax = min(B.m, c0 + kb + B.w);
+    if (g > 0 && c0 + TB * g >= rmax) return;
+    const double xv = xb[B.xoff + c0 + min(lane, kb - 1)];
+    const double xr = lane < kb ? xv : 0.0;
+    part[wv][lane] = tile_mv_part(LI + 4096 * (B.toff + k), xr, wv);
+    __syncthreads();
+    const double yk = sum4(part, lane);
+    if (g == 0) {
+        if (wv == 0 && lane < kb) yb[B.xoff + c0 + lane] = yk;
+        return;
+    }
+    const int i = c0 + TB * g + lane, ic = min(i, rmax - 1);
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int c = 16 * wv + u;
+        const double yc = lane_bcast(yk, c);
+        if (c < kb) s = fma(tb_get(AB, B, ic, c0 + c), yc, s);
+    }
+    __syncthreads();
+    part[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && i < rmax) xb[B.xoff + i] -= sum4(part, lane);
+}
+
+// Backward step k: block 0 stores z_k = UI_k y_k in xb; block g >= 1 takes
+// y[tile k - g] -= U[tile k - g, tile k] z_k.
+__global__ void __launch_bounds__(256) k_tb_bwd(const TBand *bands, const double *AB, const double *UI, double *xb,
+                                                double *yb, int k, const KScal *sc) {
+    __shared__ double part[4][TB];
+    if (sc && sc->done) return;
+    const TBand B = bands[blockIdx.y];
+    if (k >= B.nt) return;
+    const int c0 = k * TB, kb = min(TB, B.m - c0), g = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rmin = max(0, c0 - B.w);
+    if (g > 0 && c0 - TB * g + TB <= rmin) return;
+    const double yv = yb[B.xoff + c0 + min(lane, kb - 1)];
+    const double yr = lane < kb ? yv : 0.0;
+    part[wv][lane] = tile_mv_part(UI + 4096 * (B.toff + k), yr, wv);
+    __syncthreads();
+    const double zk = sum4(part, lane);
+    if (g == 0) {
+        if (wv == 0 && lane < kb) xb[B.xoff + c0 + lane] = zk;
+        return;
+    }
+    const int i = c0 - TB * g + lane, ic = max(i, rmin);
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int c = 16 * wv + u;
+        const double zc = lane_bcast(zk, c);
+        if (c < kb) s = fma(tb_get(AB, B, ic, c0 + c), zc, s);
+    }
+    __syncthreads();
+    part[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && i >= rmin) yb[B.xoff + i] -= sum4(part, lane);
+}
+
+void tband_free(Ctx *c) {
+    for (void *p : {(void *)c->tb_bands, (void *)c->tb_AB, (void *)c->tb_LI, (void *)c->tb_UI, (void *)c->tb_xb,
+                    (void *)c->tb_yb, (void *)c->tb_panel})
+        if (p) (void)hipFree(p);
+    c->tb_bands = nullptr; c->tb_AB = nullptr; c->tb_LI = nullptr; c->tb_UI = nullptr; c->tb_panel = nullptr;
+    c->tb_xb = nullptr; c->tb_yb = nullptr;
+    c->tb_nband = 0; c->tb_maxnt = 0; c->tb_gt = 0; c->tb_gf = 0; c->tb_size = 0;
+}
+
+// Bands of the node blocks n0[0..nblk] with pattern half bandwidths w[]
+// (DOFs).  Returns an error text or "".
+std::string tband_structure(Ctx *c, const std::vector<int> &n0, const std::vector<int> &w) {
+    tband_free(c);
+    const int nblk = (int)w.size();
+    std::vector<TBand> h(nblk);
+    size_t off = 0, toff = 0, xoff = 0;
+    int maxnt = 0, maxw = 0;
+    for (int b = 0; b < nblk; b++) {
+        TBand &B = h[b];
+        B.n0 = n0[b];
+        B.m = 3 * (n0[b + 1] - n0[b]);
+        B.w = w[b];
+        B.wp = w[b] + TB;
+        B.nt = (B.m + TB - 1) / TB;
+        B.pad = 0;
+        B.off = off;
+        B.toff = toff;
+        B.xoff = xoff;
+        off += (size_t)B.m * (2 * B.wp + 1);
+        toff += B.nt;
+        xoff += B.m;
+        maxnt = std::max(maxnt, B.nt);
+        maxw = std::max(maxw, B.w);
+    }
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    const size_t need = sizeof(double) * (off + 2 * 4096 * toff + 2 * xoff);
+    if (need + ((size_t)2 << 30) > freeb)
+        return "tiled band LU: " + std::to_string(need >> 20) + " MiB of band storage do not fit in the free " +
+               std::to_string(freeb >> 20) + " MiB; raise nblock";
+    if (hipMalloc((void **)&c->tb_bands, sizeof(TBand) * nblk) != hipSuccess ||
+        hipMemcpy(c->tb_bands, h.data(), sizeof(TBand) * nblk, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void **)&c->tb_AB, sizeof(double) * off) != hipSuccess ||
+        hipMalloc((void **)&c->tb_LI, sizeof(double) * 4096 * toff) != hipSuccess ||
+        hipMalloc((void **)&c->tb_UI, sizeof(double) * 4096 * toff) != hipSuccess ||
+        hipMalloc((void **)&c->tb_xb, sizeof(double) * xoff) != hipSuccess ||
+        hipMalloc((void **)&c->tb_yb, sizeof(double) * xoff) != hipSuccess ||
+        hipMalloc((void **)&c->tb_panel, sizeof(double) * TB_PANEL * nblk) != hipSuccess) {
+        tband_free(c);
+        return "hipMalloc failed for the tiled band LU";
+    }
+    c->tb_nband = nblk;
+    c->tb_maxnt = maxnt;
+    c->tb_gt = (maxw + TB - 1) / TB;
+    c->tb_gf = (maxw + 2 * TB - 2) / TB;   // tiles below (above) a panel that its L (U) reaches
+    c->tb_size = off;
+    return "";
+}
+
+void launch_tband_setup(Ctx *c) {
+    launch_zero(c, c->tb_AB, c->tb_size);
+    hipLaunchKernelGGL(k_tb_load, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->node_blk,
+                       c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col);
+    const int gt = std::max(1, c->tb_gt);
+    for (int k = 0; k < c->tb_maxnt; k++) {
+        hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
+                           k);
+        hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
+                           c->tb_panel, k, gt);
+        hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, k,
+                           gt);
+    }
+    hipLaunchKernelGGL(k_tb_inv, dim3(c->tb_maxnt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
+                       c->tb_LI, c->tb_UI);
+}
+
+void launch_tband_solve(Ctx *c, double *x, const KScal *sc) {
+    const dim3 gn(nblocks_for(c->N, 256));
+    hipLaunchKernelGGL(k_tb_gather, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, x, c->tb_xb);
+    const dim3 g(1 + c->tb_gf, c->tb_nband);
+    for (int k = 0; k < c->tb_maxnt; k++)
+        hipLaunchKernelGGL(k_tb_fwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_LI, c->tb_xb, c->tb_yb,
+                           k, sc);
+    for (int k = c->tb_maxnt - 1; k >= 0; k--)
+        hipLaunchKernelGGL(k_tb_bwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_UI, c->tb_xb, c->tb_yb,
+                           k, sc);
+    hipLaunchKernelGGL(k_tb_scatter, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, c->tb_xb, x, sc);
+}
+
+}  // namespace sw2

@@ -49,7 +49,7 @@ class Opts(C.Structure):
     _fields_ = [("tol_nonlin", C.c_double), ("inc_nonlin", C.c_double), ("max_nonlin_it", C.c_int),
                 ("max_lin_it", C.c_int), ("max_cuts", C.c_int), ("lin_rtol", C.c_double), ("lin_atol", C.c_double),
                 ("precision", C.c_int), ("preconditioner", C.c_int), ("check_every", C.c_int),
-                ("nblock", C.c_int), ("graphs", C.c_int), ("krylov", C.c_int)]
+                ("nblock", C.c_int), ("graphs", C.c_int), ("krylov", C.c_int), ("lu_tiled", C.c_int)]
 
 
 class Report(C.Structure):

@@ -107,6 +107,9 @@ typedef struct {
     int krylov;          /* 0 (default): BiCGSTAB (solv_bcgstab.c); 1: restarted GMRES(30), left-preconditioned,
                             stopping on the Arnoldi 2-norm of the preconditioned residual (no counterpart in the
                             reference; single models, precision 0 or 1) */
+    int lu_tiled;        /* preconditioner 1: 0 (default) factors node blocks of up to 2,730 nodes with one
+                            workgroup each and larger ones (up to OP BLK 1, the whole matrix) with the tiled
+                            band LU; 1 uses the tiled band LU for every block */
 } adh_sw2_opts;
 
 typedef struct {

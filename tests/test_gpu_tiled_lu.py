@@ -1,0 +1,118 @@
+"""The tiled band LU (the profile-LU preconditioner for node blocks of any
+size, up to OP BLK 1, the reference's default: one LU of the whole matrix)
+on the device against the oracle's skyline LU (solv_blk_*.c restated).
+
+Both factor the same Jacobi-scaled system in the reference's node order
+without pivoting and with the same SMALL guard, so an OP BLK 1 solve is
+exact on both sides (BiCGSTAB stops after its prologue) and the solutions
+agree to rounding; the Newton sequences then agree in iteration counts and
+to rounding in their histories.
+"""
+import numpy as np
+import pytest
+
+import qa_cases
+from gpu_helpers import pair, say, stepped_problem
+from adhv8_kraken_amd import mesh
+from adhv8_kraken_amd.sw2 import SW2Model, default_opts, YES
+from oracle.oracle import Oracle, opts as oopts, set_log_mode
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"channel-40x20": mesh.channel_config1(), "floodplain-20x10": mesh.floodplain(20, 10)}
+
+
+@pytest.fixture(scope="module")
+def stepped():
+    return {k: stepped_problem(p) for k, p in CASES.items()}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("nblock", [1, 3])
+def test_tiled_lu_linear_solve_against_the_oracle(name, nblock, stepped):
+    """BiCGSTAB with the tiled LU of `nblock` node blocks against the
+    oracle's skyline LU on the same scaled system (the floodplain's systems
+    have near-dry rows, whose tiny pivots take the guard): one block is an
+    exact solve on both sides (solutions within 1e-9); with three blocks
+    both iterate to the solve's 1e-12 tolerance, and on the ill-conditioned
+    floodplain system (60-80 iterations, the counts differing with the
+    preconditioner's rounding) the two solutions agree within 1e-7."""
+    p = CASES[name]
+    say(f"tiled LU linear solve: {name} nblock={nblock}")
+    o, g = pair(p, stepped[name], default_opts(preconditioner=1, nblock=nblock, lu_tiled=1, lin_rtol=1e-12,
+                                               lin_atol=0.0))
+    o.shift(); g.shift(); o.init(); g.init(); o.resid(); g.resid(); o.load(); g.load()
+    oo = oopts(prec=1, nblock=nblock, lin_rtol=1e-12, lin_atol=0.0)
+    o.linear_setup(oo)
+    g.linear_sys_setup()
+    ok_o, x_o, it_o = o.linear_solve(oo)
+    vd, x_g, it_g = g.linear_sys_solve()
+    rel = np.abs(x_g - x_o).max() / np.abs(x_o).max()
+    say(f"  iterations oracle={it_o} gpu={it_g} rel.diff={rel:.3e}")
+    assert vd == YES and ok_o == 1
+    if nblock == 1:
+        assert it_o <= 1 and it_g <= 1
+    assert rel < (1e-7 if (nblock > 1 and name.startswith("flood")) else 1e-9)
+
+
+def test_tiled_lu_equals_the_workgroup_lu():
+    """The same blocks factored by the one-workgroup band LU and by the tiled
+    LU (both exact, in the same order) give the same solution to rounding."""
+    p = CASES["channel-40x20"]
+    sols = []
+    for tiled in (0, 1):
+        g = SW2Model(p, default_opts(preconditioner=1, nblock=2, lu_tiled=tiled, lin_rtol=1e-13, lin_atol=0.0))
+        g.shift(); g.init(); g.resid(); g.load()
+        g.linear_sys_setup()
+        vd, x, it = g.linear_sys_solve()
+        assert vd == YES
+        sols.append((x, it))
+        g.close()
+    rel = np.abs(sols[1][0] - sols[0][0]).max() / np.abs(sols[0][0]).max()
+    say(f"  workgroup LU its={sols[0][1]} tiled its={sols[1][1]} rel.diff={rel:.3e}")
+    assert rel < 1e-11
+
+
+def test_op_blk_1_beyond_the_workgroup_lu_newton():
+    """OP BLK 1 on a mesh whose single block (7,381 nodes, 22,143 DOFs) is
+    beyond the one-workgroup LU, so the tiled LU is chosen by itself: one
+    time step of Newton against the oracle's OP BLK 1 run: same iterations,
+    every linear system solved by the LU alone on both sides (BiCGSTAB stops
+    after its prologue), residual history within 1e-9 of R0 and state within
+    1e-9.  The bound is the LUs' rounding, not a solve tolerance: the two
+    factor in different orders, without pivoting, and the history entries
+    differ by 1.1e-10 of R0 at the third entry (3.6e-7 of the entry itself;
+    1e-13 of R0 at the last)."""
+    set_log_mode(1)
+    p = mesh.bumpy_channel(120, 60, seed=11, dt=5.0)
+    say(f"OP BLK 1 newton: {p.nnodes} nodes")
+    o = Oracle(p)
+    g = SW2Model(p, default_opts(preconditioner=1, nblock=1))
+    o.shift(); g.shift(); o.init(); g.init()
+    r0 = np.abs(o.resid()).max()
+    tol = 1e-8 * r0
+    g.set_opts(default_opts(preconditioner=1, nblock=1, tol_nonlin=tol, max_nonlin_it=30, lin_rtol=1e-12,
+                            lin_atol=0.0))
+    res_o = o.newton(oopts(tol_nonlin=tol, prec=1, nblock=1, max_nonlin_it=30, lin_rtol=1e-12, lin_atol=0.0))
+    res_g = g.newton()
+    say(f"  oracle its={res_o['iterations']} lin={res_o.get('lin_its')} hist={res_o['hist_max']}")
+    say(f"  gpu    its={res_g['iterations']} lin={res_g['lin_its']} hist={res_g['hist_max']}")
+    assert res_g["verdict"] == YES and res_o["ok"] == 1
+    assert res_g["iterations"] == res_o["iterations"]
+    assert res_g["lin_its"] == 0
+    assert np.allclose(res_g["hist_max"], res_o["hist_max"], rtol=0, atol=1e-9 * r0)
+    for a, b in zip(o.get_state(), g.get_state()):
+        assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(a).max())
+    g.close()
+
+
+def test_qa_deck_with_the_tiled_lu():
+    """angle_nb (the reference's OP BLK 1 deck) run through the tiled LU
+    reproduces its error.out.last as the one-workgroup LU does."""
+    from test_qa_golden import CASES as QA
+    ours, ref, _, rtol = QA["angle_nb"]
+    p, ctl = qa_cases.load("angle_nb")
+    h, v, log, err = qa_cases.run_library("angle_nb", p, ctl, preconditioner=1, nblock=ctl["nblock"], lu_tiled=1,
+                                          lin_rtol=ctl.get("lin_rtol", 1e-5), lin_atol=1e-10)
+    say(f"  angle_nb tiled LU: ours {err[ours]} reference {ctl['expected'][ref]}")
+    np.testing.assert_allclose(err[ours], ctl["expected"][ref], rtol=rtol)
