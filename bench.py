@@ -349,6 +349,61 @@ def ensemble_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def direct_mode(args):
+    """Instances of the wet/dry floodplain ensemble one at a time with the
+    reference's default preconditioner, OP BLK 1 (one LU of each instance's
+    whole matrix, the tiled band LU), FP64: a bounded sample (the smallest,
+    median and largest instance unless --instances picks others, --steps time
+    steps each, default 1) of the systems block Jacobi does not solve."""
+    from adhv8_kraken_amd import mesh
+    from adhv8_kraken_amd.sw2 import SW2Model, default_opts, YES
+    sizes = mesh.log_uniform_sizes(64, 50000, 500000, 5)
+    order = np.argsort(sizes)
+    pick = ([int(k) for k in args.instances.split(",")] if args.instances else
+            [int(order[0]), int(order[len(order) // 2]), int(order[-1])])
+    steps = args.steps if args.steps_set else 1
+    rows, tri_newton, seconds, fails = [], 0, 0.0, 0
+    for k in pick:
+        ncx, ncy = mesh._grid_for(sizes[k])
+        p = mesh.floodplain(ncx, ncy, seed=5 * 1000 + k, dt=2.0)
+        g = SW2Model(p, default_opts(preconditioner=1, nblock=1, max_nonlin_it=40))
+        g.shift(); g.init()
+        r0 = float(np.abs(g.resid()).max())
+        g.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+        g.set_opts(default_opts(preconditioner=1, nblock=1, max_nonlin_it=40, tol_nonlin=1e-8 * r0))
+        its = setup_ms = solve_ms = 0
+        t0 = time.time()
+        verdicts = []
+        for _ in range(steps):
+            r = g.timestep(max_retries=2)
+            verdicts.append(int(r["verdict"] == YES))
+            its += r["iterations"]
+            setup_ms += r["ms_setup"]
+            solve_ms += r["ms_solve"]
+        dt = time.time() - t0
+        g.close()
+        fails += verdicts.count(0)
+        tri_newton += p.nelems2d * its
+        seconds += dt
+        rows.append(dict(instance=k, cells=f"{ncx}x{ncy}", triangles=p.nelems2d, newton_its=its,
+                         steps_ok=sum(verdicts), seconds=round(dt, 2),
+                         factor_ms_per_newton=round(setup_ms / max(its, 1), 1),
+                         solve_ms_per_newton=round(solve_ms / max(its, 1), 1)))
+        log(f"  instance {k} ({ncx}x{ncy}): {its} Newton its, {sum(verdicts)}/{steps} steps, {dt:.1f} s")
+    out = {
+        "metric": "SW2 floodplain instances with OP BLK 1 on the device (triangles x Newton iterations per second)",
+        "value": round(tri_newton / seconds), "unit": "triangle-Newton steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "wet/dry floodplain ensemble instances (64 instances, 50k-500k triangles, seed 5), "
+                               "one at a time, OP BLK 1 (tiled band LU of the whole matrix), BiCGSTAB, Newton to "
+                               "1e-8 of the opening residual, dt 2 s cut by 0.25 per failed attempt (2 retries)",
+                   "instances": pick, "time_steps": steps},
+        "failed_steps": fails, "instances": rows,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def time_control_mode(args, ps, precision, desc, ntri):
     """Every instance integrated to t = --time-span with the reference's time
     control, capped at --wall seconds: reports how far the instances got."""
@@ -409,14 +464,20 @@ def main():
     ap.add_argument("--graphs", type=int, default=0, help="1: replay captured HIP graphs of Krylov chunks")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes of the element list kernels (for clean kernel traces)")
-    ap.add_argument("--workload", default="channel", choices=["channel", "channel-ensemble", "floodplain-ensemble"],
-                    help="channel: the 1M-triangle headline; the ensembles print their own JSON line")
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "channel-ensemble", "floodplain-ensemble", "floodplain-direct"],
+                    help="channel: the 1M-triangle headline; the others print their own JSON line")
+    ap.add_argument("--instances", default="",
+                    help="floodplain-direct: comma-separated instance indices (default: smallest, median, largest)")
     ap.add_argument("--compare", type=int, default=0,
                     help="ensembles: also run the first K instances one by one through the single-model path")
     ap.add_argument("--time-span", type=float, default=0.0,
                     help="ensembles: run the reference's time control (retry with dt/4 until a step succeeds, "
                          "regrow dt x2 to the deck's dt) for every instance up to this simulated time")
     ap.add_argument("--wall", type=float, default=300.0, help="wall-clock cap of --time-span runs, seconds")
+    ap.add_argument("--op-blk1-steps", type=int, default=1,
+                    help="time steps of the FP64 OP BLK 1 form (the reference's default preconditioner) run as a "
+                         "bounded sample beside the block-Jacobi forms; 0 skips it")
     ap.add_argument("--kernels", action="store_true",
                     help="profiling mode: one time step per form, then the K1 kernel, the FD Jacobian "
                          "and the calibration copy launched back to back (for rocprofv3 counter passes)")
@@ -431,6 +492,9 @@ def main():
               f"GPU; sharding across GPUs (domain decomposition, or instances for ensembles) is not built",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "floodplain-direct":
+        direct_mode(args)
+        return
     if args.workload != "channel":
         ensemble_mode(args)
         return
@@ -456,6 +520,13 @@ def main():
             roof[prec]["jac_us"] = round(g.time_kernel(1, reps=5) * 1e3, 1)
         if (prec, pc) == (1, 2):
             jroof = jacobian_roofline(g, p)
+        g.close()
+    # the reference's default preconditioner, OP BLK 1 (one LU of the whole
+    # matrix: the tiled band LU), FP64: a bounded sample of the same workload
+    op1 = None
+    if args.op_blk1_steps > 0:
+        log(f"form precision=0 preconditioner=1 nblock=1 (OP BLK 1), {args.op_blk1_steps} time step(s)")
+        g, op1 = run_form(p, 0, args.op_blk1_steps, 0, 1e-8, 0, autotune=False, preconditioner=1)
         g.close()
     cpu, o_state = (None, None) if args.no_cpu else cpu_baseline(p)
     r64 = res[(0, 2)]
@@ -502,6 +573,12 @@ def main():
         "fp32_refined_block_jacobi": form_out(res[(1, 2)], "FP32 matrix and block-Jacobi inverses, FP64 vectors"),
         "fp32_refined_ilu0": form_out(res[(1, 3)], "FP32 matrix and ILU(0) factors, FP64 vectors"),
         "fp32_all": form_out(res[(2, 2)], "FP32 matrix and Krylov vectors; FP64 residual, update"),
+        "fp64_op_blk1": None if op1 is None else {
+            "value": round(op1["newton_per_s"], 4), "newton_its": op1["newton_its"], "lin_its": op1["lin_its"],
+            "time_steps": args.op_blk1_steps, "phases_ms": op1["phases_ms"],
+            "form": "FP64, OP BLK 1: the reference's default preconditioner, one LU of the whole matrix "
+                    "(tiled band LU on the device), a bounded sample of the first time step(s)",
+            "first_step_max_rel_error_vs_fp64_gpu": rel_err(op1["first"], r64["first"])},
         "list_kernel_shapes": r32["tuning"],
         "roofline": jroof,
         "spmv_roofline": {"fp32_matrix_fp64_vectors": roof[1], "fp64": roof[0], "fp32": roof[2]},

@@ -116,3 +116,39 @@ def test_qa_deck_with_the_tiled_lu():
                                           lin_rtol=ctl.get("lin_rtol", 1e-5), lin_atol=1e-10)
     say(f"  angle_nb tiled LU: ours {err[ours]} reference {ctl['expected'][ref]}")
     np.testing.assert_allclose(err[ours], ctl["expected"][ref], rtol=rtol)
+
+
+def test_floodplain_instance_converges_with_op_blk_1_as_the_oracle():
+    """The smallest wet/dry floodplain of the 64-instance ensemble (50,176
+    triangles, 40 % of the nodes dry at the start, the systems that block
+    Jacobi, ILU(0) and GMRES do not solve): its first time step with OP BLK 1
+    on the device against the oracle's OP BLK 1 run (committed fixture, made
+    by tests/golden/make_floodplain_direct.py): the same initial residual bit
+    for bit, the same Newton iteration count, the residual history within
+    1e-6 of R0 and the state at 512 nodes within 1e-6 of its largest value
+    (the nearly singular near-dry rows amplify the two LUs' different
+    rounding; measured agreement is printed)."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "floodplain_direct_step0.npz"))
+    p = mesh.floodplain(int(f["ncx"]), int(f["ncy"]), seed=5 * 1000 + int(f["instance"]), dt=2.0)
+    say(f"floodplain instance {int(f['instance'])}: {p.nnodes} nodes, {p.nelems2d} triangles, OP BLK 1")
+    g = SW2Model(p, default_opts(preconditioner=1, nblock=1, max_nonlin_it=40))
+    g.shift(); g.init()
+    r0 = float(np.abs(g.resid()).max())
+    assert r0 == float(f["r0"])
+    g.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+    g.set_opts(default_opts(preconditioner=1, nblock=1, max_nonlin_it=40, tol_nonlin=1e-8 * r0))
+    r = g.timestep(max_retries=0)
+    h, v, *_ = g.get_state()
+    g.close()
+    v = np.asarray(v).reshape(-1, 2)
+    idx = f["nodes"]
+    dh = np.abs(r["hist_max"] - f["hist_max"]).max() / r0 if len(r["hist_max"]) == len(f["hist_max"]) else np.inf
+    eh = np.abs(h[idx] - f["h"]).max() / np.abs(f["h"]).max()
+    ev = max(np.abs(v[idx, 0] - f["u"]).max(), np.abs(v[idx, 1] - f["v"]).max()) / \
+        max(np.abs(f["u"]).max(), np.abs(f["v"]).max())
+    say(f"  oracle its={int(f['iterations'])} gpu its={r['iterations']} lin={r['lin_its']}; "
+        f"history {dh:.2e} R0, depth {eh:.2e}, velocity {ev:.2e}")
+    assert int(f["ok"]) == 1 and r["verdict"] == YES
+    assert r["iterations"] == int(f["iterations"])
+    assert dh <= 1e-6 and eh <= 1e-6 and ev <= 1e-6
