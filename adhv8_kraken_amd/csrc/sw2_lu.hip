@@ -19,9 +19,9 @@
 //
 // Factorisation, per panel of TB columns (one launch each, all bands of a
 // model at once):
-//   k_tb_diag  one wave factors the TB x TB diagonal block in registers
-//              (row t in lane t, pivot rows broadcast with readlane);
-//   k_tb_trsm  one wave per 64-row tile of L (rows below the panel:
+//   k_tb_diag  four waves factor the TB x TB diagonal block in registers
+//              (row t in lane t, 16 columns per wave, blocked by waves);
+//   k_tb_trsm  a workgroup per 64-row tile of L (rows below the panel:
 //              L = A U~^-1 by substitution, U~ = U with the guarded pivots)
 //              and per 64-column tile of U (right of the panel: U = L^-1 A);
 //   k_tb_gemm  the trailing w x w window -= L U, 64 x 64 tiles, 4 x 4
@@ -79,116 +79,182 @@ __global__ void k_tb_load(int N, const int *node_blk, const TBand *bands, double
     }
 }
 
-// Right-looking LU of panel k's diagonal block, one wave per band: lane t
-// holds row t; at pivot p every lane below takes its multiplier (entry times
-// the reciprocal guarded pivot) and updates its row with row p, read from
-// lane p (readlane).
-// It also writes the block for k_tb_trsm: U by rows with the reciprocal
-// guarded pivots, L by columns.
+// LU of panel k's diagonal block, one workgroup per band, blocked by 16
+// columns: lane t of each of the four waves holds row t, wave w columns
+// 16 w .. 16 w + 15.  Wave P factors its 16 columns (right-looking: the pivot
+// read from lane p with readlane, the multipliers as entry times the
+// reciprocal guarded pivot), publishes them in LDS, and after one barrier the
+// waves to its right update their columns with them (the same right-looking
+// steps, multipliers read from LDS); then wave P + 1 factors its (updated)
+// columns.  Four barriers per block instead of one per pivot.  It also writes
+// the block for k_tb_trsm.
 constexpr int TB_PANEL = 2 * TB * TB + TB;   // doubles of one band's panel copy
-__global__ void __launch_bounds__(64) k_tb_diag(const TBand *bands, double *AB, double *panel, int k) {
+__global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB, double *panel, int k) {
+    __shared__ double pan[2][TB][17];   // a factored 16-column panel (double-buffered by panel parity)
+    __shared__ double rd[TB];           // 1 / guarded pivots
     const TBand B = bands[blockIdx.x];
     if (k >= B.nt) return;
-    const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
-    double r[TB];
-    // loads at clamped (in-band) positions, then zeroed outside the block: no
-    // load waits on a branch, so all 64 are in flight together
-    const int tc = min(t, kb - 1);
+    const int c0 = k * TB, kb = min(TB, B.m - c0);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tc = min(lane, kb - 1);
+    double r[16];
+    // loads at clamped (in-band) positions, then zeroed outside the block: the
+    // rows and columns past a short last block are zero, their pivots guarded
+    // and their multipliers zero
 #pragma unroll
-    for (int j = 0; j < TB; j++) {
-        const double v = tb_get(AB, B, c0 + tc, c0 + min(j, kb - 1));
-        r[j] = (t < kb && j < kb) ? v : 0.0;
+    for (int j = 0; j < 16; j++) {
+        const int col = 16 * wv + j;
+        const double v = tb_get(AB, B, c0 + tc, c0 + min(col, kb - 1));
+        r[j] = (lane < kb && col < kb) ? v : 0.0;
     }
-    double myrden = 0.0;   // lane t: 1 / guarded pivot of row t
 #pragma unroll
-    for (int p = 0; p < TB; p++) {
-        if (p < kb) {   // uniform
-            const double rden = 1.0 / guarded(lane_bcast(r[p], p));
-            if (t == p) myrden = rden;
-            const double l = t > p ? r[p] * rden : 0.0;
-            if (t > p) r[p] = l;
+    for (int P = 0; P < 4; P++) {
+        if (wv == P) {
 #pragma unroll
-            for (int q = p + 1; q < TB; q++) r[q] = fma(-l, lane_bcast(r[q], p), r[q]);
+            for (int pj = 0; pj < 16; pj++) {
+                const int p = 16 * P + pj;
+                const double rden = 1.0 / guarded(lane_bcast(r[pj], p));
+                if (lane == p) rd[p] = rden;
+                const double l = lane > p ? r[pj] * rden : 0.0;
+                if (lane > p) r[pj] = l;
+#pragma unroll
+                for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-l, lane_bcast(r[qj], p), r[qj]);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++) pan[P & 1][lane][j] = r[j];
+        }
+        __syncthreads();
+        if (wv > P) {
+#pragma unroll
+            for (int sj = 0; sj < 16; sj++) {
+                const int sp = 16 * P + sj;
+                const double l = lane > sp ? pan[P & 1][lane][sj] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; j++) r[j] = fma(-l, lane_bcast(r[j], sp), r[j]);
+            }
         }
     }
-    if (t < kb) {
+    __syncthreads();
+    // [strict U by rows | strict L by columns | 1 / guarded pivots]: zeros on and
+    // across the diagonal let k_tb_trsm update every column at every step
+    double *P = panel + (size_t)TB_PANEL * blockIdx.x;
 #pragma unroll
-        for (int j = 0; j < TB; j++)
-            if (j < kb) tb_at(AB, B, c0 + t, c0 + j) = r[j];
+    for (int j = 0; j < 16; j++) {
+        const int col = 16 * wv + j;
+        if (lane < kb && col < kb) tb_at(AB, B, c0 + lane, c0 + col) = r[j];
+        P[TB * lane + col] = col > lane ? r[j] : 0.0;              // U strictly above the diagonal, row `lane`
+        P[TB * TB + TB * col + lane] = col < lane ? r[j] : 0.0;    // L, column `col` (unit diagonal not stored)
     }
-    double *P = panel + (size_t)TB_PANEL * blockIdx.x;   // [U rows | L columns | 1 / guarded pivot]
-#pragma unroll
-    for (int j = 0; j < TB; j++) {
-        P[TB * t + j] = j >= t ? r[j] : 0.0;              // U, row t
-        P[TB * TB + TB * j + t] = j < t ? r[j] : 0.0;     // L, column j (unit diagonal not stored)
-    }
-    P[2 * TB * TB + t] = t < kb ? myrden : 0.0;
+    if (threadIdx.x < TB) P[2 * TB * TB + threadIdx.x] = (int)threadIdx.x < kb ? rd[threadIdx.x] : 0.0;
 }
 
 // Panel k's L tiles (blockIdx.x < gl: 64 rows each below the panel) and U
-// tiles (blockIdx.x >= gl: 64 columns each right of it), one wave per tile.
-// Lane t holds a row (L side) or a column (U side) of its tile; the factored
-// diagonal block's entries it meets are the same for every lane and are read
-// from the panel copy through restrict-qualified pointers (scalar loads).
-__global__ void __launch_bounds__(64) k_tb_trsm(const TBand *bands, double *AB, const double *__restrict__ panel,
-                                                int k, int gl) {
-    __shared__ double T[TB][TB + 1];   // U side: the tile, transposed through LDS
+// tiles (blockIdx.x >= gl: 64 columns each right of it), one workgroup of
+// four waves per tile, blocked by 16 as k_tb_diag.  L side: lane t holds row
+// t of the tile, wave w its columns 16 w .. 16 w + 15; wave P forms the
+// multipliers of its 16 columns, publishes them in LDS, and after one barrier
+// the waves to its right take the rank-16 update.  U side: lane t holds
+// column t, wave w its rows 16 w .. 16 w + 15; wave P finishes its rows and
+// the waves below take their update.  The diagonal block's entries come from
+// an LDS copy of the panel block (broadcast reads; strict triangles, so every
+// update runs over whole 16-column groups).
+__global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB, const double *panel, int k,
+                                                 int gl) {
+    __shared__ double T[TB][TB + 1];    // U side: the tile, transposed through LDS
+    __shared__ double pan[2][TB][17];   // a finished 16-column (L) or 16-row (U) group, by parity
+    __shared__ double Pl[TB_PANEL];     // the panel block: strict U rows | strict L columns | 1 / guarded pivots
     const TBand B = bands[blockIdx.y];
     if (k >= B.nt) return;
-    const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
+    const int c0 = k * TB, kb = min(TB, B.m - c0);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
     const bool lside = (int)blockIdx.x < gl;
     const int g = lside ? blockIdx.x : blockIdx.x - gl;
     const int base = r0 + TB * g;
     if (base >= r1) return;
-    const double *__restrict__ U = panel + (size_t)TB_PANEL * blockIdx.y;
-    const double *__restrict__ L = U + TB * TB;
-    const double *__restrict__ R = U + 2 * TB * TB;
-    double r[TB];
+    const double *Pg = panel + (size_t)TB_PANEL * blockIdx.y;
+    for (int e = threadIdx.x; e < TB_PANEL; e += 256) Pl[e] = Pg[e];
+    const double *U = Pl, *L = Pl + TB * TB, *R = Pl + 2 * TB * TB;
+    double r[16];
     if (lside) {
-        const int i = base + t;
+        const int i = base + lane;
         const bool act = i < r1;
         const int ic = min(i, r1 - 1);
 #pragma unroll
-        for (int j = 0; j < TB; j++) {
-            const double v = tb_get(AB, B, ic, c0 + min(j, kb - 1));   // clamped: in flight together
-            r[j] = (act && j < kb) ? v : 0.0;
+        for (int j = 0; j < 16; j++) {
+            const int col = 16 * wv + j;
+            const double v = tb_get(AB, B, ic, c0 + min(col, kb - 1));   // clamped: in flight together
+            r[j] = (act && col < kb) ? v : 0.0;
         }
+        __syncthreads();
 #pragma unroll
-        for (int p = 0; p < TB; p++) {
-            if (p < kb) {
-                const double l = r[p] * R[p];
-                r[p] = l;
+        for (int P = 0; P < 4; P++) {
+            if (wv == P) {
 #pragma unroll
-                for (int q = p + 1; q < TB; q++) r[q] = fma(-l, U[TB * p + q], r[q]);
+                for (int pj = 0; pj < 16; pj++) {
+                    const int p = 16 * P + pj;
+                    const double l = r[pj] * R[p];
+                    r[pj] = l;
+#pragma unroll
+                    for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-l, U[TB * p + 16 * P + qj], r[qj]);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) pan[P & 1][lane][j] = r[j];
+            }
+            __syncthreads();
+            if (wv > P) {
+#pragma unroll
+                for (int sj = 0; sj < 16; sj++) {
+                    const double l = pan[P & 1][lane][sj];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) r[j] = fma(-l, U[TB * (16 * P + sj) + 16 * wv + j], r[j]);
+                }
             }
         }
-        if (act) {
 #pragma unroll
-            for (int j = 0; j < TB; j++)
-                if (j < kb) tb_at(AB, B, i, c0 + j) = r[j];
+        for (int j = 0; j < 16; j++) {
+            const int col = 16 * wv + j;
+            if (act && col < kb) tb_at(AB, B, i, c0 + col) = r[j];
         }
     } else {
-        const int ncol = min(TB, r1 - base), tc = min(t, kb - 1);
-        for (int jj = 0; jj < TB; jj++) {
-            const double v = tb_get(AB, B, c0 + tc, base + min(jj, ncol - 1));
-            T[t][jj] = (t < kb && jj < ncol) ? v : 0.0;
+        const int ncol = min(TB, r1 - base);
+        for (int e = threadIdx.x; e < TB * TB; e += 256) {
+            const int row = e & (TB - 1), col = e >> 6;   // down a column: consecutive addresses
+            const double v = tb_get(AB, B, c0 + min(row, kb - 1), base + min(col, ncol - 1));
+            T[row][col] = (row < kb && col < ncol) ? v : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < TB; p++) r[p] = T[p][t];
+        for (int j = 0; j < 16; j++) r[j] = T[16 * wv + j][lane];
 #pragma unroll
-        for (int p = 0; p < TB; p++) {
-            if (p < kb) {
+        for (int P = 0; P < 4; P++) {
+            if (wv == P) {
 #pragma unroll
-                for (int q = p + 1; q < TB; q++) r[q] = fma(-L[TB * p + q], r[p], r[q]);
+                for (int pj = 0; pj < 16; pj++) {
+                    const int p = 16 * P + pj;
+#pragma unroll
+                    for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-L[TB * p + 16 * P + qj], r[pj], r[qj]);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) pan[P & 1][lane][j] = r[j];
+            }
+            __syncthreads();
+            if (wv > P) {
+#pragma unroll
+                for (int sj = 0; sj < 16; sj++) {
+                    const double u = pan[P & 1][lane][sj];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) r[j] = fma(-L[TB * (16 * P + sj) + 16 * wv + j], u, r[j]);
+                }
             }
         }
 #pragma unroll
-        for (int p = 0; p < TB; p++) T[p][t] = r[p];
+        for (int j = 0; j < 16; j++) T[16 * wv + j][lane] = r[j];
         __syncthreads();
-        for (int jj = 0; jj < ncol; jj++)
-            if (t < kb) tb_at(AB, B, c0 + t, base + jj) = T[t][jj];
+        for (int e = threadIdx.x; e < TB * TB; e += 256) {
+            const int row = e & (TB - 1), col = e >> 6;
+            if (row < kb && col < ncol) tb_at(AB, B, c0 + row, base + col) = T[row][col];
+        }
     }
 }
 
@@ -458,9 +524,9 @@ void launch_tband_setup(Ctx *c) {
                        c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col);
     const int gt = std::max(1, c->tb_gt);
     for (int k = 0; k < c->tb_maxnt; k++) {
-        hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
-                           k);
-        hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
+        hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
+                           c->tb_panel, k);
+        hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
                            c->tb_panel, k, gt);
         hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, k,
                            gt);

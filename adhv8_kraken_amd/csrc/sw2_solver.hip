@@ -28,6 +28,17 @@
 
 namespace sw2 {
 
+// FP64 matrix and preconditioner entries are read once per product:
+// non-temporal loads keep them from displacing the gathered vector in L2
+// (57 against 63 us for the FP64 K1 at 1M triangles; the FP32 streams are
+// faster with ordinary loads)
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p) {
+    if constexpr (sizeof(T) == 8) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+
 // model tables of an ensemble (null for one model)
 #define BI(c) ((c)->ninst ? (const int *)(c)->blk_inst : (const int *)nullptr)
 #define IB0(c) ((c)->ninst ? (const int *)(c)->inst_blk0 : (const int *)nullptr)
@@ -357,9 +368,9 @@ __device__ __forceinline__ void amult_prec_row(int n, const typename P::M *diag,
 #pragma unroll
         for (int b = 0; b < K; b++) {
             const MT *a = A + 576 * b;
-            p0 += a[0] * g[b][0] + a[64] * g[b][1] + a[128] * g[b][2];
-            p1 += a[192] * g[b][0] + a[256] * g[b][1] + a[320] * g[b][2];
-            p2 += a[384] * g[b][0] + a[448] * g[b][1] + a[512] * g[b][2];
+            p0 += ldnt(a) * g[b][0] + ldnt(a + 64) * g[b][1] + ldnt(a + 128) * g[b][2];
+            p1 += ldnt(a + 192) * g[b][0] + ldnt(a + 256) * g[b][1] + ldnt(a + 320) * g[b][2];
+            p2 += ldnt(a + 384) * g[b][0] + ldnt(a + 448) * g[b][1] + ldnt(a + 512) * g[b][2];
         }
     } else {
         const int k0 = sell_ptr[s], k1 = sell_ptr[s + 1];
@@ -374,18 +385,18 @@ __device__ __forceinline__ void amult_prec_row(int n, const typename P::M *diag,
             p2 += a[384] * y0 + a[448] * y1 + a[512] * y2;
         }
     }
-    T a0 = D[0] * x0 + D[64] * x1 + D[128] * x2;
-    T a1 = D[192] * x0 + D[256] * x1 + D[320] * x2;
-    T a2 = D[384] * x0 + D[448] * x1 + D[512] * x2;
+    T a0 = ldnt(D) * x0 + ldnt(D + 64) * x1 + ldnt(D + 128) * x2;
+    T a1 = ldnt(D + 192) * x0 + ldnt(D + 256) * x1 + ldnt(D + 320) * x2;
+    T a2 = ldnt(D + 384) * x0 + ldnt(D + 448) * x1 + ldnt(D + 512) * x2;
     a0 += p0; a1 += p1; a2 += p2;
     if constexpr (!BJ) {
         y[0] = a0; y[1] = a1; y[2] = a2;
         return;
     }
     const MT *M = dinv + didx9(n, 0);
-    y[0] = M[0] * a0 + M[64] * a1 + M[128] * a2;
-    y[1] = M[192] * a0 + M[256] * a1 + M[320] * a2;
-    y[2] = M[384] * a0 + M[448] * a1 + M[512] * a2;
+    y[0] = ldnt(M) * a0 + ldnt(M + 64) * a1 + ldnt(M + 128) * a2;
+    y[1] = ldnt(M + 192) * a0 + ldnt(M + 256) * a1 + ldnt(M + 320) * a2;
+    y[2] = ldnt(M + 384) * a0 + ldnt(M + 448) * a1 + ldnt(M + 512) * a2;
 }
 
 template <typename T>

@@ -125,9 +125,9 @@ def test_floodplain_instance_converges_with_op_blk_1_as_the_oracle():
     on the device against the oracle's OP BLK 1 run (committed fixture, made
     by tests/golden/make_floodplain_direct.py): the same initial residual bit
     for bit, the same Newton iteration count, the residual history within
-    1e-6 of R0 and the state at 512 nodes within 1e-6 of its largest value
+    1e-7 of R0 and the state at 512 nodes within 1e-7 of its largest value
     (the nearly singular near-dry rows amplify the two LUs' different
-    rounding; measured agreement is printed)."""
+    rounding; measured 3.2e-9 of R0, 2.0e-8 in depth, 1.2e-14 in velocity)."""
     import os
     f = np.load(os.path.join(os.path.dirname(__file__), "golden", "floodplain_direct_step0.npz"))
     p = mesh.floodplain(int(f["ncx"]), int(f["ncy"]), seed=5 * 1000 + int(f["instance"]), dt=2.0)
@@ -151,4 +151,4 @@ def test_floodplain_instance_converges_with_op_blk_1_as_the_oracle():
         f"history {dh:.2e} R0, depth {eh:.2e}, velocity {ev:.2e}")
     assert int(f["ok"]) == 1 and r["verdict"] == YES
     assert r["iterations"] == int(f["iterations"])
-    assert dh <= 1e-6 and eh <= 1e-6 and ev <= 1e-6
+    assert dh <= 1e-7 and eh <= 1e-7 and ev <= 1e-7
