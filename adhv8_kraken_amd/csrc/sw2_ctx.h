@@ -112,8 +112,8 @@ struct Ctx {
     // launch shape of the four list kernels (Jacobian wet/general, residual
     // wet/general): block size and grid-stride grid size
     struct Tuning {
-        int bs[4] = {128, 256, 128, 128};
-        int grid[4] = {8192, 1024, 2048, 512};
+        int bs[4] = {256, 256, 128, 128};
+        int grid[4] = {4096, 1024, 2048, 512};
         // Jacobian lists: 0 a lane per (triangle, column, sign), 1 column-uniform
         // waves, 2 column-uniform waves reading their triangles' inputs from LDS,
         // 3 a wave per 32-triangle group doing its nine columns from one staging

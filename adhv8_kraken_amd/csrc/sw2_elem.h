@@ -564,10 +564,12 @@ HD void elem_old_terms(const ElemData &D, int ie, double *pre) {
 // (GlobalSrc), or a copy a kernel has staged in LDS (sw2_kernels.hip).  Node
 // fields are addressed by the triangle's local node 0..2.
 struct GlobalSrc {
+    static constexpr bool has_rough0 = false;   // see body_resid_src's friction coefficient
     const ElemData &D;
     const int *nd;
     int ie;
     const double *pre_;
+    HDM double rough0() const { return 0.0; }
     HDM double x(int i) const { return D.x[nd[i]]; }
     HDM double y(int i) const { return D.y[nd[i]]; }
     HDM double z(int i) const { return D.z[nd[i]]; }
@@ -596,6 +598,27 @@ HD void body_resid(const ElemData &D, const double *pre, int ie, double pert, in
     const int nd[3] = {ndp ? ndp[0] : D.e2d[3 * ie], ndp ? ndp[1] : D.e2d[3 * ie + 1],
                        ndp ? ndp[2] : D.e2d[3 * ie + 2]};
     body_resid_src<WET>(D, GlobalSrc{D, nd, ie, pre}, ie, pert, pnode, pvar, psign, dry_rule, elem_rhs, nd);
+}
+
+// The drag coefficient of the triangle at its unperturbed state, by the
+// operations body_resid_src performs to reach it (node coordinates and
+// depths, wd_average's depth average, the SMALL floor, drag_coef).
+template <bool WET, typename SRC>
+HD double unperturbed_roughness(const ElemData &D, const SRC &S) {
+    v3 nds[3];
+    double h[3];
+    v2 vel[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        nds[i].x = S.x(i); nds[i].y = S.y(i); nds[i].z = S.z(i);
+        h[i] = S.head(i);
+        vel[i].x = S.u(i); vel[i].y = S.v(i);
+    }
+    double h_wd = int_tri_f(1., 1., h);
+    v2 vwd;
+    wd_average<WET>(nds, h, vel, S.djac(), &h_wd, &vwd);
+    const double ks = S.ks();
+    return ks > SMALL ? drag_coef(dmax(h_wd, SMALL), ks, D.logt) : 0.0;
 }
 
 // The body residual with its inputs read through S (GlobalSrc or a staged
@@ -669,7 +692,17 @@ HD void body_resid_src(const ElemData &D, const SRC &S, int ie, double pert, int
 
     const double h_fric = dmax(h_wd, SMALL);
     const double ks = S.ks();
-    const double roughness = ks > SMALL ? drag_coef(h_fric, ks, D.logt) : 0.0;
+    // h_wd depends on the depths alone (wd_average's f_avg), so under a u or v
+    // perturbation the drag coefficient is the unperturbed triangle's; a source
+    // that carries it (computed once per triangle by unperturbed_roughness, the
+    // same operations on the same inputs) saves its log for those columns
+    double roughness;
+    if constexpr (SRC::has_rough0) {
+        if (pvar == PERTURB_U || pvar == PERTURB_V) roughness = S.rough0();
+        else roughness = ks > SMALL ? drag_coef(h_fric, ks, D.logt) : 0.0;
+    } else {
+        roughness = ks > SMALL ? drag_coef(h_fric, ks, D.logt) : 0.0;
+    }
 
     double ev_st = 0., ev_tr = 0.;
     const int turb = D.mat_turb[imat];

@@ -276,12 +276,17 @@ __global__ void __launch_bounds__(BS) k_jac2d_cols(ElemData D, const int *list, 
 // many of these loads sit deep in the element code, each waited for on its
 // own; here they cost one round trip per triangle group.  Same arithmetic,
 // same results.
-constexpr int LREC = 57;   // doubles per staged triangle (integers held as exact doubles)
+constexpr int LREC = 58;   // doubles per staged triangle (integers held as exact doubles)
 enum : int { LR_NODE = 0, LR_GRAD = 18, LR_PRE = 24, LR_DJAC = 48, LR_KS = 49, LR_SRC = 50, LR_MAT = 51,
-             LR_WD = 52, LR_SRCF = 53, LR_NBC = 54 };
+             LR_WD = 52, LR_SRCF = 53, LR_NBC = 54, LR_ROUGH0 = 57 };
 
-struct LdsSrc {
+// R0: the record also carries the unperturbed drag coefficient (LR_ROUGH0),
+// which the residual then uses for u and v perturbations
+template <bool R0>
+struct LdsSrcT {
+    static constexpr bool has_rough0 = R0;
     const double *r;   // the triangle's staged record (LDS)
+    __device__ __forceinline__ double rough0() const { return r[LR_ROUGH0]; }
     __device__ __forceinline__ double x(int i) const { return r[6 * i]; }
     __device__ __forceinline__ double y(int i) const { return r[6 * i + 1]; }
     __device__ __forceinline__ double z(int i) const { return r[6 * i + 2]; }
@@ -298,6 +303,7 @@ struct LdsSrc {
     __device__ __forceinline__ double src() const { return r[LR_SRC]; }
     __device__ __forceinline__ double pre(int k) const { return r[LR_PRE + k]; }
 };
+using LdsSrc = LdsSrcT<false>;
 
 // The triangle's 57 inputs into rec: lane half 0 takes one field of each
 // pair, half 1 the other (both fields of a step come from one array where
@@ -349,8 +355,8 @@ __device__ __forceinline__ void stage_triangle(const ElemData &D, int ie, const 
 // One Jacobian column (node j, variable var) of the staged triangle: the
 // +eps residual on lanes 0-31, the -eps one on lanes 32-63, the difference
 // quotient, the Dirichlet edits, and the stores of the +eps lanes.
-template <bool WET>
-__device__ __forceinline__ void lds_column(const ElemData &DL, const LdsSrc &S, int ie, const int *nodes, int j,
+template <bool WET, typename SRC>
+__device__ __forceinline__ void lds_column(const ElemData &DL, const SRC &S, int ie, const int *nodes, int j,
                                            int var, bool minus, bool act, int b0, int b1, int b2,
                                            const int *blkpos, double *EM, float *EM32) {
     const double val = var == 0 ? S.u(j) : (var == 1 ? S.v(j) : S.head(j));
@@ -442,7 +448,14 @@ __global__ void __launch_bounds__(BS, 2) k_jac2d_lds(ElemData D, const int *list
         stage_triangle(D, ie, nodes, minus ? 1 : 0, rec);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const LdsSrc S{rec};
+        if constexpr (GROUP) {
+            // the unperturbed drag coefficient, once for the six u and v columns
+            const double rough0 = unperturbed_roughness<WET>(DL, LdsSrc{rec});
+            if (!minus) rec[LR_ROUGH0] = rough0;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        const LdsSrcT<GROUP> S{rec};
         if constexpr (GROUP) {
 #pragma unroll 1
             for (int q = 0; q < 9; q++) {
