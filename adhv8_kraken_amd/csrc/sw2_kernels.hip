@@ -1036,13 +1036,15 @@ void launch_copy(Ctx *c, const double *src, double *dst, size_t n) {
     if (n) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, n, src, dst);
 }
 
+// grid-stride: the tiled LU's bands exceed 2^31 doubles (4.7e9 at 1M triangles)
 __global__ void k_zero(size_t n, double *x) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = 0.0;
 }
 
 void launch_zero(Ctx *c, double *p, size_t n) {
-    hipLaunchKernelGGL(k_zero, dim3(nblocks_for((int)n, 256)), dim3(256), 0, c->stream, n, p);
+    const size_t blocks = std::max<size_t>(1, std::min<size_t>((n + 255) / 256, (size_t)1 << 20));
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, c->stream, n, p);
 }
 
 // rounding probe: device cr_log, sqrt and division of each input

@@ -152,3 +152,34 @@ def test_floodplain_instance_converges_with_op_blk_1_as_the_oracle():
     assert int(f["ok"]) == 1 and r["verdict"] == YES
     assert r["iterations"] == int(f["iterations"])
     assert dh <= 1e-7 and eh <= 1e-7 and ev <= 1e-7
+
+
+def test_op_blk_1_at_1m_triangles_solves_each_newton_system():
+    """OP BLK 1 on the 1000 x 500-cell channel (1.5M unknowns, 37.9 GB of band:
+    more than 2^31 doubles, which the band zeroing between factorisations has
+    to cover): for the first three Newton systems, the LU solution alone
+    (BiCGSTAB stopped after its prologue) has a true relative residual
+    |A x - b|inf / |b|inf below 1e-12 against the unscaled system."""
+    import scipy.sparse as sp
+    p = mesh.bumpy_channel(1000, 500, seed=2)
+    N = p.nnodes
+    g = SW2Model(p, default_opts(preconditioner=1, nblock=1, lin_rtol=1.0, lin_atol=0.0, max_lin_it=1))
+    rp, col = g.pattern()
+    nrow = np.repeat(np.arange(N), np.diff(rp))
+    a = np.tile(np.repeat(np.arange(3), 3), N)
+    b = np.tile(np.arange(3), 3 * N)
+    rows = np.concatenate([np.repeat(3 * np.arange(N), 9) + a,
+                           np.repeat(3 * nrow, 9) + np.tile(np.repeat(np.arange(3), 3), len(col))])
+    cols = np.concatenate([np.repeat(3 * np.arange(N), 9) + b, np.repeat(3 * col, 9) + np.tile(np.arange(3), 3 * len(col))])
+    g.shift(); g.init()
+    for k in range(3):
+        g.resid(); g.load()
+        diag, offd, rhs = g.get_system()
+        A = sp.csr_matrix((np.concatenate([diag, offd]), (rows, cols)), shape=(3 * N, 3 * N))
+        g.linear_sys_setup()
+        vd, x, it = g.linear_sys_solve()
+        rel = np.abs(A @ x - rhs).max() / np.abs(rhs).max()
+        say(f"  Newton system {k}: |Ax-b|/|b| = {rel:.2e}")
+        assert it == 0 and rel < 1e-12
+        g.inc(x)
+    g.close()
