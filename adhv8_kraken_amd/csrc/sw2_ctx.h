@@ -47,7 +47,7 @@ struct TBand {
     int w;         // half bandwidth of the pattern (DOFs)
     int wp;        // storage half bandwidth: w + 64
     int nt;        // 64-DOF tiles (panels)
-    int pad;
+    int pad;       // always 0 (a sweep outside the Krylov loop reads it as its done flag)
     size_t off;    // first double of the band
     size_t toff;   // first tile in the per-tile inverse arrays
     size_t xoff;   // first DOF in the band-ordered vectors

@@ -54,6 +54,16 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
     return __hiloint2double(hi, lo);
 }
 
+// Band record b in SGPRs at once: the empty asm statement uses every field, so
+// the compiler cannot sink the fields' loads past the kernel's first branches
+// (one scalar round trip instead of three before the first vector load);
+// `flag` (a done flag read beside it) is held the same way.
+__device__ __forceinline__ TBand band_rec(const TBand *bands, int b, int flag = 0) {
+    const TBand B = bands[b];
+    asm volatile("" ::"s"(B.m), "s"(B.w), "s"(B.wp), "s"(B.nt), "s"(B.off), "s"(B.toff), "s"(B.xoff), "s"(flag));
+    return B;
+}
+
 __device__ __forceinline__ double guarded(double piv) { return fabs(piv) > SMALL ? piv : SMALL; }
 
 // Scaled matrix (diagonal blocks and SELL off-diagonal blocks) into the bands;
@@ -92,7 +102,7 @@ constexpr int TB_PANEL = 2 * TB * TB + TB;   // doubles of one band's panel copy
 __global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB, double *panel, int k) {
     __shared__ double pan[2][TB][17];   // a factored 16-column panel (double-buffered by panel parity)
     __shared__ double rd[TB];           // 1 / guarded pivots
-    const TBand B = bands[blockIdx.x];
+    const TBand B = band_rec(bands, blockIdx.x);
     if (k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -163,7 +173,7 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
     __shared__ double T[TB][TB + 1];    // U side: the tile, transposed through LDS
     __shared__ double pan[2][TB][17];   // a finished 16-column (L) or 16-row (U) group, by parity
     __shared__ double Pl[TB_PANEL];     // the panel block: strict U rows | strict L columns | 1 / guarded pivots
-    const TBand B = bands[blockIdx.y];
+    const TBand B = band_rec(bands, blockIdx.y);
     if (k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -173,7 +183,22 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
     const int base = r0 + TB * g;
     if (base >= r1) return;
     const double *Pg = panel + (size_t)TB_PANEL * blockIdx.y;
-    for (int e = threadIdx.x; e < TB_PANEL; e += 256) Pl[e] = Pg[e];
+    // all of a thread's loads first, then its LDS stores (a load-store loop
+    // would wait for each load in turn)
+    {
+        constexpr int NP = (TB_PANEL + 255) / 256;
+        double v[NP];
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+            const int e = threadIdx.x + 256 * u;
+            v[u] = Pg[e < TB_PANEL ? e : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+            const int e = threadIdx.x + 256 * u;
+            if (e < TB_PANEL) Pl[e] = v[u];
+        }
+    }
     const double *U = Pl, *L = Pl + TB * TB, *R = Pl + 2 * TB * TB;
     double r[16];
     if (lside) {
@@ -218,10 +243,16 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
         }
     } else {
         const int ncol = min(TB, r1 - base);
-        for (int e = threadIdx.x; e < TB * TB; e += 256) {
-            const int row = e & (TB - 1), col = e >> 6;   // down a column: consecutive addresses
-            const double v = tb_get(AB, B, c0 + min(row, kb - 1), base + min(col, ncol - 1));
-            T[row][col] = (row < kb && col < ncol) ? v : 0.0;
+        double tv[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = threadIdx.x + 256 * u, row = e & (TB - 1), col = e >> 6;   // down a column
+            tv[u] = tb_get(AB, B, c0 + min(row, kb - 1), base + min(col, ncol - 1));
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = threadIdx.x + 256 * u, row = e & (TB - 1), col = e >> 6;
+            T[row][col] = (row < kb && col < ncol) ? tv[u] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -251,8 +282,9 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
 #pragma unroll
         for (int j = 0; j < 16; j++) T[16 * wv + j][lane] = r[j];
         __syncthreads();
-        for (int e = threadIdx.x; e < TB * TB; e += 256) {
-            const int row = e & (TB - 1), col = e >> 6;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = threadIdx.x + 256 * u, row = e & (TB - 1), col = e >> 6;
             if (row < kb && col < ncol) tb_at(AB, B, c0 + row, base + col) = T[row][col];
         }
     }
@@ -263,7 +295,7 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
 __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, int k, int gt) {
     __shared__ double Ls[TB][TB];       // Ls[p][row]
     __shared__ double Us[TB][TB + 1];   // Us[p][col]
-    const TBand B = bands[blockIdx.y];
+    const TBand B = band_rec(bands, blockIdx.y);
     if (k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
@@ -281,12 +313,18 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
         for (int u = 0; u < 4; u++)
             cv[u][v] = tb_get(AB, B, min(i0 + 4 * tr + u, r1 - 1), min(j0 + 4 * tc + v, r1 - 1));
     // both tiles read down their columns (consecutive addresses across threads)
-    for (int e = t; e < TB * TB; e += 256) {
-        const int lo = e & (TB - 1), hi = e >> 6;
-        const double lv = tb_get(AB, B, min(i0 + lo, r1 - 1), c0 + min(hi, kb - 1));
-        const double uv = tb_get(AB, B, c0 + min(lo, kb - 1), min(j0 + hi, r1 - 1));
-        Ls[hi][lo] = (hi < kb && i0 + lo < r1) ? lv : 0.0;   // row lo, panel column hi
-        Us[lo][hi] = (lo < kb && j0 + hi < r1) ? uv : 0.0;   // panel row lo, column hi
+    double lv[16], uv[16];   // all loads in flight before the LDS stores
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = t + 256 * u, lo = e & (TB - 1), hi = e >> 6;
+        lv[u] = tb_get(AB, B, min(i0 + lo, r1 - 1), c0 + min(hi, kb - 1));
+        uv[u] = tb_get(AB, B, c0 + min(lo, kb - 1), min(j0 + hi, r1 - 1));
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = t + 256 * u, lo = e & (TB - 1), hi = e >> 6;
+        Ls[hi][lo] = (hi < kb && i0 + lo < r1) ? lv[u] : 0.0;   // row lo, panel column hi
+        Us[lo][hi] = (lo < kb && j0 + hi < r1) ? uv[u] : 0.0;   // panel row lo, column hi
     }
     __syncthreads();
     double acc[4][4] = {};
@@ -319,7 +357,7 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
 __global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double *AB, double *LI, double *UI) {
     __shared__ double D[TB][TB + 1];
     __shared__ double T[TB][TB + 1];
-    const TBand B = bands[blockIdx.y];
+    const TBand B = band_rec(bands, blockIdx.y);
     const int k = blockIdx.x;
     if (k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
@@ -374,15 +412,14 @@ __global__ void k_tb_scatter(int N, const int *node_blk, const TBand *bands, con
 // Sweep kernels: four waves per workgroup split a tile's 64 columns (16
 // each); lane r accumulates row r over its wave's columns (the column value
 // broadcast from lane c) and the four partial sums are added in LDS in wave
-// order.
-__device__ __forceinline__ double tile_mv_part(const double *M, double xr, int wv) {
+// order.  Every global load of a step (the inverse tile, the vector slice, the
+// off-diagonal tile and the entry it updates) depends only on the band record,
+// so all are issued before the first reduction: one memory round trip per
+// step instead of one per phase.
+__device__ __forceinline__ double tile_mv_part(const double (&m)[16], double xr, int wv) {
     double s = 0.0;
-    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int c = 16 * wv + u;
-        s = fma(M[64 * c + lane], lane_bcast(xr, c), s);
-    }
+    for (int u = 0; u < 16; u++) s = fma(m[u], lane_bcast(xr, 16 * wv + u), s);
     return s;
 }
 __device__ __forceinline__ double sum4(double (*part)[TB], int lane) {
@@ -392,71 +429,91 @@ __device__ __forceinline__ double sum4(double (*part)[TB], int lane) {
 // Forward step k: block 0 stores y_k = LI_k x_k in yb; block g >= 1 takes
 // x[tile k + g] -= L[tile k + g, tile k] y_k.
 __global__ void __launch_bounds__(256) k_tb_fwd(const TBand *bands, const double *AB, const double *LI, double *xb,
-                                                double *yb, int k, const KScal *sc) {
+                                                double *yb, int k, const int *done) {
     __shared__ double part[4][TB];
-    if (sc && sc->done) return;
-    const TBand B = bands[blockIdx.y];
-    if (k >= B.nt) return;
+    const int stop = *done;   // the band record and the flag fetched together
+    const TBand B = band_rec(bands, blockIdx.y, stop);
+    if (stop || k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0), g = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rmax = min(B.m, c0 + kb + B.w);
     if (g > 0 && c0 + TB * g >= rmax) return;
+    const int i = c0 + TB * g + lane, ic = min(i, rmax - 1);
     const double xv = xb[B.xoff + c0 + min(lane, kb - 1)];
+    const double *M = LI + 4096 * (B.toff + k);
+    double m[16], a[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) m[u] = M[64 * (16 * wv + u) + lane];
+    double xi = 0.0;
+    if (g > 0) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = tb_get(AB, B, ic, c0 + min(16 * wv + u, kb - 1));
+        xi = xb[B.xoff + ic];
+    }
     const double xr = lane < kb ? xv : 0.0;
-    part[wv][lane] = tile_mv_part(LI + 4096 * (B.toff + k), xr, wv);
+    part[wv][lane] = tile_mv_part(m, xr, wv);
     __syncthreads();
     const double yk = sum4(part, lane);
     if (g == 0) {
         if (wv == 0 && lane < kb) yb[B.xoff + c0 + lane] = yk;
         return;
     }
-    const int i = c0 + TB * g + lane, ic = min(i, rmax - 1);
     double s = 0.0;
 #pragma unroll
     for (int u = 0; u < 16; u++) {
         const int c = 16 * wv + u;
         const double yc = lane_bcast(yk, c);
-        if (c < kb) s = fma(tb_get(AB, B, ic, c0 + c), yc, s);
+        if (c < kb) s = fma(a[u], yc, s);
     }
     __syncthreads();
     part[wv][lane] = s;
     __syncthreads();
-    if (wv == 0 && i < rmax) xb[B.xoff + i] -= sum4(part, lane);
+    if (wv == 0 && i < rmax) xb[B.xoff + i] = xi - sum4(part, lane);
 }
 
 // Backward step k: block 0 stores z_k = UI_k y_k in xb; block g >= 1 takes
 // y[tile k - g] -= U[tile k - g, tile k] z_k.
 __global__ void __launch_bounds__(256) k_tb_bwd(const TBand *bands, const double *AB, const double *UI, double *xb,
-                                                double *yb, int k, const KScal *sc) {
+                                                double *yb, int k, const int *done) {
     __shared__ double part[4][TB];
-    if (sc && sc->done) return;
-    const TBand B = bands[blockIdx.y];
-    if (k >= B.nt) return;
+    const int stop = *done;   // the band record and the flag fetched together
+    const TBand B = band_rec(bands, blockIdx.y, stop);
+    if (stop || k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0), g = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rmin = max(0, c0 - B.w);
     if (g > 0 && c0 - TB * g + TB <= rmin) return;
+    const int i = c0 - TB * g + lane, ic = max(i, rmin);
     const double yv = yb[B.xoff + c0 + min(lane, kb - 1)];
+    const double *M = UI + 4096 * (B.toff + k);
+    double m[16], a[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) m[u] = M[64 * (16 * wv + u) + lane];
+    double yi = 0.0;
+    if (g > 0) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = tb_get(AB, B, ic, c0 + min(16 * wv + u, kb - 1));
+        yi = yb[B.xoff + ic];
+    }
     const double yr = lane < kb ? yv : 0.0;
-    part[wv][lane] = tile_mv_part(UI + 4096 * (B.toff + k), yr, wv);
+    part[wv][lane] = tile_mv_part(m, yr, wv);
     __syncthreads();
     const double zk = sum4(part, lane);
     if (g == 0) {
         if (wv == 0 && lane < kb) xb[B.xoff + c0 + lane] = zk;
         return;
     }
-    const int i = c0 - TB * g + lane, ic = max(i, rmin);
     double s = 0.0;
 #pragma unroll
     for (int u = 0; u < 16; u++) {
         const int c = 16 * wv + u;
         const double zc = lane_bcast(zk, c);
-        if (c < kb) s = fma(tb_get(AB, B, ic, c0 + c), zc, s);
+        if (c < kb) s = fma(a[u], zc, s);
     }
     __syncthreads();
     part[wv][lane] = s;
     __syncthreads();
-    if (wv == 0 && i >= rmin) yb[B.xoff + i] -= sum4(part, lane);
+    if (wv == 0 && i >= rmin) yb[B.xoff + i] = yi - sum4(part, lane);
 }
 
 void tband_free(Ctx *c) {
@@ -539,12 +596,14 @@ void launch_tband_solve(Ctx *c, double *x, const KScal *sc) {
     const dim3 gn(nblocks_for(c->N, 256));
     hipLaunchKernelGGL(k_tb_gather, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, x, c->tb_xb);
     const dim3 g(1 + c->tb_gf, c->tb_nband);
+    // the Krylov loop's done flag, or the first band record's pad (always 0)
+    const int *done = sc ? &sc->done : &c->tb_bands[0].pad;
     for (int k = 0; k < c->tb_maxnt; k++)
         hipLaunchKernelGGL(k_tb_fwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_LI, c->tb_xb, c->tb_yb,
-                           k, sc);
+                           k, done);
     for (int k = c->tb_maxnt - 1; k >= 0; k--)
         hipLaunchKernelGGL(k_tb_bwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_UI, c->tb_xb, c->tb_yb,
-                           k, sc);
+                           k, done);
     hipLaunchKernelGGL(k_tb_scatter, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, c->tb_xb, x, sc);
 }
 
