@@ -17,15 +17,17 @@
 // block be addressed without bounds tests.  Seen with leading dimension 2 wp
 // the band is a column-major dense matrix, so tiles are strided blocks.
 //
-// Factorisation, per panel of TB columns (one launch each, all bands of a
-// model at once):
-//   k_tb_diag  four waves factor the TB x TB diagonal block in registers
-//              (row t in lane t, 16 columns per wave, blocked by waves);
+// Factorisation, per panel of TB columns (two launches, all bands of a model
+// at once):
 //   k_tb_trsm  a workgroup per 64-row tile of L (rows below the panel:
 //              L = A U~^-1 by substitution, U~ = U with the guarded pivots)
 //              and per 64-column tile of U (right of the panel: U = L^-1 A);
 //   k_tb_gemm  the trailing w x w window -= L U, 64 x 64 tiles, 4 x 4
-//              register blocks per thread from LDS.
+//              register blocks per thread from LDS; the workgroup of the
+//              first tile, the next panel's diagonal block, then factors it
+//              (four waves, row t in lane t, 16 columns per wave, blocked by
+//              waves) while the other tiles are updated.
+// k_tb_diag factors panel 0's diagonal block the same way.
 // Then k_tb_inv forms the inverses of every diagonal block's unit-lower and
 // upper factors (all blocks in parallel), so that each step of the
 // substitutions is matrix-vector work: the forward sweep is one launch per
@@ -89,18 +91,62 @@ __global__ void k_tb_load(int N, const int *node_blk, const TBand *bands, double
     }
 }
 
-// LU of panel k's diagonal block, one workgroup per band, blocked by 16
-// columns: lane t of each of the four waves holds row t, wave w columns
-// 16 w .. 16 w + 15.  Wave P factors its 16 columns (right-looking: the pivot
-// read from lane p with readlane, the multipliers as entry times the
-// reciprocal guarded pivot), publishes them in LDS, and after one barrier the
-// waves to its right update their columns with them (the same right-looking
-// steps, multipliers read from LDS); then wave P + 1 factors its (updated)
-// columns.  Four barriers per block instead of one per pivot.  It also writes
-// the block for k_tb_trsm.
+// LU of a 64 x 64 diagonal block, blocked by 16 columns: lane t of each of
+// the four waves holds row t (r[16]), wave w columns 16 w .. 16 w + 15.  Wave
+// P factors its 16 columns (right-looking: the pivot read from lane p with
+// readlane, the multipliers as entry times the reciprocal guarded pivot),
+// publishes them in LDS, and after one barrier the waves to its right update
+// their columns with them (the same right-looking steps, multipliers read
+// from LDS); then wave P + 1 factors its (updated) columns.  Four barriers per
+// block instead of one per pivot.  Then the factors go back into the band and
+// [strict U by rows | strict L by columns | 1 / guarded pivots] into the
+// band's panel copy P for k_tb_trsm (zeros on and across the diagonal let it
+// update every column at every step).  pan: [2][TB][17] and rd: [TB] in LDS.
 constexpr int TB_PANEL = 2 * TB * TB + TB;   // doubles of one band's panel copy
+__device__ __forceinline__ void diag_lu_store(double (&r)[16], double (*pan)[TB][17], double *rd, double *AB,
+                                              const TBand &B, double *P, int c0, int kb, int lane, int wv) {
+#pragma unroll
+    for (int Q = 0; Q < 4; Q++) {
+        if (wv == Q) {
+#pragma unroll
+            for (int pj = 0; pj < 16; pj++) {
+                const int p = 16 * Q + pj;
+                const double rden = 1.0 / guarded(lane_bcast(r[pj], p));
+                if (lane == p) rd[p] = rden;
+                const double l = lane > p ? r[pj] * rden : 0.0;
+                if (lane > p) r[pj] = l;
+#pragma unroll
+                for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-l, lane_bcast(r[qj], p), r[qj]);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++) pan[Q & 1][lane][j] = r[j];
+        }
+        __syncthreads();
+        if (wv > Q) {
+#pragma unroll
+            for (int sj = 0; sj < 16; sj++) {
+                const int sp = 16 * Q + sj;
+                const double l = lane > sp ? pan[Q & 1][lane][sj] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; j++) r[j] = fma(-l, lane_bcast(r[j], sp), r[j]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int col = 16 * wv + j;
+        if (lane < kb && col < kb) tb_at(AB, B, c0 + lane, c0 + col) = r[j];
+        P[TB * lane + col] = col > lane ? r[j] : 0.0;              // U strictly above the diagonal, row `lane`
+        P[TB * TB + TB * col + lane] = col < lane ? r[j] : 0.0;    // L, column `col` (unit diagonal not stored)
+    }
+    if (wv == 0) P[2 * TB * TB + lane] = lane < kb ? rd[lane] : 0.0;
+}
+
+// Panel 0's diagonal block, one workgroup per band (every later panel's is
+// factored by k_tb_gemm of the panel before, which updates it last).
 __global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB, double *panel, int k) {
-    __shared__ double pan[2][TB][17];   // a factored 16-column panel (double-buffered by panel parity)
+    __shared__ double pan[2][TB][17];   // a factored 16-column group (double-buffered by group parity)
     __shared__ double rd[TB];           // 1 / guarded pivots
     const TBand B = band_rec(bands, blockIdx.x);
     if (k >= B.nt) return;
@@ -117,45 +163,7 @@ __global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB,
         const double v = tb_get(AB, B, c0 + tc, c0 + min(col, kb - 1));
         r[j] = (lane < kb && col < kb) ? v : 0.0;
     }
-#pragma unroll
-    for (int P = 0; P < 4; P++) {
-        if (wv == P) {
-#pragma unroll
-            for (int pj = 0; pj < 16; pj++) {
-                const int p = 16 * P + pj;
-                const double rden = 1.0 / guarded(lane_bcast(r[pj], p));
-                if (lane == p) rd[p] = rden;
-                const double l = lane > p ? r[pj] * rden : 0.0;
-                if (lane > p) r[pj] = l;
-#pragma unroll
-                for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-l, lane_bcast(r[qj], p), r[qj]);
-            }
-#pragma unroll
-            for (int j = 0; j < 16; j++) pan[P & 1][lane][j] = r[j];
-        }
-        __syncthreads();
-        if (wv > P) {
-#pragma unroll
-            for (int sj = 0; sj < 16; sj++) {
-                const int sp = 16 * P + sj;
-                const double l = lane > sp ? pan[P & 1][lane][sj] : 0.0;
-#pragma unroll
-                for (int j = 0; j < 16; j++) r[j] = fma(-l, lane_bcast(r[j], sp), r[j]);
-            }
-        }
-    }
-    __syncthreads();
-    // [strict U by rows | strict L by columns | 1 / guarded pivots]: zeros on and
-    // across the diagonal let k_tb_trsm update every column at every step
-    double *P = panel + (size_t)TB_PANEL * blockIdx.x;
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int col = 16 * wv + j;
-        if (lane < kb && col < kb) tb_at(AB, B, c0 + lane, c0 + col) = r[j];
-        P[TB * lane + col] = col > lane ? r[j] : 0.0;              // U strictly above the diagonal, row `lane`
-        P[TB * TB + TB * col + lane] = col < lane ? r[j] : 0.0;    // L, column `col` (unit diagonal not stored)
-    }
-    if (threadIdx.x < TB) P[2 * TB * TB + threadIdx.x] = (int)threadIdx.x < kb ? rd[threadIdx.x] : 0.0;
+    diag_lu_store(r, pan, rd, AB, B, panel + (size_t)TB_PANEL * blockIdx.x, c0, kb, lane, wv);
 }
 
 // Panel k's L tiles (blockIdx.x < gl: 64 rows each below the panel) and U
@@ -291,27 +299,36 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
 }
 
 // Trailing window of panel k: A[r0:r1, r0:r1] -= L[r0:r1, panel] U[panel, r0:r1],
-// one 64 x 64 tile per workgroup (256 threads, a 4 x 4 block each).
-__global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, int k, int gt) {
-    __shared__ double Ls[TB][TB];       // Ls[p][row]
-    __shared__ double Us[TB][TB + 1];   // Us[p][col]
+// one 64 x 64 tile per workgroup (256 threads, a 4 x 4 block each).  The
+// workgroup of the window's first tile -- panel k + 1's diagonal block, which
+// no later update of panel k touches -- goes on to factor that block
+// (diag_lu_store), while the other tiles are updated: panel k + 1's diagonal
+// LU leaves the critical path of the rank-64 update.
+__global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, double *panel, int k, int gt) {
+    __shared__ double sm[TB * TB + TB * (TB + 1)];
+    double (*Ls)[TB] = reinterpret_cast<double (*)[TB]>(sm);                 // Ls[p][row]
+    double (*Us)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm + TB * TB);   // Us[p][col]
     const TBand B = band_rec(bands, blockIdx.y);
     if (k >= B.nt) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
     const int I = blockIdx.x % gt, J = blockIdx.x / gt;
     const int i0 = r0 + TB * I, j0 = r0 + TB * J;
-    if (i0 >= r1 || j0 >= r1) return;
+    const bool next = I == 0 && J == 0 && k + 1 < B.nt;   // the tile is panel k + 1's diagonal block
+    if (!next && (i0 >= r1 || j0 >= r1)) return;
+    // the next diagonal block is read whole (rows and columns past the window
+    // keep their values: their products are zero), other tiles in the window
+    const int cl = next ? min(B.m, r0 + TB) : r1;
     const int t = threadIdx.x;
     const int tr = t & 15, tc = t >> 4;   // rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3
-    // this thread's 16 entries of the tile, fetched first (clamped positions,
-    // all in the window) so that their loads overlap the product
+    // this thread's 16 entries of the tile, fetched first (clamped positions)
+    // so that their loads overlap the product
     double cv[4][4];
 #pragma unroll
     for (int v = 0; v < 4; v++)
 #pragma unroll
         for (int u = 0; u < 4; u++)
-            cv[u][v] = tb_get(AB, B, min(i0 + 4 * tr + u, r1 - 1), min(j0 + 4 * tc + v, r1 - 1));
+            cv[u][v] = tb_get(AB, B, min(i0 + 4 * tr + u, cl - 1), min(j0 + 4 * tc + v, cl - 1));
     // both tiles read down their columns (consecutive addresses across threads)
     double lv[16], uv[16];   // all loads in flight before the LDS stores
 #pragma unroll
@@ -337,6 +354,28 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
         for (int u = 0; u < 4; u++)
 #pragma unroll
             for (int v = 0; v < 4; v++) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+    }
+    if (next) {
+        // the updated block through LDS into the diagonal LU's layout
+        double (*D)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm);
+        __syncthreads();   // the product's LDS reads are done
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) D[4 * tr + u][4 * tc + v] = cv[u][v] - acc[u][v];
+        __syncthreads();
+        const int kb1 = min(TB, B.m - r0);
+        const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+        double r[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int col = 16 * wv + j;
+            r[j] = (lane < kb1 && col < kb1) ? D[lane][col] : 0.0;
+        }
+        double (*pan)[TB][17] = reinterpret_cast<double (*)[TB][17]>(sm + TB * (TB + 1));
+        double *rd = sm + TB * (TB + 1) + 2 * TB * 17;
+        diag_lu_store(r, pan, rd, AB, B, panel + (size_t)TB_PANEL * blockIdx.y, r0, kb1, lane, wv);
+        return;
     }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -580,13 +619,13 @@ void launch_tband_setup(Ctx *c) {
     hipLaunchKernelGGL(k_tb_load, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->node_blk,
                        c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col);
     const int gt = std::max(1, c->tb_gt);
+    hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
+                       0);
     for (int k = 0; k < c->tb_maxnt; k++) {
-        hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
-                           c->tb_panel, k);
         hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
                            c->tb_panel, k, gt);
-        hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, k,
-                           gt);
+        hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
+                           c->tb_panel, k, gt);
     }
     hipLaunchKernelGGL(k_tb_inv, dim3(c->tb_maxnt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
                        c->tb_LI, c->tb_UI);
