@@ -576,10 +576,20 @@ extern "C" void adh_sw2_hip_destroy(adh_sw2_hip *h) {
     delete h;
 }
 
+// what an ensemble handle runs: BiCGSTAB with 3x3 block Jacobi or the
+// profile LU of each model's node blocks (tiled band LU)
+static int check_batch_opts(const adh_sw2_opts *o) {
+    if (o->preconditioner != 1 && o->preconditioner != 2)
+        return fail(ADH_SW2_EUNSUPPORTED, "ensembles use the 3x3 block-Jacobi preconditioner or the profile LU");
+    if (o->krylov != 0) return fail(ADH_SW2_EUNSUPPORTED, "ensembles use BiCGSTAB");
+    return 0;
+}
+
 extern "C" int adh_sw2_hip_set_opts(adh_sw2_hip *h, const adh_sw2_opts *o) {
     if (!h || !o) return fail(ADH_SW2_EINVAL, "null argument");
     int rc = check_opts(o);
     if (rc) return rc;
+    if (h->c.ninst && (rc = check_batch_opts(o))) return rc;
     h->c.opts = *o;
     if (h->c.opts.check_every < 1) h->c.opts.check_every = 1;
     return 0;
@@ -1306,7 +1316,18 @@ static int krylov_batch(Ctx &c, const std::vector<int> &active, std::vector<int>
     chunk_plan(c, first, more);
     int maxit = 0;
     int left = cap;
-    for (int chunk = 0;; chunk++) {
+    // the tiled band LU: flags read after the prologue and every iteration,
+    // as krylov() does (a direct solve usually ends at the prologue)
+    bool all_done0 = false;
+    if (c.opts.preconditioner == 1 && c.band_tiled) {
+        first = more = 1;
+        HIPCHK(hipMemcpyAsync(c.h_ksc, c.ksc, sizeof(KScal) * n, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        all_done0 = true;
+        for (int i = 0; i < n; i++) all_done0 = all_done0 && c.h_ksc[i].done;
+        if (all_done0) launch_krylov_epilogue(&c, true);
+    }
+    for (int chunk = 0; !all_done0; chunk++) {
         int want = chunk == 0 ? first : more;
         int rc = run_iters(c, std::max(1, std::min(want, left)));
         if (rc) return rc;
@@ -1343,6 +1364,7 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
     std::vector<double> tol(n);
     for (int i = 0; i < n; i++) tol[i] = std::isnan(c.h_inst_tol[i]) ? o.tol_nonlin : c.h_inst_tol[i];
     std::vector<Norms> nm;
+    if ((rc = ensure_band(c))) return rc;   // the profile LU's bands, once per partition
     if ((rc = set_mask(c, run))) return rc;
     launch_init(&c, nullptr);   // masked: only the models of this attempt
     launch_resid(&c);
@@ -1429,10 +1451,8 @@ static int newton_batch(Ctx &c, const std::vector<int> &run, adh_sw2_report *rep
 extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const adh_sw2_opts *opts,
                                         adh_sw2_hip **out) {
     if (!meshes || n < 1 || !out) return fail(ADH_SW2_EINVAL, "null argument or empty ensemble");
-    if (opts && opts->preconditioner != 2)
-        return fail(ADH_SW2_EUNSUPPORTED, "ensembles use the 3x3 block-Jacobi preconditioner");
-    if (opts && opts->krylov != 0) return fail(ADH_SW2_EUNSUPPORTED, "ensembles use BiCGSTAB");
     int rc;
+    if (opts && (rc = check_batch_opts(opts))) return rc;
     for (int i = 0; i < n; i++)
         if ((rc = validate(meshes + i))) return rc;
     const adh_sw2_mesh &m0 = meshes[0];

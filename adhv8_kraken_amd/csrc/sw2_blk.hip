@@ -129,56 +129,78 @@ void band_free(Ctx *c) {
 
 // Builds the block partition and band offsets for `nblock` blocks
 // (node_order.c:64-104 serial partition: N/nblock consecutive nodes per
-// block, the last block taking the rest).  Returns an error text or "".
+// block, the last block taking the rest).  An ensemble partitions each model's
+// node range so, as that model alone would be (the gap nodes between models
+// belong to no block), and always takes the tiled band LU, whose launches
+// cover every model's bands.  Returns an error text or "".
 std::string band_structure(Ctx *c, int nblock) {
     band_free(c);
     const int N = c->N;
     if (nblock < 1) return "nblock must be >= 1";
     nblock = std::min(nblock, N);
-    int inc = N / nblock;
-    if (inc < 1) inc = 1;
-    std::vector<int> n0(nblock + 1), w(nblock), node_blk(N);
-    std::vector<size_t> off(nblock + 1, 0);
-    int last = 0;
-    for (int b = 0; b < nblock; b++) {
-        n0[b] = last;
-        last = (b == nblock - 1) ? N : last + inc;
+    std::vector<int> r0, r1;   // node ranges to partition
+    if (c->ninst) {
+        for (int i = 0; i < c->ninst; i++) {
+            r0.push_back(c->h_inst_node0[i]);
+            r1.push_back(c->h_inst_node0[i] + c->h_inst_N[i]);
+        }
+    } else {
+        r0.push_back(0);
+        r1.push_back(N);
     }
-    n0[nblock] = N;
+    std::vector<int> s0, s1, binst;   // block b: nodes [s0[b], s1[b]) of model binst[b]
+    for (size_t i = 0; i < r0.size(); i++) {
+        const int ni = r1[i] - r0[i], nbi = std::min(nblock, ni);
+        const int inc = std::max(1, ni / nbi);
+        int last = r0[i];
+        for (int b = 0; b < nbi; b++) {
+            s0.push_back(last);
+            last = (b == nbi - 1) ? r1[i] : last + inc;
+            s1.push_back(last);
+            binst.push_back((int)i);
+        }
+    }
+    const int nbk = (int)s0.size();
+    std::vector<int> w(nbk), node_blk(N, -1);
+    std::vector<size_t> off(nbk + 1, 0);
     int max_n = 0;
-    for (int b = 0; b < nblock; b++) {
+    for (int b = 0; b < nbk; b++) {
         int maxd = 0;
-        for (int a = n0[b]; a < n0[b + 1]; a++) {
+        for (int a = s0[b]; a < s1[b]; a++) {
             node_blk[a] = b;
             for (int p = c->h_row_ptr[a]; p < c->h_row_ptr[a + 1]; p++) {
                 const int cn = c->h_col[p];
-                if (cn >= n0[b] && cn < n0[b + 1]) maxd = std::max(maxd, std::abs(cn - a));
+                if (cn >= s0[b] && cn < s1[b]) maxd = std::max(maxd, std::abs(cn - a));
             }
         }
         w[b] = 3 * maxd + 2;
-        const int nb = 3 * (n0[b + 1] - n0[b]);
+        const int nb = 3 * (s1[b] - s0[b]);
         max_n = std::max(max_n, nb);
         off[b + 1] = off[b] + (size_t)nb * (2 * w[b] + 1);
     }
     // blocks beyond the one-workgroup band LU (its solve keeps a block's
-    // right-hand side in LDS), or when asked for: the tiled band LU
-    if (max_n > BAND_MAX_DOF || off[nblock] * sizeof(double) > BAND_MAX_BYTES || c->opts.lu_tiled) {
+    // right-hand side in LDS), ensembles, or when asked for: the tiled band LU
+    if (c->ninst || max_n > BAND_MAX_DOF || off[nbk] * sizeof(double) > BAND_MAX_BYTES || c->opts.lu_tiled) {
         if (hipMalloc((void **)&c->node_blk, sizeof(int) * N) != hipSuccess ||
             hipMemcpy(c->node_blk, node_blk.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess) {
             band_free(c);
             return "hipMalloc failed for the profile-LU blocks";
         }
-        std::string err = tband_structure(c, n0, w);
+        std::string err = tband_structure(c, s0, s1, binst, w);
         if (!err.empty()) {
             band_free(c);
             return err;
         }
         c->band_tiled = 1;
         c->band_nblock = nblock;
-        c->band_nb = nblock;
+        c->band_nb = nbk;
         c->band_max_n = max_n;
         return "";
     }
+    // one model: consecutive blocks, n0[b + 1] = the end of block b
+    std::vector<int> n0(s0);
+    n0.push_back(N);
+    nblock = nbk;
     auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
         if (hipMalloc(dst, bytes ? bytes : 8) != hipSuccess) return false;
         return bytes == 0 || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;

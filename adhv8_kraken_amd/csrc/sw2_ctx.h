@@ -48,6 +48,8 @@ struct TBand {
     int wp;        // storage half bandwidth: w + 64
     int nt;        // 64-DOF tiles (panels)
     int pad;       // always 0 (a sweep outside the Krylov loop reads it as its done flag)
+    int inst;      // model of an ensemble (0 for a single model)
+    int pad2;
     size_t off;    // first double of the band
     size_t toff;   // first tile in the per-tile inverse arrays
     size_t xoff;   // first DOF in the band-ordered vectors
@@ -301,7 +303,9 @@ std::string band_structure(Ctx *c, int nblock);
 void band_free(Ctx *c);
 void launch_band_setup(Ctx *c);
 void launch_band_solve(Ctx *c, double *x, const KScal *sc);
-std::string tband_structure(Ctx *c, const std::vector<int> &n0, const std::vector<int> &w);
+// blocks b: nodes [s0[b], s1[b]) of model inst[b], pattern half bandwidth w[b]
+std::string tband_structure(Ctx *c, const std::vector<int> &s0, const std::vector<int> &s1,
+                            const std::vector<int> &inst, const std::vector<int> &w);
 void tband_free(Ctx *c);
 void launch_tband_setup(Ctx *c);
 void launch_tband_solve(Ctx *c, double *x, const KScal *sc);

@@ -17,8 +17,8 @@
 // block be addressed without bounds tests.  Seen with leading dimension 2 wp
 // the band is a column-major dense matrix, so tiles are strided blocks.
 //
-// Factorisation, per panel of TB columns (two launches, all bands of a model
-// at once):
+// Factorisation, per panel of TB columns (two launches over all bands at once:
+// a model's node blocks, or every block of every model of an ensemble):
 //   k_tb_trsm  a workgroup per 64-row tile of L (rows below the panel:
 //              L = A U~^-1 by substitution, U~ = U with the guarded pivots)
 //              and per 64-column tile of U (right of the panel: U = L^-1 A);
@@ -62,19 +62,25 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
 // `flag` (a done flag read beside it) is held the same way.
 __device__ __forceinline__ TBand band_rec(const TBand *bands, int b, int flag = 0) {
     const TBand B = bands[b];
-    asm volatile("" ::"s"(B.m), "s"(B.w), "s"(B.wp), "s"(B.nt), "s"(B.off), "s"(B.toff), "s"(B.xoff), "s"(flag));
+    asm volatile("" ::"s"(B.m), "s"(B.w), "s"(B.wp), "s"(B.nt), "s"(B.inst), "s"(B.off), "s"(B.toff), "s"(B.xoff),
+                 "s"(flag));
     return B;
 }
 
 __device__ __forceinline__ double guarded(double piv) { return fabs(piv) > SMALL ? piv : SMALL; }
 
 // Scaled matrix (diagonal blocks and SELL off-diagonal blocks) into the bands;
-// the bands are zeroed before.
+// the bands are zeroed before.  Ensembles: nodes of no block (the gaps between
+// models) and models outside the mask are skipped.
 __global__ void k_tb_load(int N, const int *node_blk, const TBand *bands, double *AB, const double *diag,
-                          const double *offd, const int *row_ptr, const int *sell_ptr, const int *sell_col) {
+                          const double *offd, const int *row_ptr, const int *sell_ptr, const int *sell_col,
+                          const int *mask) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= N) return;
-    const TBand B = bands[node_blk[a]];
+    const int bk = node_blk[a];
+    if (bk < 0) return;
+    const TBand B = bands[bk];
+    if (mask && !mask[B.inst]) return;
     const int n1 = B.n0 + B.m / 3;
     const int ia = 3 * (a - B.n0);
     for (int r = 0; r < 3; r++)
@@ -145,11 +151,12 @@ __device__ __forceinline__ void diag_lu_store(double (&r)[16], double (*pan)[TB]
 
 // Panel 0's diagonal block, one workgroup per band (every later panel's is
 // factored by k_tb_gemm of the panel before, which updates it last).
-__global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB, double *panel, int k) {
+__global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB, double *panel, int k,
+                                                 const int *mask) {
     __shared__ double pan[2][TB][17];   // a factored 16-column group (double-buffered by group parity)
     __shared__ double rd[TB];           // 1 / guarded pivots
     const TBand B = band_rec(bands, blockIdx.x);
-    if (k >= B.nt) return;
+    if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tc = min(lane, kb - 1);
@@ -177,12 +184,12 @@ __global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB,
 // an LDS copy of the panel block (broadcast reads; strict triangles, so every
 // update runs over whole 16-column groups).
 __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB, const double *panel, int k,
-                                                 int gl) {
+                                                 int gl, const int *mask) {
     __shared__ double T[TB][TB + 1];    // U side: the tile, transposed through LDS
     __shared__ double pan[2][TB][17];   // a finished 16-column (L) or 16-row (U) group, by parity
     __shared__ double Pl[TB_PANEL];     // the panel block: strict U rows | strict L columns | 1 / guarded pivots
     const TBand B = band_rec(bands, blockIdx.y);
-    if (k >= B.nt) return;
+    if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
@@ -304,12 +311,13 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
 // no later update of panel k touches -- goes on to factor that block
 // (diag_lu_store), while the other tiles are updated: panel k + 1's diagonal
 // LU leaves the critical path of the rank-64 update.
-__global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, double *panel, int k, int gt) {
+__global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, double *panel, int k, int gt,
+                                                 const int *mask) {
     __shared__ double sm[TB * TB + TB * (TB + 1)];
     double (*Ls)[TB] = reinterpret_cast<double (*)[TB]>(sm);                 // Ls[p][row]
     double (*Us)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm + TB * TB);   // Us[p][col]
     const TBand B = band_rec(bands, blockIdx.y);
-    if (k >= B.nt) return;
+    if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int r0 = c0 + kb, r1 = min(B.m, r0 + B.w);
     const int I = blockIdx.x % gt, J = blockIdx.x / gt;
@@ -393,12 +401,13 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
 // unit-lower one, UI the upper one with the pivots themselves; 64 x 64
 // column-major per tile (padding rows and columns of a short last tile are
 // identity).  One wave per tile: lane j forms column j.
-__global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double *AB, double *LI, double *UI) {
+__global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double *AB, double *LI, double *UI,
+                                                const int *mask) {
     __shared__ double D[TB][TB + 1];
     __shared__ double T[TB][TB + 1];
     const TBand B = band_rec(bands, blockIdx.y);
     const int k = blockIdx.x;
-    if (k >= B.nt) return;
+    if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0), t = threadIdx.x;
     for (int j = 0; j < TB; j++)
         D[t][j] = (t < kb && j < kb) ? tb_get(AB, B, c0 + t, c0 + j) : (t == j ? 1.0 : 0.0);
@@ -432,19 +441,24 @@ __global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double 
     for (int j = 0; j < TB; j++) UI[o + 64 * j + t] = T[t][j];
 }
 
-// sliced node vector <-> band-ordered vector
+// sliced node vector <-> band-ordered vector (nodes of no block skipped; the
+// scatter leaves a model that has converged, or is outside the mask, as it is)
 __global__ void k_tb_gather(int N, const int *node_blk, const TBand *bands, const double *x, double *xb) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= N) return;
-    const TBand B = bands[node_blk[a]];
+    const int bk = node_blk[a];
+    if (bk < 0) return;
+    const TBand B = bands[bk];
     for (int v = 0; v < 3; v++) xb[B.xoff + 3 * (a - B.n0) + v] = x[vidx3(a, v)];
 }
 __global__ void k_tb_scatter(int N, const int *node_blk, const TBand *bands, const double *xb, double *x,
-                             const KScal *sc) {
-    if (sc && sc->done) return;
+                             const KScal *sc, const int *mask) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= N) return;
-    const TBand B = bands[node_blk[a]];
+    const int bk = node_blk[a];
+    if (bk < 0) return;
+    const TBand B = bands[bk];
+    if ((sc && sc[B.inst].done) || (mask && !mask[B.inst])) return;
     for (int v = 0; v < 3; v++) x[vidx3(a, v)] = xb[B.xoff + 3 * (a - B.n0) + v];
 }
 
@@ -468,11 +482,13 @@ __device__ __forceinline__ double sum4(double (*part)[TB], int lane) {
 // Forward step k: block 0 stores y_k = LI_k x_k in yb; block g >= 1 takes
 // x[tile k + g] -= L[tile k + g, tile k] y_k.
 __global__ void __launch_bounds__(256) k_tb_fwd(const TBand *bands, const double *AB, const double *LI, double *xb,
-                                                double *yb, int k, const int *done) {
+                                                double *yb, int k, const int *done, const int *mask,
+                                                const KScal *sc) {
     __shared__ double part[4][TB];
     const int stop = *done;   // the band record and the flag fetched together
     const TBand B = band_rec(bands, blockIdx.y, stop);
-    if (stop || k >= B.nt) return;
+    // ensembles: the band's model, converged or outside the mask, is skipped
+    if (stop || k >= B.nt || (mask && (!mask[B.inst] || (sc && sc[B.inst].done)))) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0), g = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rmax = min(B.m, c0 + kb + B.w);
@@ -513,11 +529,13 @@ __global__ void __launch_bounds__(256) k_tb_fwd(const TBand *bands, const double
 // Backward step k: block 0 stores z_k = UI_k y_k in xb; block g >= 1 takes
 // y[tile k - g] -= U[tile k - g, tile k] z_k.
 __global__ void __launch_bounds__(256) k_tb_bwd(const TBand *bands, const double *AB, const double *UI, double *xb,
-                                                double *yb, int k, const int *done) {
+                                                double *yb, int k, const int *done, const int *mask,
+                                                const KScal *sc) {
     __shared__ double part[4][TB];
     const int stop = *done;   // the band record and the flag fetched together
     const TBand B = band_rec(bands, blockIdx.y, stop);
-    if (stop || k >= B.nt) return;
+    // ensembles: the band's model, converged or outside the mask, is skipped
+    if (stop || k >= B.nt || (mask && (!mask[B.inst] || (sc && sc[B.inst].done)))) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0), g = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rmin = max(0, c0 - B.w);
@@ -564,9 +582,10 @@ void tband_free(Ctx *c) {
     c->tb_nband = 0; c->tb_maxnt = 0; c->tb_gt = 0; c->tb_gf = 0; c->tb_size = 0;
 }
 
-// Bands of the node blocks n0[0..nblk] with pattern half bandwidths w[]
-// (DOFs).  Returns an error text or "".
-std::string tband_structure(Ctx *c, const std::vector<int> &n0, const std::vector<int> &w) {
+// Bands of the node blocks (nodes [s0[b], s1[b]) of model inst[b]) with
+// pattern half bandwidths w[] (DOFs).  Returns an error text or "".
+std::string tband_structure(Ctx *c, const std::vector<int> &s0, const std::vector<int> &s1,
+                            const std::vector<int> &inst, const std::vector<int> &w) {
     tband_free(c);
     const int nblk = (int)w.size();
     std::vector<TBand> h(nblk);
@@ -574,12 +593,14 @@ std::string tband_structure(Ctx *c, const std::vector<int> &n0, const std::vecto
     int maxnt = 0, maxw = 0;
     for (int b = 0; b < nblk; b++) {
         TBand &B = h[b];
-        B.n0 = n0[b];
-        B.m = 3 * (n0[b + 1] - n0[b]);
+        B.n0 = s0[b];
+        B.m = 3 * (s1[b] - s0[b]);
         B.w = w[b];
         B.wp = w[b] + TB;
         B.nt = (B.m + TB - 1) / TB;
         B.pad = 0;
+        B.inst = inst[b];
+        B.pad2 = 0;
         B.off = off;
         B.toff = toff;
         B.xoff = xoff;
@@ -614,36 +635,46 @@ std::string tband_structure(Ctx *c, const std::vector<int> &n0, const std::vecto
     return "";
 }
 
+// the models the masked launches of an ensemble work on (null: every band)
+static const int *tb_mask(const Ctx *c) { return c->ninst ? c->inst_mask : nullptr; }
+
 void launch_tband_setup(Ctx *c) {
+    const int *mask = tb_mask(c);
     launch_zero(c, c->tb_AB, c->tb_size);
     hipLaunchKernelGGL(k_tb_load, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->node_blk,
-                       c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col);
+                       c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col, mask);
     const int gt = std::max(1, c->tb_gt);
     hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
-                       0);
+                       0, mask);
     for (int k = 0; k < c->tb_maxnt; k++) {
         hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
-                           c->tb_panel, k, gt);
+                           c->tb_panel, k, gt, mask);
         hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
-                           c->tb_panel, k, gt);
+                           c->tb_panel, k, gt, mask);
     }
     hipLaunchKernelGGL(k_tb_inv, dim3(c->tb_maxnt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
-                       c->tb_LI, c->tb_UI);
+                       c->tb_LI, c->tb_UI, mask);
 }
 
+// sc: the Krylov scalars (one per model) whose done flags end the sweeps, or
+// null outside the iteration loop
 void launch_tband_solve(Ctx *c, double *x, const KScal *sc) {
+    const int *mask = tb_mask(c);
     const dim3 gn(nblocks_for(c->N, 256));
     hipLaunchKernelGGL(k_tb_gather, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, x, c->tb_xb);
     const dim3 g(1 + c->tb_gf, c->tb_nband);
-    // the Krylov loop's done flag, or the first band record's pad (always 0)
-    const int *done = sc ? &sc->done : &c->tb_bands[0].pad;
+    // one model: its done flag, or the first band record's pad (always 0),
+    // read beside the band record; ensembles: each band's model's flag
+    const int *done = (sc && !mask) ? &sc->done : &c->tb_bands[0].pad;
+    const KScal *msc = mask ? sc : nullptr;
     for (int k = 0; k < c->tb_maxnt; k++)
         hipLaunchKernelGGL(k_tb_fwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_LI, c->tb_xb, c->tb_yb,
-                           k, done);
+                           k, done, mask, msc);
     for (int k = c->tb_maxnt - 1; k >= 0; k--)
         hipLaunchKernelGGL(k_tb_bwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_UI, c->tb_xb, c->tb_yb,
-                           k, done);
-    hipLaunchKernelGGL(k_tb_scatter, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, c->tb_xb, x, sc);
+                           k, done, mask, msc);
+    hipLaunchKernelGGL(k_tb_scatter, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, c->tb_xb, x, sc,
+                       mask);
 }
 
 }  // namespace sw2

@@ -569,10 +569,13 @@ __global__ void __launch_bounds__(256) k_bcg_k3(int N, KVecs<P> V, double *part,
 // y = A x with x = p (S = false) or x = s = r - alpha Ap (S = true, s stored)
 template <typename P, int K, bool S>
 __global__ void __launch_bounds__(256) k_pl_spmv(int N, KVecs<P> V, const int *sell_ptr, const int *sell_col,
-                                                 const double *x, double *y, const KScal *sc, int check_done) {
+                                                 const double *x, double *y, const int *blk_inst, const KScal *sc,
+                                                 int check_done) {
+    int nend = N;  // ensembles: the block's model and that model's last node + 1
+    if (blk_inst) { sc += blk_inst[2 * blockIdx.x]; nend = blk_inst[2 * blockIdx.x + 1]; }
     if (check_done && sc->done) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    if (n >= nend) return;
     double a[3];
     if constexpr (S) {
         VecS<double> sf{V.r, V.Ap, -sc->alpha};
@@ -763,7 +766,7 @@ static void pl_prologue_k(Ctx *c) {
     hipLaunchKernelGGL(k_pl_p1<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
     hipLaunchKernelGGL(k_fin<FIN_P1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
     hipLaunchKernelGGL((k_pl_spmv<P, K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
-                       c->sell_col, V.u, V.As, c->ksc, 0);
+                       c->sell_col, V.u, V.As, BI(c), c->ksc, 0);
     prec_solve(c, V.As, nullptr);
     hipLaunchKernelGGL(k_pl_p2<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c), c->ksc);
     hipLaunchKernelGGL(k_fin<FIN_P2>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
@@ -775,13 +778,13 @@ static void pl_iters_k(Ctx *c, int k) {
     KVecs<P> V = vecs<P>(c);
     for (int i = 0; i < k; i++) {
         hipLaunchKernelGGL((k_pl_spmv<P, K, false>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
-                           c->sell_col, V.p, V.Ap, c->ksc, 1);
+                           c->sell_col, V.p, V.Ap, BI(c), c->ksc, 1);
         prec_solve(c, V.Ap, c->ksc);
         hipLaunchKernelGGL(k_pl_gamma<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
         hipLaunchKernelGGL(k_fin<FIN_K1>, dim3(NFIN(c)), dim3(256), 0, c->stream, nb, c->red.part, c->ksc, IB0(c));
         hipLaunchKernelGGL((k_pl_spmv<P, K, true>), dim3(nb), dim3(256), 0, c->stream, c->N, V, c->sell_ptr,
-                           c->sell_col, nullptr, V.As, c->ksc, 1);
+                           c->sell_col, nullptr, V.As, BI(c), c->ksc, 1);
         prec_solve(c, V.As, c->ksc);
         hipLaunchKernelGGL(k_pl_omega<P>, dim3(nb), dim3(256), 0, c->stream, c->N, V, c->red.part, BI(c),
                            c->ksc);
@@ -987,7 +990,7 @@ static void gm_prec(Ctx *c, double *x) {
 template <typename P, int K>
 static void gm_spmv_k(Ctx *c, const double *x, double *y) {
     hipLaunchKernelGGL((k_pl_spmv<P, K, false>), dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N,
-                       vecs<P>(c), c->sell_ptr, c->sell_col, x, y, c->ksc, 0);
+                       vecs<P>(c), c->sell_ptr, c->sell_col, x, y, nullptr, c->ksc, 0);
 }
 
 // y = M^-1 A x

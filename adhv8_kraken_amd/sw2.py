@@ -398,7 +398,8 @@ class SW2Ensemble(_Tunable):
     """Independent SW2 models advanced together on one GPU: one device mesh
     holds them side by side, so every phase is one launch over all models,
     while Newton, Krylov and time-step decisions stay per model (each model
-    computes what it computes alone).  Block-Jacobi preconditioner only."""
+    computes what it computes alone).  3x3 block Jacobi, or the profile LU
+    of each model's node blocks (opts.nblock per model, tiled band LU)."""
 
     def __init__(self, problems, opts: Opts | None = None):
         L = lib()
@@ -492,6 +493,52 @@ class SW2Ensemble(_Tunable):
                             hist_l2=hists[i][1][:k].copy(), hist_inc=hists[i][2][:k].copy(),
                             hist_lin_it=hists[i][3][:k].copy()))
         return out
+
+
+def profile_lu_bytes(p, nblock: int = 1) -> int:
+    """Device bytes the tiled band LU takes for problem p cut into `nblock`
+    node blocks by the reference's serial partition (node_order.c:64-104:
+    N / nblock consecutive nodes per block, the last taking the rest).  A
+    block of n nodes has m = 3 n DOFs and pattern half bandwidth w = 3 d + 2
+    (d: the largest node-number distance of two nodes of one element inside
+    the block); it stores the band, m (2 (w + 64) + 1) doubles, the inverses
+    of its 64 x 64 diagonal factor tiles (2 x 4096 doubles per 64 DOFs), two
+    band-ordered vectors and one panel copy (8,256 doubles)."""
+    N = int(p.nnodes)
+    nb = min(max(1, int(nblock)), N)
+    inc = max(1, N // nb)
+    blk = np.minimum(np.arange(N) // inc, nb - 1)
+    maxd = np.zeros(nb, dtype=np.int64)
+    e2 = np.asarray(p.e2d, dtype=np.int64).reshape(-1, 3)
+    e1 = np.asarray(p.e1d, dtype=np.int64).reshape(-1, 2)
+    for a, b in [(e2[:, 0], e2[:, 1]), (e2[:, 1], e2[:, 2]), (e2[:, 0], e2[:, 2]), (e1[:, 0], e1[:, 1])]:
+        same = blk[a] == blk[b]
+        if np.any(same):
+            np.maximum.at(maxd, blk[a[same]], np.abs(a[same] - b[same]))
+    n = np.bincount(blk, minlength=nb).astype(np.int64)
+    m = 3 * n
+    w = 3 * maxd + 2
+    doubles = m * (2 * (w + 64) + 1) + 2 * 4096 * ((m + 63) // 64) + 2 * m + 8256
+    return int(8 * doubles.sum())
+
+
+def lu_groups(problems, nblock: int = 1, budget: float = 200e9) -> list:
+    """Indices of `problems` in groups for ensembles with the profile LU,
+    whose bands must fit in device memory together: by increasing storage
+    (instances of like size share a group, so a group's launches, as many
+    as its largest band has panels, serve all its members), each group's
+    profile_lu_bytes within `budget` (an instance above it alone)."""
+    sizes = [profile_lu_bytes(p, nblock) for p in problems]
+    groups, cur, tot = [], [], 0
+    for i in sorted(range(len(problems)), key=lambda k: sizes[k]):
+        if cur and tot + sizes[i] > budget:
+            groups.append(cur)
+            cur, tot = [], 0
+        cur.append(i)
+        tot += sizes[i]
+    if cur:
+        groups.append(cur)
+    return groups
 
 
 def cr_log(x: float) -> float:

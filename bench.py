@@ -413,6 +413,100 @@ def direct_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def direct_ensemble_mode(args):
+    """The whole wet/dry floodplain ensemble (or the --instances subset) with
+    the reference's default preconditioner, OP BLK 1, inside the ensemble
+    handle: instances grouped by band storage (sw2.lu_groups, at most
+    --lu-budget GB of bands per group, like sizes together), each group one
+    handle whose tiled-LU launches factor and sweep all its members' bands
+    at once; FP64, --steps time steps (default 1).  --compare K also runs the
+    K smallest instances one at a time through the single-model path."""
+    from adhv8_kraken_amd import mesh
+    from adhv8_kraken_amd.sw2 import SW2Ensemble, SW2Model, default_opts, lu_groups, profile_lu_bytes, YES
+    ps = mesh.floodplain_ensemble()
+    idx = [int(k) for k in args.instances.split(",")] if args.instances else list(range(len(ps)))
+    steps = args.steps if args.steps_set else 1
+    sub = [ps[i] for i in idx]
+    groups = [[idx[j] for j in g] for g in lu_groups(sub, 1, args.lu_budget * 1e9)]
+    log(f"floodplain-direct-ensemble: {len(idx)} instances in {len(groups)} groups "
+        f"({[len(g) for g in groups]}), {sum(profile_lu_bytes(ps[i]) for i in idx) / 1e9:.1f} GB of bands")
+
+    def opts(tol=1e-8):
+        return default_opts(preconditioner=1, nblock=1, max_nonlin_it=40, tol_nonlin=tol)
+
+    rows, tri_newton, seconds, fails = [], 0, 0.0, 0
+    for gi, g in enumerate(groups):
+        gp = [ps[i] for i in g]
+        ens = SW2Ensemble(gp, opts())
+        r0 = ens.residual_norm()
+        for j, p in enumerate(gp):
+            ens.set_state(j, p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+            ens.set_tol(j, 1e-8 * r0[j])
+        its = np.zeros(len(g), dtype=np.int64)
+        ok = np.zeros(len(g), dtype=np.int64)
+        t0 = time.time()
+        for _ in range(steps):
+            rs = ens.timestep(max_retries=2)
+            for j, r in enumerate(rs):
+                its[j] += r["iterations"]
+                ok[j] += r["verdict"] == YES
+        dt = time.time() - t0
+        ens.close()
+        tn = int(sum(its[j] * gp[j].nelems2d for j in range(len(g))))
+        tri_newton += tn
+        seconds += dt
+        fails += int(steps * len(g) - ok.sum())
+        rows.append(dict(instances=g, triangles=int(sum(p.nelems2d for p in gp)),
+                         band_gb=round(sum(profile_lu_bytes(p) for p in gp) / 1e9, 1),
+                         newton_its=int(its.sum()), max_newton_its=int(its.max()), steps_ok=int(ok.sum()),
+                         seconds=round(dt, 2), tri_newton_per_s=round(tn / dt)))
+        log(f"  group {gi}: {len(g)} instances, {int(its.sum())} Newton its (max {int(its.max())}), "
+            f"{int(ok.sum())}/{steps * len(g)} steps ok, {dt:.1f} s")
+    seq = None
+    if args.compare > 0:
+        small = sorted(idx, key=lambda i: profile_lu_bytes(ps[i]))[: args.compare]
+        tn_e, t_e, tn_s, t_s = 0, 0.0, 0, 0.0
+        gp = [ps[i] for i in small]
+        ens = SW2Ensemble(gp, opts())
+        r0 = ens.residual_norm()
+        for j, p in enumerate(gp):
+            ens.set_state(j, p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+            ens.set_tol(j, 1e-8 * r0[j])
+        t0 = time.time()
+        rs = ens.timestep(max_retries=2)
+        t_e = time.time() - t0
+        tn_e = sum(r["iterations"] * p.nelems2d for r, p in zip(rs, gp))
+        ens.close()
+        for j, p in enumerate(gp):
+            m = SW2Model(p, opts())
+            m.shift(); m.init()
+            m.set_state(p.head0, p.vel0, p.head0, p.vel0, p.head0, p.vel0)
+            m.set_opts(opts(1e-8 * r0[j]))
+            t0 = time.time()
+            r = m.timestep(max_retries=2)
+            t_s += time.time() - t0
+            tn_s += r["iterations"] * p.nelems2d
+            m.close()
+        seq = dict(instances=small, ensemble_tri_newton_per_s=round(tn_e / t_e),
+                   sequential_tri_newton_per_s=round(tn_s / t_s), speedup=round((tn_e / t_e) / (tn_s / t_s), 2))
+        log(f"  {len(small)} smallest instances, one step: ensemble {tn_e / t_e / 1e6:.2f} vs one at a time "
+            f"{tn_s / t_s / 1e6:.2f} M triangle-Newton/s")
+    out = {
+        "metric": "SW2 floodplain ensemble with OP BLK 1 inside the ensemble handle (triangles x Newton iterations "
+                  "per second)",
+        "value": round(tri_newton / seconds), "unit": "triangle-Newton steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "wet/dry floodplain ensemble (64 instances, 50k-500k triangles, seed 5), OP BLK 1 "
+                               "(tiled band LU of each instance's whole matrix), BiCGSTAB, Newton to 1e-8 of each "
+                               "instance's opening residual, dt 2 s cut by 0.25 per failed attempt (2 retries), "
+                               f"groups of at most {args.lu_budget:g} GB of bands",
+                   "instances": len(idx), "time_steps": steps},
+        "seconds": round(seconds, 2), "failed_steps": fails, "groups": rows, "vs_one_at_a_time": seq,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def time_control_mode(args, ps, precision, desc, ntri):
     """Every instance integrated to t = --time-span with the reference's time
     control, capped at --wall seconds: reports how far the instances got."""
@@ -478,10 +572,14 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes of the element list kernels (for clean kernel traces)")
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "channel-ensemble", "floodplain-ensemble", "floodplain-direct"],
+                    choices=["channel", "channel-ensemble", "floodplain-ensemble", "floodplain-direct",
+                             "floodplain-direct-ensemble"],
                     help="channel: the 1M-triangle headline; the others print their own JSON line")
     ap.add_argument("--instances", default="",
-                    help="floodplain-direct: comma-separated instance indices (default: smallest, median, largest)")
+                    help="floodplain-direct: comma-separated instance indices (default: smallest, median, "
+                         "largest); floodplain-direct-ensemble: the subset to run (default: all 64)")
+    ap.add_argument("--lu-budget", type=float, default=200.0,
+                    help="floodplain-direct-ensemble: GB of profile-LU bands per ensemble group")
     ap.add_argument("--compare", type=int, default=0,
                     help="ensembles: also run the first K instances one by one through the single-model path")
     ap.add_argument("--time-span", type=float, default=0.0,
@@ -509,6 +607,9 @@ def main():
         sys.exit(2)
     if args.workload == "floodplain-direct":
         direct_mode(args)
+        return
+    if args.workload == "floodplain-direct-ensemble":
+        direct_ensemble_mode(args)
         return
     if args.workload != "channel":
         ensemble_mode(args)

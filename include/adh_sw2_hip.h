@@ -231,8 +231,12 @@ int adh_sw2_hip_set_series(adh_sw2_hip *h, const double *e1d_val, const double *
  * model in an ensemble computes exactly what it computes alone.
  * The models must share gravity, tau_temporal, tau_pg, the drying limits and
  * viscosity; meshes, materials, boundary conditions, states and dt may
- * differ.  Ensembles use the 3x3 block-Jacobi preconditioner (opts NULL or
- * preconditioner 2) in any precision form.  The handle works with
+ * differ.  Ensembles use BiCGSTAB with the 3x3 block-Jacobi preconditioner
+ * (opts NULL or preconditioner 2) in any precision form, or with the profile
+ * LU (preconditioner 1): each model's nodes cut into opts.nblock blocks as
+ * that model alone would be, all models' bands factored by the tiled band LU
+ * in the same launches (their storage must fit in device memory together:
+ * set-up fails with ADH_SW2_EUNSUPPORTED otherwise).  The handle works with
  * set_opts and destroy; the single-model state, init and step calls are not
  * meant for it. */
 int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const adh_sw2_opts *opts, adh_sw2_hip **out);

@@ -79,6 +79,44 @@ def test_ensemble_matches_single_models(family, precision):
     assert sum(ereps[s][i]["verdict"] == YES for s in range(STEPS) for i in range(len(ps))) >= 3 * len(ps) // 2
 
 
+@pytest.mark.parametrize("family,nblock", [("floodplain", 1), ("channel", 1), ("channel", 2)])
+def test_ensemble_profile_lu_matches_single_models(family, nblock):
+    """The profile LU (OP BLK) in an ensemble: each model's node range is cut
+    into `nblock` blocks as it is alone, and the tiled band LU factors and
+    sweeps every model's bands in the same launches (a model outside the
+    mask or converged is skipped).  Against single runs with the tiled LU:
+    bit-identical states, verdicts, iteration counts and histories."""
+    ps, tol = _family(family)
+    o = default_opts(preconditioner=1, nblock=nblock, lu_tiled=1, tol_nonlin=tol, max_nonlin_it=30)
+    say(f"ensemble profile LU {family} nblock={nblock}: {[p.nnodes for p in ps]} nodes")
+    single = []
+    for p in ps:
+        g = SW2Model(p, o)
+        reps = [g.timestep(max_retries=RETRIES, hist_cap=256) for _ in range(STEPS)]
+        h, v, *_ = g.get_state()
+        single.append((reps, h, v))
+        g.close()
+    ens = SW2Ensemble(ps, o)
+    ereps = [ens.timestep(max_retries=RETRIES, hist_cap=256) for _ in range(STEPS)]
+    for i in range(len(ps)):
+        reps, h, v = single[i]
+        for s in range(STEPS):
+            a, b = reps[s], ereps[s][i]
+            say(f"  model {i} step {s}: single vd={a['verdict']} its={a['iterations']} lin={a['lin_its']} "
+                f"attempts={_attempts(a)} | ensemble vd={b['verdict']} its={b['iterations']} lin={b['lin_its']} "
+                f"attempts={_attempts(b)}")
+            assert a["verdict"] == b["verdict"]
+            assert a["iterations"] == b["iterations"]
+            assert a["lin_its"] == b["lin_its"]
+            for key in ("hist_max", "hist_l2", "hist_lin_it"):
+                assert np.array_equal(a[key], b[key], equal_nan=True), (key, a[key], b[key])
+        eh, ev = ens.get_state(i)
+        assert np.array_equal(h, eh), int(np.count_nonzero(h != eh))
+        assert np.array_equal(v, ev), int(np.count_nonzero(v != ev))
+    ens.close()
+    assert all(ereps[s][i]["verdict"] == YES for s in range(STEPS) for i in range(len(ps)))
+
+
 def test_ensemble_state_roundtrip_and_dt():
     """Per-model state and dt calls address their own model's nodes only."""
     ps, _ = _family("channel")
@@ -122,6 +160,34 @@ def test_ensemble_subset_steps_and_dt_readback():
     for _ in range(2):
         rs = ens.timestep(max_retries=RETRIES, which=[1, 0, 1])
         assert rs[1]["verdict"] == YES and rs[1]["iterations"] == 0
+    for i in range(3):
+        h, v = ens.get_state(i)
+        assert np.array_equal(h, single[i][0]) and np.array_equal(v, single[i][1]), i
+    ens.close()
+
+
+def test_ensemble_profile_lu_subset_steps():
+    """With the profile LU, a model left out of a time step is neither
+    factored nor swept (the tiled LU's launches skip its bands) and stays
+    untouched; the stepped models match their single runs bit for bit."""
+    ps, tol = _family("floodplain")
+    ps = ps[:3]
+    o = default_opts(preconditioner=1, nblock=1, lu_tiled=1, tol_nonlin=tol, max_nonlin_it=30)
+    counts = [3, 1, 2]
+    single = []
+    for p, k in zip(ps, counts):
+        g = SW2Model(p, o)
+        for _ in range(k):
+            g.timestep(max_retries=RETRIES)
+        h, v, *_ = g.get_state()
+        single.append((h, v))
+        g.close()
+    ens = SW2Ensemble(ps, o)
+    ens.timestep(max_retries=RETRIES, which=[1, 1, 1])
+    rs = ens.timestep(max_retries=RETRIES, which=[1, 0, 1])
+    assert rs[1]["iterations"] == 0
+    rs = ens.timestep(max_retries=RETRIES, which=[1, 0, 0])
+    assert rs[1]["iterations"] == 0 and rs[2]["iterations"] == 0
     for i in range(3):
         h, v = ens.get_state(i)
         assert np.array_equal(h, single[i][0]) and np.array_equal(v, single[i][1]), i
