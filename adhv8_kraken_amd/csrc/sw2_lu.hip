@@ -181,13 +181,18 @@ __global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB,
 // the waves to its right take the rank-16 update.  U side: lane t holds
 // column t, wave w its rows 16 w .. 16 w + 15; wave P finishes its rows and
 // the waves below take their update.  The diagonal block's entries come from
-// an LDS copy of the panel block (broadcast reads; strict triangles, so every
-// update runs over whole 16-column groups).
+// an LDS copy of the half of the panel block the side uses (broadcast reads;
+// strict triangles, so every update runs over whole 16-column groups): the
+// L side the strict U rows and the reciprocal guarded pivots, the U side the
+// strict L columns.  LDS: that copy, and one region that is the U side's
+// transpose buffer before and after the blocked steps and the published
+// 16-column (16-row) groups during them -- 66.5 KB, two workgroups per CU.
 __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB, const double *panel, int k,
                                                  int gl, const int *mask) {
-    __shared__ double T[TB][TB + 1];    // U side: the tile, transposed through LDS
-    __shared__ double pan[2][TB][17];   // a finished 16-column (L) or 16-row (U) group, by parity
-    __shared__ double Pl[TB_PANEL];     // the panel block: strict U rows | strict L columns | 1 / guarded pivots
+    __shared__ double sm[2 * TB * (TB + 1)];
+    double (*T)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm);      // U side: the tile, transposed
+    double (*pan)[TB][17] = reinterpret_cast<double (*)[TB][17]>(sm);    // a finished group, by parity
+    double *Pl = sm + TB * (TB + 1);                                     // the panel half
     const TBand B = band_rec(bands, blockIdx.y);
     if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
@@ -197,26 +202,21 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
     const int g = lside ? blockIdx.x : blockIdx.x - gl;
     const int base = r0 + TB * g;
     if (base >= r1) return;
-    const double *Pg = panel + (size_t)TB_PANEL * blockIdx.y;
-    // all of a thread's loads first, then its LDS stores (a load-store loop
-    // would wait for each load in turn)
+    // the panel half: [strict U rows | 1 / guarded pivots] (L side) or strict
+    // L columns (U side); a thread's loads all issued before its LDS stores
+    const double *Pg = panel + (size_t)TB_PANEL * blockIdx.y + (lside ? 0 : TB * TB);
     {
-        constexpr int NP = (TB_PANEL + 255) / 256;
-        double v[NP];
+        double v[16];
 #pragma unroll
-        for (int u = 0; u < NP; u++) {
-            const int e = threadIdx.x + 256 * u;
-            v[u] = Pg[e < TB_PANEL ? e : 0];
-        }
+        for (int u = 0; u < 16; u++) v[u] = Pg[threadIdx.x + 256 * u];
+        const double rv = lside && threadIdx.x < TB ? Pg[2 * TB * TB + threadIdx.x] : 0.0;
 #pragma unroll
-        for (int u = 0; u < NP; u++) {
-            const int e = threadIdx.x + 256 * u;
-            if (e < TB_PANEL) Pl[e] = v[u];
-        }
+        for (int u = 0; u < 16; u++) Pl[threadIdx.x + 256 * u] = v[u];
+        if (lside && threadIdx.x < TB) Pl[TB * TB + threadIdx.x] = rv;
     }
-    const double *U = Pl, *L = Pl + TB * TB, *R = Pl + 2 * TB * TB;
     double r[16];
     if (lside) {
+        const double *U = Pl, *R = Pl + TB * TB;
         const int i = base + lane;
         const bool act = i < r1;
         const int ic = min(i, r1 - 1);
@@ -257,6 +257,7 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
             if (act && col < kb) tb_at(AB, B, i, c0 + col) = r[j];
         }
     } else {
+        const double *L = Pl;
         const int ncol = min(TB, r1 - base);
         double tv[16];
 #pragma unroll
@@ -272,6 +273,7 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 16; j++) r[j] = T[16 * wv + j][lane];
+        __syncthreads();   // the groups below overwrite the transpose buffer
 #pragma unroll
         for (int P = 0; P < 4; P++) {
             if (wv == P) {
@@ -294,6 +296,7 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
                 }
             }
         }
+        // the last group reads ended before the last barrier above
 #pragma unroll
         for (int j = 0; j < 16; j++) T[16 * wv + j][lane] = r[j];
         __syncthreads();
