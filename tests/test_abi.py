@@ -79,9 +79,9 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 
 
 def test_ensemble_arguments_are_checked_before_the_device():
-    """adh_sw2_hip_batch_create rejects empty ensembles, the profile-LU
-    preconditioner and models that disagree on the shared constants before
-    any device call."""
+    """adh_sw2_hip_batch_create rejects empty ensembles, the block ILU(0)
+    preconditioner, GMRES and models that disagree on the shared constants
+    before any device call."""
     from adhv8_kraken_amd import mesh
     L = sw2.lib()
     keep = []
@@ -91,7 +91,9 @@ def test_ensemble_arguments_are_checked_before_the_device():
     arr[1] = sw2._mesh_struct(b, keep)
     h = C.c_void_p()
     assert L.adh_sw2_hip_batch_create(arr, 0, None, C.byref(h)) == -1
-    o = sw2.default_opts(preconditioner=1)
+    o = sw2.default_opts(preconditioner=3)
+    assert L.adh_sw2_hip_batch_create(arr, 2, C.byref(o), C.byref(h)) == -5
+    o = sw2.default_opts(krylov=1)
     assert L.adh_sw2_hip_batch_create(arr, 2, C.byref(o), C.byref(h)) == -5
     arr[1].gravity = 9.81
     assert L.adh_sw2_hip_batch_create(arr, 2, None, C.byref(h)) == -1
