@@ -29,10 +29,11 @@
 //              waves) while the other tiles are updated.
 // k_tb_diag factors panel 0's diagonal block the same way.
 // Then k_tb_inv forms the inverses of every diagonal block's unit-lower and
-// upper factors (all blocks in parallel), so that each step of the
-// substitutions is matrix-vector work: the forward sweep is one launch per
-// tile (y_k = L_kk^-1 x_k, then the tiles below take -= L_ik y_k, one wave
-// each), the backward sweep likewise upwards.
+// upper factors (all blocks in parallel) and k_tb_scale multiplies every
+// off-diagonal tile by its column's inverse, so that each step of the
+// substitutions is independent matrix-vector work: the forward sweep is one
+// launch per tile (block 0 y_k = L_kk^-1 x_k, the blocks below
+// x_i -= (L_ik L_kk^-1) x_k), the backward sweep likewise upwards.
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -444,6 +445,69 @@ __global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double 
     for (int j = 0; j < TB; j++) UI[o + 64 * j + t] = T[t][j];
 }
 
+// After k_tb_inv: every off-diagonal tile of L (below a diagonal block) times
+// that block's LI, and of U (above it) times its UI, in place, so that a sweep
+// step's off-diagonal update reads one tile and no inverse: x_i -= (L_ik LI_k)
+// x_k = L_ik y_k, y_i -= (U_ik UI_k) y_k = U_ik z_k.  The products fill at most
+// the band's TB extra diagonals (entries within w + 63 of the diagonal), which
+// is what they are stored for.  One workgroup per tile, 4 x 4 register blocks
+// from LDS as in k_tb_gemm.
+__global__ void __launch_bounds__(256) k_tb_scale(const TBand *bands, double *AB, const double *LI, const double *UI,
+                                                  int gf, const int *mask) {
+    __shared__ double As[TB][TB];       // As[q][row]: the tile, column q
+    __shared__ double Ms[TB][TB + 1];   // Ms[q][col]: the inverse
+    const TBand B = band_rec(bands, blockIdx.z);
+    const int k = blockIdx.x;
+    if (k >= B.nt || (mask && !mask[B.inst])) return;
+    const bool low = (int)blockIdx.y < gf;
+    const int g = (low ? (int)blockIdx.y : (int)blockIdx.y - gf) + 1;
+    const int c0 = k * TB, kb = min(TB, B.m - c0);
+    // the tile's first row, and its rows inside the reach of the panel's columns
+    const int tr0 = low ? c0 + TB * g : c0 - TB * g;
+    const int i0 = low ? tr0 : max(tr0, max(0, c0 - B.w));
+    const int i1 = low ? min(tr0 + TB, min(B.m, c0 + kb + B.w)) : tr0 + TB;
+    if (i0 >= i1) return;
+    const int t = threadIdx.x;
+    const double *M = (low ? LI : UI) + 4096 * (B.toff + k);
+    double av[16], mv[16];   // all loads in flight before the LDS stores
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = t + 256 * u, lo = e & (TB - 1), hi = e >> 6;
+        av[u] = tb_get(AB, B, min(max(tr0 + lo, i0), i1 - 1), c0 + min(hi, kb - 1));
+        mv[u] = M[e];   // column-major: row lo, column hi
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = t + 256 * u, lo = e & (TB - 1), hi = e >> 6;
+        const int r = tr0 + lo;
+        As[hi][lo] = (r >= i0 && r < i1 && hi < kb) ? av[u] : 0.0;
+        Ms[lo][hi] = mv[u];
+    }
+    __syncthreads();
+    const int tr = t & 15, tc = t >> 4;   // rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3
+    double acc[4][4] = {};
+#pragma unroll 8
+    for (int q = 0; q < kb; q++) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { a[u] = As[q][4 * tr + u]; b[u] = Ms[q][4 * tc + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int c = 4 * tc + v;
+        if (c >= kb) continue;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int r = tr0 + 4 * tr + u;
+            if (r >= i0 && r < i1) tb_at(AB, B, r, c0 + c) = acc[u][v];
+        }
+    }
+}
+
 // sliced node vector <-> band-ordered vector (nodes of no block skipped; the
 // scatter leaves a model that has converged, or is outside the mask, as it is)
 __global__ void k_tb_gather(int N, const int *node_blk, const TBand *bands, const double *x, double *xb) {
@@ -468,10 +532,10 @@ __global__ void k_tb_scatter(int N, const int *node_blk, const TBand *bands, con
 // Sweep kernels: four waves per workgroup split a tile's 64 columns (16
 // each); lane r accumulates row r over its wave's columns (the column value
 // broadcast from lane c) and the four partial sums are added in LDS in wave
-// order.  Every global load of a step (the inverse tile, the vector slice, the
-// off-diagonal tile and the entry it updates) depends only on the band record,
-// so all are issued before the first reduction: one memory round trip per
-// step instead of one per phase.
+// order.  Block 0 applies the diagonal block's inverse; block g >= 1 the
+// off-diagonal tile scaled by it (k_tb_scale), to the same input slice, so
+// the blocks of a step are independent and each reads one tile: all of a
+// block's loads are issued at once after a single band-record fetch.
 __device__ __forceinline__ double tile_mv_part(const double (&m)[16], double xr, int wv) {
     double s = 0.0;
 #pragma unroll
@@ -483,7 +547,7 @@ __device__ __forceinline__ double sum4(double (*part)[TB], int lane) {
 }
 
 // Forward step k: block 0 stores y_k = LI_k x_k in yb; block g >= 1 takes
-// x[tile k + g] -= L[tile k + g, tile k] y_k.
+// x[tile k + g] -= (L LI_k)[tile k + g, tile k] x_k.
 __global__ void __launch_bounds__(256) k_tb_fwd(const TBand *bands, const double *AB, const double *LI, double *xb,
                                                 double *yb, int k, const int *done, const int *mask,
                                                 const KScal *sc) {
@@ -498,39 +562,29 @@ __global__ void __launch_bounds__(256) k_tb_fwd(const TBand *bands, const double
     if (g > 0 && c0 + TB * g >= rmax) return;
     const int i = c0 + TB * g + lane, ic = min(i, rmax - 1);
     const double xv = xb[B.xoff + c0 + min(lane, kb - 1)];
-    const double *M = LI + 4096 * (B.toff + k);
-    double m[16], a[16];
+    double m[16], xi = 0.0;
+    if (g == 0) {
+        const double *M = LI + 4096 * (B.toff + k);
 #pragma unroll
-    for (int u = 0; u < 16; u++) m[u] = M[64 * (16 * wv + u) + lane];
-    double xi = 0.0;
-    if (g > 0) {
+        for (int u = 0; u < 16; u++) m[u] = M[64 * (16 * wv + u) + lane];
+    } else {
 #pragma unroll
-        for (int u = 0; u < 16; u++) a[u] = tb_get(AB, B, ic, c0 + min(16 * wv + u, kb - 1));
+        for (int u = 0; u < 16; u++) m[u] = tb_get(AB, B, ic, c0 + min(16 * wv + u, kb - 1));
         xi = xb[B.xoff + ic];
     }
     const double xr = lane < kb ? xv : 0.0;
     part[wv][lane] = tile_mv_part(m, xr, wv);
     __syncthreads();
-    const double yk = sum4(part, lane);
+    if (wv != 0) return;
     if (g == 0) {
-        if (wv == 0 && lane < kb) yb[B.xoff + c0 + lane] = yk;
-        return;
+        if (lane < kb) yb[B.xoff + c0 + lane] = sum4(part, lane);
+    } else if (i < rmax) {
+        xb[B.xoff + i] = xi - sum4(part, lane);
     }
-    double s = 0.0;
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int c = 16 * wv + u;
-        const double yc = lane_bcast(yk, c);
-        if (c < kb) s = fma(a[u], yc, s);
-    }
-    __syncthreads();
-    part[wv][lane] = s;
-    __syncthreads();
-    if (wv == 0 && i < rmax) xb[B.xoff + i] = xi - sum4(part, lane);
 }
 
 // Backward step k: block 0 stores z_k = UI_k y_k in xb; block g >= 1 takes
-// y[tile k - g] -= U[tile k - g, tile k] z_k.
+// y[tile k - g] -= (U UI_k)[tile k - g, tile k] y_k.
 __global__ void __launch_bounds__(256) k_tb_bwd(const TBand *bands, const double *AB, const double *UI, double *xb,
                                                 double *yb, int k, const int *done, const int *mask,
                                                 const KScal *sc) {
@@ -545,35 +599,25 @@ __global__ void __launch_bounds__(256) k_tb_bwd(const TBand *bands, const double
     if (g > 0 && c0 - TB * g + TB <= rmin) return;
     const int i = c0 - TB * g + lane, ic = max(i, rmin);
     const double yv = yb[B.xoff + c0 + min(lane, kb - 1)];
-    const double *M = UI + 4096 * (B.toff + k);
-    double m[16], a[16];
+    double m[16], yi = 0.0;
+    if (g == 0) {
+        const double *M = UI + 4096 * (B.toff + k);
 #pragma unroll
-    for (int u = 0; u < 16; u++) m[u] = M[64 * (16 * wv + u) + lane];
-    double yi = 0.0;
-    if (g > 0) {
+        for (int u = 0; u < 16; u++) m[u] = M[64 * (16 * wv + u) + lane];
+    } else {
 #pragma unroll
-        for (int u = 0; u < 16; u++) a[u] = tb_get(AB, B, ic, c0 + min(16 * wv + u, kb - 1));
+        for (int u = 0; u < 16; u++) m[u] = tb_get(AB, B, ic, c0 + min(16 * wv + u, kb - 1));
         yi = yb[B.xoff + ic];
     }
     const double yr = lane < kb ? yv : 0.0;
     part[wv][lane] = tile_mv_part(m, yr, wv);
     __syncthreads();
-    const double zk = sum4(part, lane);
+    if (wv != 0) return;
     if (g == 0) {
-        if (wv == 0 && lane < kb) xb[B.xoff + c0 + lane] = zk;
-        return;
+        if (lane < kb) xb[B.xoff + c0 + lane] = sum4(part, lane);
+    } else if (i >= rmin) {
+        yb[B.xoff + i] = yi - sum4(part, lane);
     }
-    double s = 0.0;
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int c = 16 * wv + u;
-        const double zc = lane_bcast(zk, c);
-        if (c < kb) s = fma(a[u], zc, s);
-    }
-    __syncthreads();
-    part[wv][lane] = s;
-    __syncthreads();
-    if (wv == 0 && i >= rmin) yb[B.xoff + i] = yi - sum4(part, lane);
 }
 
 void tband_free(Ctx *c) {
@@ -657,6 +701,9 @@ void launch_tband_setup(Ctx *c) {
     }
     hipLaunchKernelGGL(k_tb_inv, dim3(c->tb_maxnt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
                        c->tb_LI, c->tb_UI, mask);
+    const int gf = std::max(1, c->tb_gf);
+    hipLaunchKernelGGL(k_tb_scale, dim3(c->tb_maxnt, 2 * gf, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands,
+                       c->tb_AB, c->tb_LI, c->tb_UI, gf, mask);
 }
 
 // sc: the Krylov scalars (one per model) whose done flags end the sweeps, or
