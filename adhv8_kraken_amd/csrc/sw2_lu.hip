@@ -662,7 +662,8 @@ std::string tband_structure(Ctx *c, const std::vector<int> &s0, const std::vecto
     const size_t need = sizeof(double) * (off + 2 * 4096 * toff + 2 * xoff);
     if (need + ((size_t)2 << 30) > freeb)
         return "tiled band LU: " + std::to_string(need >> 20) + " MiB of band storage do not fit in the free " +
-               std::to_string(freeb >> 20) + " MiB; raise nblock";
+               std::to_string(freeb >> 20) + " MiB; raise nblock" +
+               (c->ninst ? ", or step the models in groups that fit" : "");
     if (hipMalloc((void **)&c->tb_bands, sizeof(TBand) * nblk) != hipSuccess ||
         hipMemcpy(c->tb_bands, h.data(), sizeof(TBand) * nblk, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc((void **)&c->tb_AB, sizeof(double) * off) != hipSuccess ||
