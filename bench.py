@@ -445,11 +445,13 @@ def direct_ensemble_mode(args):
         its = np.zeros(len(g), dtype=np.int64)
         ok = np.zeros(len(g), dtype=np.int64)
         t0 = time.time()
-        for _ in range(steps):
+        for st in range(steps):
             rs = ens.timestep(max_retries=2)
             for j, r in enumerate(rs):
                 its[j] += r["iterations"]
                 ok[j] += r["verdict"] == YES
+            log(f"  group {gi} step {st}: {sum(r['iterations'] for r in rs)} Newton its, "
+                f"{sum(r['verdict'] == YES for r in rs)}/{len(g)} ok, {time.time() - t0:.1f} s")
         dt = time.time() - t0
         ens.close()
         tn = int(sum(its[j] * gp[j].nelems2d for j in range(len(g))))
