@@ -1285,6 +1285,7 @@ extern "C" int adh_sw2_hip_timestep(adh_sw2_hip *h, adh_sw2_report *rep, int *ve
 static int set_mask(Ctx &c, const std::vector<int> &m) {
     c.split_valid = 0;  // the element lists hold the masked models only
     HIPCHK(hipMemcpyAsync(c.inst_mask, m.data(), sizeof(int) * c.ninst, hipMemcpyHostToDevice, c.stream));
+    c.h_inst_mask = m;
     // the host vector may change before the copy runs
     HIPCHK(hipStreamSynchronize(c.stream));
     return 0;
@@ -1577,6 +1578,7 @@ extern "C" int adh_sw2_hip_batch_create(const adh_sw2_mesh *meshes, int n, const
     }
     c.h_inst_node0 = node0;
     c.h_inst_N = inst_nodes;
+    c.h_inst_mask = mask;
     c.h_inst_e0 = E0;
     c.h_inst_b0 = B0;
     c.h_inst_E.resize(n);

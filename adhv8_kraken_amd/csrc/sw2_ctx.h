@@ -157,6 +157,7 @@ struct Ctx {
     int *inst_nodes;       // [ninst] nodes of each model (the l2 norm's count)
     int *node_inst;        // [Np] model of each node (gap nodes: the model before them)
     int *inst_mask;        // [ninst] 1 = the masked node kernels update this model
+    std::vector<int> h_inst_mask;   // its host copy
     int *elem_inst, *edge_inst;
     double *dt_inst;
     int *loc_bc_inst;
@@ -182,6 +183,7 @@ struct Ctx {
     // workgroup band LU's size, or opts.lu_tiled): band_tiled selects it
     int band_tiled, band_lu_tiled_opt;
     int tb_nband, tb_maxnt, tb_gt, tb_gf;
+    std::vector<int> tb_h_nt, tb_h_inst;   // host copies: each band's panels and model
     size_t tb_size;
     TBand *tb_bands;
     double *tb_AB, *tb_LI, *tb_UI, *tb_xb, *tb_yb, *tb_panel;

@@ -627,6 +627,7 @@ void tband_free(Ctx *c) {
     c->tb_bands = nullptr; c->tb_AB = nullptr; c->tb_LI = nullptr; c->tb_UI = nullptr; c->tb_panel = nullptr;
     c->tb_xb = nullptr; c->tb_yb = nullptr;
     c->tb_nband = 0; c->tb_maxnt = 0; c->tb_gt = 0; c->tb_gf = 0; c->tb_size = 0;
+    c->tb_h_nt.clear(); c->tb_h_inst.clear();
 }
 
 // Bands of the node blocks (nodes [s0[b], s1[b]) of model inst[b]) with
@@ -677,6 +678,9 @@ std::string tband_structure(Ctx *c, const std::vector<int> &s0, const std::vecto
     }
     c->tb_nband = nblk;
     c->tb_maxnt = maxnt;
+    c->tb_h_nt.resize(nblk);
+    c->tb_h_inst.resize(nblk);
+    for (int b = 0; b < nblk; b++) { c->tb_h_nt[b] = h[b].nt; c->tb_h_inst[b] = h[b].inst; }
     c->tb_gt = (maxw + TB - 1) / TB;
     c->tb_gf = (maxw + 2 * TB - 2) / TB;   // tiles below (above) a panel that its L (U) reaches
     c->tb_size = off;
@@ -686,24 +690,37 @@ std::string tband_structure(Ctx *c, const std::vector<int> &s0, const std::vecto
 // the models the masked launches of an ensemble work on (null: every band)
 static const int *tb_mask(const Ctx *c) { return c->ninst ? c->inst_mask : nullptr; }
 
+// panels of the largest band whose model is in the mask: the launches past
+// it would find every band's model masked (an ensemble whose large models
+// have converged stops paying for their panels)
+static int tb_active_nt(const Ctx *c) {
+    if (!c->ninst || c->h_inst_mask.empty()) return c->tb_maxnt;
+    int nt = 0;
+    for (int b = 0; b < c->tb_nband; b++)
+        if (c->h_inst_mask[c->tb_h_inst[b]]) nt = std::max(nt, c->tb_h_nt[b]);
+    return nt;
+}
+
 void launch_tband_setup(Ctx *c) {
     const int *mask = tb_mask(c);
+    const int nt = tb_active_nt(c);
+    if (nt == 0) return;
     launch_zero(c, c->tb_AB, c->tb_size);
     hipLaunchKernelGGL(k_tb_load, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->node_blk,
                        c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col, mask);
     const int gt = std::max(1, c->tb_gt);
     hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
                        0, mask);
-    for (int k = 0; k < c->tb_maxnt; k++) {
+    for (int k = 0; k < nt; k++) {
         hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
                            c->tb_panel, k, gt, mask);
         hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
                            c->tb_panel, k, gt, mask);
     }
-    hipLaunchKernelGGL(k_tb_inv, dim3(c->tb_maxnt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
+    hipLaunchKernelGGL(k_tb_inv, dim3(nt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
                        c->tb_LI, c->tb_UI, mask);
     const int gf = std::max(1, c->tb_gf);
-    hipLaunchKernelGGL(k_tb_scale, dim3(c->tb_maxnt, 2 * gf, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands,
+    hipLaunchKernelGGL(k_tb_scale, dim3(nt, 2 * gf, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands,
                        c->tb_AB, c->tb_LI, c->tb_UI, gf, mask);
 }
 
@@ -718,10 +735,11 @@ void launch_tband_solve(Ctx *c, double *x, const KScal *sc) {
     // read beside the band record; ensembles: each band's model's flag
     const int *done = (sc && !mask) ? &sc->done : &c->tb_bands[0].pad;
     const KScal *msc = mask ? sc : nullptr;
-    for (int k = 0; k < c->tb_maxnt; k++)
+    const int nt = tb_active_nt(c);
+    for (int k = 0; k < nt; k++)
         hipLaunchKernelGGL(k_tb_fwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_LI, c->tb_xb, c->tb_yb,
                            k, done, mask, msc);
-    for (int k = c->tb_maxnt - 1; k >= 0; k--)
+    for (int k = nt - 1; k >= 0; k--)
         hipLaunchKernelGGL(k_tb_bwd, g, dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_UI, c->tb_xb, c->tb_yb,
                            k, done, mask, msc);
     hipLaunchKernelGGL(k_tb_scatter, gn, dim3(256), 0, c->stream, c->N, c->node_blk, c->tb_bands, c->tb_xb, x, sc,
