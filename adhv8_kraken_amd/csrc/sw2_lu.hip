@@ -22,10 +22,10 @@
 //   k_tb_trsm  a workgroup per 64-row tile of L (rows below the panel:
 //              L = A U~^-1 by substitution, U~ = U with the guarded pivots)
 //              and per 64-column tile of U (right of the panel: U = L^-1 A);
-//   k_tb_gemm  the trailing w x w window -= L U, 64 x 64 tiles, 4 x 4
+//   k_tb_gemm  the trailing w x w window -= L U, 64 x 64 tiles, 2 x 4
 //              register blocks per thread from LDS; the workgroup of the
 //              first tile, the next panel's diagonal block, then factors it
-//              (four waves, row t in lane t, 16 columns per wave, blocked by
+//              (eight waves, row t in lane t, 8 columns per wave, blocked by
 //              waves) while the other tiles are updated.
 // k_tb_diag factors panel 0's diagonal block the same way.
 // Then k_tb_inv forms the inverses of every diagonal block's unit-lower and
@@ -42,7 +42,10 @@
 
 namespace sw2 {
 
-constexpr int TB = 64;   // tile and panel width (DOFs)
+constexpr int TB = 64;          // tile and panel width (DOFs)
+constexpr int LW = 8;           // columns (rows) per wave in the blocked factor and solves
+constexpr int NWV = TB / LW;    // waves per workgroup of those kernels
+constexpr int NT = 64 * NWV;    // their threads
 
 __device__ __forceinline__ size_t tb_idx(const TBand &B, int i, int j) {
     return B.off + (size_t)(B.wp + i - j) + (size_t)j * (2 * B.wp + 1);
@@ -98,51 +101,54 @@ __global__ void k_tb_load(int N, const int *node_blk, const TBand *bands, double
     }
 }
 
-// LU of a 64 x 64 diagonal block, blocked by 16 columns: lane t of each of
-// the four waves holds row t (r[16]), wave w columns 16 w .. 16 w + 15.  Wave
-// P factors its 16 columns (right-looking: the pivot read from lane p with
+// LU of a 64 x 64 diagonal block, blocked by LW = 8 columns: lane t of each of
+// the eight waves holds row t (r[LW]), wave w columns 8 w .. 8 w + 7.  Wave
+// P factors its 8 columns (right-looking: the pivot read from lane p with
 // readlane, the multipliers as entry times the reciprocal guarded pivot),
 // publishes them in LDS, and after one barrier the waves to its right update
 // their columns with them (the same right-looking steps, multipliers read
-// from LDS); then wave P + 1 factors its (updated) columns.  Four barriers per
-// block instead of one per pivot.  Then the factors go back into the band and
+// from LDS); then wave P + 1 factors its (updated) columns.  Eight barriers
+// per block instead of one per pivot; the serial part of each phase, one
+// wave's pivots and the next wave's update, is half what 16-column groups
+// give, and every entry takes the same updates in the same order as with any
+// grouping.  Then the factors go back into the band and
 // [strict U by rows | strict L by columns | 1 / guarded pivots] into the
 // band's panel copy P for k_tb_trsm (zeros on and across the diagonal let it
-// update every column at every step).  pan: [2][TB][17] and rd: [TB] in LDS.
+// update every column at every step).  pan: [2][TB][LW + 1] and rd: [TB] in LDS.
 constexpr int TB_PANEL = 2 * TB * TB + TB;   // doubles of one band's panel copy
-__device__ __forceinline__ void diag_lu_store(double (&r)[16], double (*pan)[TB][17], double *rd, double *AB,
+__device__ __forceinline__ void diag_lu_store(double (&r)[LW], double (*pan)[TB][LW + 1], double *rd, double *AB,
                                               const TBand &B, double *P, int c0, int kb, int lane, int wv) {
 #pragma unroll
-    for (int Q = 0; Q < 4; Q++) {
+    for (int Q = 0; Q < NWV; Q++) {
         if (wv == Q) {
 #pragma unroll
-            for (int pj = 0; pj < 16; pj++) {
-                const int p = 16 * Q + pj;
+            for (int pj = 0; pj < LW; pj++) {
+                const int p = LW * Q + pj;
                 const double rden = 1.0 / guarded(lane_bcast(r[pj], p));
                 if (lane == p) rd[p] = rden;
                 const double l = lane > p ? r[pj] * rden : 0.0;
                 if (lane > p) r[pj] = l;
 #pragma unroll
-                for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-l, lane_bcast(r[qj], p), r[qj]);
+                for (int qj = pj + 1; qj < LW; qj++) r[qj] = fma(-l, lane_bcast(r[qj], p), r[qj]);
             }
 #pragma unroll
-            for (int j = 0; j < 16; j++) pan[Q & 1][lane][j] = r[j];
+            for (int j = 0; j < LW; j++) pan[Q & 1][lane][j] = r[j];
         }
         __syncthreads();
         if (wv > Q) {
 #pragma unroll
-            for (int sj = 0; sj < 16; sj++) {
-                const int sp = 16 * Q + sj;
+            for (int sj = 0; sj < LW; sj++) {
+                const int sp = LW * Q + sj;
                 const double l = lane > sp ? pan[Q & 1][lane][sj] : 0.0;
 #pragma unroll
-                for (int j = 0; j < 16; j++) r[j] = fma(-l, lane_bcast(r[j], sp), r[j]);
+                for (int j = 0; j < LW; j++) r[j] = fma(-l, lane_bcast(r[j], sp), r[j]);
             }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int col = 16 * wv + j;
+    for (int j = 0; j < LW; j++) {
+        const int col = LW * wv + j;
         if (lane < kb && col < kb) tb_at(AB, B, c0 + lane, c0 + col) = r[j];
         P[TB * lane + col] = col > lane ? r[j] : 0.0;              // U strictly above the diagonal, row `lane`
         P[TB * TB + TB * col + lane] = col < lane ? r[j] : 0.0;    // L, column `col` (unit diagonal not stored)
@@ -152,22 +158,22 @@ __device__ __forceinline__ void diag_lu_store(double (&r)[16], double (*pan)[TB]
 
 // Panel 0's diagonal block, one workgroup per band (every later panel's is
 // factored by k_tb_gemm of the panel before, which updates it last).
-__global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB, double *panel, int k,
-                                                 const int *mask) {
-    __shared__ double pan[2][TB][17];   // a factored 16-column group (double-buffered by group parity)
+__global__ void __launch_bounds__(NT) k_tb_diag(const TBand *bands, double *AB, double *panel, int k,
+                                                const int *mask) {
+    __shared__ double pan[2][TB][LW + 1];   // a factored column group (double-buffered by group parity)
     __shared__ double rd[TB];           // 1 / guarded pivots
     const TBand B = band_rec(bands, blockIdx.x);
     if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tc = min(lane, kb - 1);
-    double r[16];
+    double r[LW];
     // loads at clamped (in-band) positions, then zeroed outside the block: the
     // rows and columns past a short last block are zero, their pivots guarded
     // and their multipliers zero
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int col = 16 * wv + j;
+    for (int j = 0; j < LW; j++) {
+        const int col = LW * wv + j;
         const double v = tb_get(AB, B, c0 + tc, c0 + min(col, kb - 1));
         r[j] = (lane < kb && col < kb) ? v : 0.0;
     }
@@ -176,24 +182,24 @@ __global__ void __launch_bounds__(256) k_tb_diag(const TBand *bands, double *AB,
 
 // Panel k's L tiles (blockIdx.x < gl: 64 rows each below the panel) and U
 // tiles (blockIdx.x >= gl: 64 columns each right of it), one workgroup of
-// four waves per tile, blocked by 16 as k_tb_diag.  L side: lane t holds row
-// t of the tile, wave w its columns 16 w .. 16 w + 15; wave P forms the
-// multipliers of its 16 columns, publishes them in LDS, and after one barrier
-// the waves to its right take the rank-16 update.  U side: lane t holds
-// column t, wave w its rows 16 w .. 16 w + 15; wave P finishes its rows and
+// eight waves per tile, blocked by 8 as k_tb_diag.  L side: lane t holds row
+// t of the tile, wave w its columns 8 w .. 8 w + 7; wave P forms the
+// multipliers of its 8 columns, publishes them in LDS, and after one barrier
+// the waves to its right take the rank-8 update.  U side: lane t holds
+// column t, wave w its rows 8 w .. 8 w + 7; wave P finishes its rows and
 // the waves below take their update.  The diagonal block's entries come from
 // an LDS copy of the half of the panel block the side uses (broadcast reads;
-// strict triangles, so every update runs over whole 16-column groups): the
+// strict triangles, so every update runs over whole column groups): the
 // L side the strict U rows and the reciprocal guarded pivots, the U side the
 // strict L columns.  LDS: that copy, and one region that is the U side's
 // transpose buffer before and after the blocked steps and the published
-// 16-column (16-row) groups during them -- 66.5 KB, two workgroups per CU.
-__global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB, const double *panel, int k,
-                                                 int gl, const int *mask) {
+// column (row) groups during them -- 66.5 KB, two workgroups per CU.
+__global__ void __launch_bounds__(NT) k_tb_trsm(const TBand *bands, double *AB, const double *panel, int k,
+                                                int gl, const int *mask) {
     __shared__ double sm[2 * TB * (TB + 1)];
-    double (*T)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm);      // U side: the tile, transposed
-    double (*pan)[TB][17] = reinterpret_cast<double (*)[TB][17]>(sm);    // a finished group, by parity
-    double *Pl = sm + TB * (TB + 1);                                     // the panel half
+    double (*T)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm);          // U side: the tile, transposed
+    double (*pan)[TB][LW + 1] = reinterpret_cast<double (*)[TB][LW + 1]>(sm);   // a finished group, by parity
+    double *Pl = sm + TB * (TB + 1);                                         // the panel half
     const TBand B = band_rec(bands, blockIdx.y);
     if (k >= B.nt || (mask && !mask[B.inst])) return;
     const int c0 = k * TB, kb = min(TB, B.m - c0);
@@ -207,116 +213,118 @@ __global__ void __launch_bounds__(256) k_tb_trsm(const TBand *bands, double *AB,
     // L columns (U side); a thread's loads all issued before its LDS stores
     const double *Pg = panel + (size_t)TB_PANEL * blockIdx.y + (lside ? 0 : TB * TB);
     {
-        double v[16];
+        constexpr int NP = TB * TB / NT;
+        double v[NP];
 #pragma unroll
-        for (int u = 0; u < 16; u++) v[u] = Pg[threadIdx.x + 256 * u];
+        for (int u = 0; u < NP; u++) v[u] = Pg[threadIdx.x + NT * u];
         const double rv = lside && threadIdx.x < TB ? Pg[2 * TB * TB + threadIdx.x] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 16; u++) Pl[threadIdx.x + 256 * u] = v[u];
+        for (int u = 0; u < NP; u++) Pl[threadIdx.x + NT * u] = v[u];
         if (lside && threadIdx.x < TB) Pl[TB * TB + threadIdx.x] = rv;
     }
-    double r[16];
+    double r[LW];
     if (lside) {
         const double *U = Pl, *R = Pl + TB * TB;
         const int i = base + lane;
         const bool act = i < r1;
         const int ic = min(i, r1 - 1);
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int col = 16 * wv + j;
+        for (int j = 0; j < LW; j++) {
+            const int col = LW * wv + j;
             const double v = tb_get(AB, B, ic, c0 + min(col, kb - 1));   // clamped: in flight together
             r[j] = (act && col < kb) ? v : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int P = 0; P < 4; P++) {
+        for (int P = 0; P < NWV; P++) {
             if (wv == P) {
 #pragma unroll
-                for (int pj = 0; pj < 16; pj++) {
-                    const int p = 16 * P + pj;
+                for (int pj = 0; pj < LW; pj++) {
+                    const int p = LW * P + pj;
                     const double l = r[pj] * R[p];
                     r[pj] = l;
 #pragma unroll
-                    for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-l, U[TB * p + 16 * P + qj], r[qj]);
+                    for (int qj = pj + 1; qj < LW; qj++) r[qj] = fma(-l, U[TB * p + LW * P + qj], r[qj]);
                 }
 #pragma unroll
-                for (int j = 0; j < 16; j++) pan[P & 1][lane][j] = r[j];
+                for (int j = 0; j < LW; j++) pan[P & 1][lane][j] = r[j];
             }
             __syncthreads();
             if (wv > P) {
 #pragma unroll
-                for (int sj = 0; sj < 16; sj++) {
+                for (int sj = 0; sj < LW; sj++) {
                     const double l = pan[P & 1][lane][sj];
 #pragma unroll
-                    for (int j = 0; j < 16; j++) r[j] = fma(-l, U[TB * (16 * P + sj) + 16 * wv + j], r[j]);
+                    for (int j = 0; j < LW; j++) r[j] = fma(-l, U[TB * (LW * P + sj) + LW * wv + j], r[j]);
                 }
             }
         }
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int col = 16 * wv + j;
+        for (int j = 0; j < LW; j++) {
+            const int col = LW * wv + j;
             if (act && col < kb) tb_at(AB, B, i, c0 + col) = r[j];
         }
     } else {
         const double *L = Pl;
         const int ncol = min(TB, r1 - base);
-        double tv[16];
+        constexpr int NU = TB * TB / NT;
+        double tv[NU];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int e = threadIdx.x + 256 * u, row = e & (TB - 1), col = e >> 6;   // down a column
+        for (int u = 0; u < NU; u++) {
+            const int e = threadIdx.x + NT * u, row = e & (TB - 1), col = e >> 6;   // down a column
             tv[u] = tb_get(AB, B, c0 + min(row, kb - 1), base + min(col, ncol - 1));
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int e = threadIdx.x + 256 * u, row = e & (TB - 1), col = e >> 6;
+        for (int u = 0; u < NU; u++) {
+            const int e = threadIdx.x + NT * u, row = e & (TB - 1), col = e >> 6;
             T[row][col] = (row < kb && col < ncol) ? tv[u] : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 16; j++) r[j] = T[16 * wv + j][lane];
+        for (int j = 0; j < LW; j++) r[j] = T[LW * wv + j][lane];
         __syncthreads();   // the groups below overwrite the transpose buffer
 #pragma unroll
-        for (int P = 0; P < 4; P++) {
+        for (int P = 0; P < NWV; P++) {
             if (wv == P) {
 #pragma unroll
-                for (int pj = 0; pj < 16; pj++) {
-                    const int p = 16 * P + pj;
+                for (int pj = 0; pj < LW; pj++) {
+                    const int p = LW * P + pj;
 #pragma unroll
-                    for (int qj = pj + 1; qj < 16; qj++) r[qj] = fma(-L[TB * p + 16 * P + qj], r[pj], r[qj]);
+                    for (int qj = pj + 1; qj < LW; qj++) r[qj] = fma(-L[TB * p + LW * P + qj], r[pj], r[qj]);
                 }
 #pragma unroll
-                for (int j = 0; j < 16; j++) pan[P & 1][lane][j] = r[j];
+                for (int j = 0; j < LW; j++) pan[P & 1][lane][j] = r[j];
             }
             __syncthreads();
             if (wv > P) {
 #pragma unroll
-                for (int sj = 0; sj < 16; sj++) {
+                for (int sj = 0; sj < LW; sj++) {
                     const double u = pan[P & 1][lane][sj];
 #pragma unroll
-                    for (int j = 0; j < 16; j++) r[j] = fma(-L[TB * (16 * P + sj) + 16 * wv + j], u, r[j]);
+                    for (int j = 0; j < LW; j++) r[j] = fma(-L[TB * (LW * P + sj) + LW * wv + j], u, r[j]);
                 }
             }
         }
         // the last group reads ended before the last barrier above
 #pragma unroll
-        for (int j = 0; j < 16; j++) T[16 * wv + j][lane] = r[j];
+        for (int j = 0; j < LW; j++) T[LW * wv + j][lane] = r[j];
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int e = threadIdx.x + 256 * u, row = e & (TB - 1), col = e >> 6;
+        for (int u = 0; u < NU; u++) {
+            const int e = threadIdx.x + NT * u, row = e & (TB - 1), col = e >> 6;
             if (row < kb && col < ncol) tb_at(AB, B, c0 + row, base + col) = T[row][col];
         }
     }
 }
 
 // Trailing window of panel k: A[r0:r1, r0:r1] -= L[r0:r1, panel] U[panel, r0:r1],
-// one 64 x 64 tile per workgroup (256 threads, a 4 x 4 block each).  The
+// one 64 x 64 tile per workgroup (512 threads, a 2 x 4 block each).  The
 // workgroup of the window's first tile -- panel k + 1's diagonal block, which
 // no later update of panel k touches -- goes on to factor that block
 // (diag_lu_store), while the other tiles are updated: panel k + 1's diagonal
 // LU leaves the critical path of the rank-64 update.
-__global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB, double *panel, int k, int gt,
-                                                 const int *mask) {
+__global__ void __launch_bounds__(NT) k_tb_gemm(const TBand *bands, double *AB, double *panel, int k, int gt,
+                                                const int *mask) {
     __shared__ double sm[TB * TB + TB * (TB + 1)];
     double (*Ls)[TB] = reinterpret_cast<double (*)[TB]>(sm);                 // Ls[p][row]
     double (*Us)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm + TB * TB);   // Us[p][col]
@@ -332,38 +340,41 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
     // keep their values: their products are zero), other tiles in the window
     const int cl = next ? min(B.m, r0 + TB) : r1;
     const int t = threadIdx.x;
-    const int tr = t & 15, tc = t >> 4;   // rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3
-    // this thread's 16 entries of the tile, fetched first (clamped positions)
+    const int tr = t & 31, tc = t >> 5;   // rows 2 tr, 2 tr + 1, columns 4 tc .. 4 tc + 3
+    // this thread's 8 entries of the tile, fetched first (clamped positions)
     // so that their loads overlap the product
-    double cv[4][4];
+    double cv[2][4];
 #pragma unroll
     for (int v = 0; v < 4; v++)
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            cv[u][v] = tb_get(AB, B, min(i0 + 4 * tr + u, cl - 1), min(j0 + 4 * tc + v, cl - 1));
+        for (int u = 0; u < 2; u++)
+            cv[u][v] = tb_get(AB, B, min(i0 + 2 * tr + u, cl - 1), min(j0 + 4 * tc + v, cl - 1));
     // both tiles read down their columns (consecutive addresses across threads)
-    double lv[16], uv[16];   // all loads in flight before the LDS stores
+    constexpr int NU = TB * TB / NT;
+    double lv[NU], uv[NU];   // all loads in flight before the LDS stores
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int e = t + 256 * u, lo = e & (TB - 1), hi = e >> 6;
+    for (int u = 0; u < NU; u++) {
+        const int e = t + NT * u, lo = e & (TB - 1), hi = e >> 6;
         lv[u] = tb_get(AB, B, min(i0 + lo, r1 - 1), c0 + min(hi, kb - 1));
         uv[u] = tb_get(AB, B, c0 + min(lo, kb - 1), min(j0 + hi, r1 - 1));
     }
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int e = t + 256 * u, lo = e & (TB - 1), hi = e >> 6;
+    for (int u = 0; u < NU; u++) {
+        const int e = t + NT * u, lo = e & (TB - 1), hi = e >> 6;
         Ls[hi][lo] = (hi < kb && i0 + lo < r1) ? lv[u] : 0.0;   // row lo, panel column hi
         Us[lo][hi] = (lo < kb && j0 + hi < r1) ? uv[u] : 0.0;   // panel row lo, column hi
     }
     __syncthreads();
-    double acc[4][4] = {};
+    double acc[2][4] = {};
 #pragma unroll 8
     for (int p = 0; p < kb; p++) {
-        double a[4], b[4];
+        double a[2], b[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) { a[u] = Ls[p][4 * tr + u]; b[u] = Us[p][4 * tc + u]; }
+        for (int u = 0; u < 2; u++) a[u] = Ls[p][2 * tr + u];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int v = 0; v < 4; v++) b[v] = Us[p][4 * tc + v];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
 #pragma unroll
             for (int v = 0; v < 4; v++) acc[u][v] = fma(a[u], b[v], acc[u][v]);
     }
@@ -372,20 +383,20 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
         double (*D)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(sm);
         __syncthreads();   // the product's LDS reads are done
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < 2; u++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) D[4 * tr + u][4 * tc + v] = cv[u][v] - acc[u][v];
+            for (int v = 0; v < 4; v++) D[2 * tr + u][4 * tc + v] = cv[u][v] - acc[u][v];
         __syncthreads();
         const int kb1 = min(TB, B.m - r0);
         const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
-        double r[16];
+        double r[LW];
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int col = 16 * wv + j;
+        for (int j = 0; j < LW; j++) {
+            const int col = LW * wv + j;
             r[j] = (lane < kb1 && col < kb1) ? D[lane][col] : 0.0;
         }
-        double (*pan)[TB][17] = reinterpret_cast<double (*)[TB][17]>(sm + TB * (TB + 1));
-        double *rd = sm + TB * (TB + 1) + 2 * TB * 17;
+        double (*pan)[TB][LW + 1] = reinterpret_cast<double (*)[TB][LW + 1]>(sm + TB * (TB + 1));
+        double *rd = sm + TB * (TB + 1) + 2 * TB * (LW + 1);
         diag_lu_store(r, pan, rd, AB, B, panel + (size_t)TB_PANEL * blockIdx.y, r0, kb1, lane, wv);
         return;
     }
@@ -394,8 +405,8 @@ __global__ void __launch_bounds__(256) k_tb_gemm(const TBand *bands, double *AB,
         const int j = j0 + 4 * tc + v;
         if (j >= r1) continue;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int i = i0 + 4 * tr + u;
+        for (int u = 0; u < 2; u++) {
+            const int i = i0 + 2 * tr + u;
             if (i < r1) tb_at(AB, B, i, j) = cv[u][v] - acc[u][v];
         }
     }
@@ -709,12 +720,12 @@ void launch_tband_setup(Ctx *c) {
     hipLaunchKernelGGL(k_tb_load, dim3(nblocks_for(c->N, 256)), dim3(256), 0, c->stream, c->N, c->node_blk,
                        c->tb_bands, c->tb_AB, c->diag, c->offd, c->row_ptr, c->sell_ptr, c->sell_col, mask);
     const int gt = std::max(1, c->tb_gt);
-    hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
+    hipLaunchKernelGGL(k_tb_diag, dim3(c->tb_nband), dim3(NT), 0, c->stream, c->tb_bands, c->tb_AB, c->tb_panel,
                        0, mask);
     for (int k = 0; k < nt; k++) {
-        hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
+        hipLaunchKernelGGL(k_tb_trsm, dim3(2 * gt, c->tb_nband), dim3(NT), 0, c->stream, c->tb_bands, c->tb_AB,
                            c->tb_panel, k, gt, mask);
-        hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(256), 0, c->stream, c->tb_bands, c->tb_AB,
+        hipLaunchKernelGGL(k_tb_gemm, dim3(gt * gt, c->tb_nband), dim3(NT), 0, c->stream, c->tb_bands, c->tb_AB,
                            c->tb_panel, k, gt, mask);
     }
     hipLaunchKernelGGL(k_tb_inv, dim3(nt, c->tb_nband), dim3(64), 0, c->stream, c->tb_bands, c->tb_AB,
