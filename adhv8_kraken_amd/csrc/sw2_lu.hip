@@ -5,17 +5,19 @@
 // no pivoting; a multiplier is its entry over the pivot, or over SMALL when
 // |pivot| <= SMALL (solv_blk_profile.c:66-71; formed here as the entry times
 // the reciprocal, and the updates as fused multiply-adds), and the backward
-// substitution divides by the pivot itself (solv_blk_solve.c:50-52).  Entries outside a
-// row's profile are zero and stay zero (no fill outside the envelope without
-// pivoting), so factoring the band that covers every row's profile gives the
-// skyline factors; only the rounding order of the updates differs.
+// substitution divides by the pivot itself (solv_blk_solve.c:50-52).  Entries
+// outside a row's profile are zero and stay zero (no fill outside the envelope
+// without pivoting), so factoring the band that covers every row's profile
+// gives the skyline factors; only the rounding order of the updates differs.
 //
 // Storage: each band (one node block) in LAPACK band layout, column-major,
 // half bandwidth wp = w + TB (w: the pattern's), entry (i, j) at
-// off + (wp + i - j) + j (2 wp + 1).  The TB extra diagonals hold zeros and
-// let a panel's L block (rows up to TB + w - 1 below its first column) and U
-// block be addressed without bounds tests.  Seen with leading dimension 2 wp
-// the band is a column-major dense matrix, so tiles are strided blocks.
+// off + (wp + i - j) + j (2 wp + 1).  The TB extra diagonals hold zeros during
+// the factorisation, so a panel's L block (rows up to TB + w - 1 below its
+// first column) and U block are addressed without bounds tests, and after it
+// the fill of the off-diagonal tiles scaled by k_tb_scale.  Seen with leading
+// dimension 2 wp the band is a column-major dense matrix, so tiles are
+// strided blocks.
 //
 // Factorisation, per panel of TB columns (two launches over all bands at once:
 // a model's node blocks, or every block of every model of an ensemble):
