@@ -463,8 +463,8 @@ __global__ void __launch_bounds__(64) k_tb_inv(const TBand *bands, const double 
 // step's off-diagonal update reads one tile and no inverse: x_i -= (L_ik LI_k)
 // x_k = L_ik y_k, y_i -= (U_ik UI_k) y_k = U_ik z_k.  The products fill at most
 // the band's TB extra diagonals (entries within w + 63 of the diagonal), which
-// is what they are stored for.  One workgroup per tile, 4 x 4 register blocks
-// from LDS as in k_tb_gemm.
+// is what they are stored for.  One workgroup of 256 threads per tile, a 4 x 4
+// register block each from LDS.
 __global__ void __launch_bounds__(256) k_tb_scale(const TBand *bands, double *AB, const double *LI, const double *UI,
                                                   int gf, const int *mask) {
     __shared__ double As[TB][TB];       // As[q][row]: the tile, column q
