@@ -109,7 +109,8 @@ typedef struct {
                             reference; single models, precision 0 or 1) */
     int lu_tiled;        /* preconditioner 1: 0 (default) factors node blocks of up to 2,730 nodes with one
                             workgroup each and larger ones (up to OP BLK 1, the whole matrix) with the tiled
-                            band LU; 1 uses the tiled band LU for every block */
+                            band LU; 1 uses the tiled band LU for every block (an ensemble handle
+                            always does) */
 } adh_sw2_opts;
 
 typedef struct {
